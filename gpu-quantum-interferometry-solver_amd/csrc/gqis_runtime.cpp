@@ -1,0 +1,293 @@
+// gqis_runtime.cpp -- C ABI of the sweep integrator: hiprtc compile cache, module launch,
+// events and the kernel status word.  Declarations and reference citations: include/gqis_hip.h.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "gqis_hip.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t err, const char* what) {
+    return fail(GQ_E_HIP, std::string(what) + ": " + hipGetErrorString(err));
+}
+
+// FNV-1a 64-bit over the source, options and architecture: the module cache key.
+unsigned long long fnv1a(const std::string& text, unsigned long long h = 1469598103934665603ull) {
+    for (unsigned char c : text) {
+        h ^= c;
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+struct Compiled {
+    std::vector<char> code;
+    std::string log;
+    int rc = GQ_OK;
+};
+
+Compiled hiprtc_compile(const char* source, const std::string& arch, const char* const* options,
+                        int num_options) {
+    Compiled out;
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, source, "gqis_kernel.hip", 0, nullptr, nullptr) !=
+        HIPRTC_SUCCESS) {
+        out.rc = GQ_E_COMPILE;
+        out.log = "hiprtcCreateProgram failed";
+        return out;
+    }
+    std::vector<std::string> opts;
+    opts.push_back("--offload-arch=" + arch);
+    for (int i = 0; i < num_options; ++i) opts.emplace_back(options[i]);
+    std::vector<const char*> argv;
+    for (auto& o : opts) argv.push_back(o.c_str());
+    hiprtcResult r = hiprtcCompileProgram(prog, (int)argv.size(), argv.data());
+    size_t log_size = 0;
+    hiprtcGetProgramLogSize(prog, &log_size);
+    if (log_size > 1) {
+        out.log.resize(log_size);
+        hiprtcGetProgramLog(prog, &out.log[0]);
+    }
+    if (r != HIPRTC_SUCCESS) {
+        out.rc = GQ_E_COMPILE;
+        out.log = std::string("hiprtc: ") + hiprtcGetErrorString(r) + "\n" + out.log;
+        hiprtcDestroyProgram(&prog);
+        return out;
+    }
+    size_t code_size = 0;
+    hiprtcGetCodeSize(prog, &code_size);
+    out.code.resize(code_size);
+    hiprtcGetCode(prog, out.code.data());
+    hiprtcDestroyProgram(&prog);
+    return out;
+}
+
+}  // namespace
+
+struct gq_module {
+    gq_ctx* ctx = nullptr;
+    hipModule_t module = nullptr;
+    std::map<std::string, hipFunction_t> functions;
+    std::mutex mu;
+    int refs = 1;
+};
+
+struct gq_ctx {
+    int device = 0;
+    std::string arch;
+    int compute_units = 0;
+    std::mutex mu;
+    std::map<unsigned long long, gq_module*> modules;
+};
+
+extern "C" {
+
+const char* gq_last_error(void) { return g_error.c_str(); }
+
+unsigned long long gq_sweep_args_size(void) { return sizeof(gq_sweep_args); }
+
+int gq_ctx_create(int device_ordinal, gq_ctx** out) {
+    if (!out) return fail(GQ_E_INVALID, "gq_ctx_create: null output pointer");
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceCount");
+    if (device_ordinal < 0 || device_ordinal >= count)
+        return fail(GQ_E_INVALID, "gq_ctx_create: device ordinal out of range");
+    e = hipSetDevice(device_ordinal);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, device_ordinal);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+    gq_ctx* ctx = new gq_ctx();
+    ctx->device = device_ordinal;
+    std::string arch = prop.gcnArchName;
+    ctx->arch = arch.substr(0, arch.find(':'));   // "gfx950:sramecc+:xnack-" -> "gfx950"
+    ctx->compute_units = prop.multiProcessorCount;
+    *out = ctx;
+    return GQ_OK;
+}
+
+void gq_ctx_destroy(gq_ctx* ctx) {
+    if (!ctx) return;
+    for (auto& kv : ctx->modules) {
+        if (kv.second->module) (void)hipModuleUnload(kv.second->module);
+        delete kv.second;
+    }
+    delete ctx;
+}
+
+int gq_ctx_info(gq_ctx* ctx, char* arch, int arch_len, int* compute_units) {
+    if (!ctx) return fail(GQ_E_INVALID, "gq_ctx_info: null context");
+    if (arch && arch_len > 0) {
+        std::snprintf(arch, (size_t)arch_len, "%s", ctx->arch.c_str());
+    }
+    if (compute_units) *compute_units = ctx->compute_units;
+    return GQ_OK;
+}
+
+int gq_compile_check(const char* source, const char* arch, const char* const* options,
+                     int num_options, long long* code_size) {
+    if (!source || !arch) return fail(GQ_E_INVALID, "gq_compile_check: null source or arch");
+    Compiled c = hiprtc_compile(source, arch, options, num_options);
+    if (c.rc != GQ_OK) return fail(c.rc, c.log);
+    if (code_size) *code_size = (long long)c.code.size();
+    return GQ_OK;
+}
+
+int gq_module_compile(gq_ctx* ctx, const char* source, const char* const* options, int num_options,
+                      gq_module** out, int* cached_out, double* build_seconds) {
+    if (!ctx || !source || !out) return fail(GQ_E_INVALID, "gq_module_compile: null argument");
+    auto t0 = std::chrono::steady_clock::now();
+    std::string key_text = ctx->arch;
+    key_text.push_back('\0');
+    for (int i = 0; i < num_options; ++i) {
+        key_text += options[i];
+        key_text.push_back('\0');
+    }
+    unsigned long long key = fnv1a(source, fnv1a(key_text));
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        auto it = ctx->modules.find(key);
+        if (it != ctx->modules.end()) {
+            it->second->refs++;
+            *out = it->second;
+            if (cached_out) *cached_out = 1;
+            if (build_seconds) *build_seconds = 0.0;
+            return GQ_OK;
+        }
+    }
+    Compiled c = hiprtc_compile(source, ctx->arch, options, num_options);
+    if (c.rc != GQ_OK) return fail(c.rc, c.log);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    hipModule_t mod = nullptr;
+    e = hipModuleLoadData(&mod, c.code.data());
+    if (e != hipSuccess) return hip_fail(e, "hipModuleLoadData");
+    gq_module* m = new gq_module();
+    m->ctx = ctx;
+    m->module = mod;
+    m->refs = 2;   // one for the cache, one for the caller
+    {
+        std::lock_guard<std::mutex> lock(ctx->mu);
+        ctx->modules[key] = m;
+    }
+    *out = m;
+    if (cached_out) *cached_out = 0;
+    if (build_seconds) {
+        *build_seconds =
+            std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    return GQ_OK;
+}
+
+static int get_function(gq_module* m, const char* name, hipFunction_t* fn) {
+    std::lock_guard<std::mutex> lock(m->mu);
+    auto it = m->functions.find(name);
+    if (it != m->functions.end()) {
+        *fn = it->second;
+        return GQ_OK;
+    }
+    hipError_t e = hipModuleGetFunction(fn, m->module, name);
+    if (e != hipSuccess)
+        return fail(GQ_E_NOT_FOUND, std::string("kernel not found in module: ") + name);
+    m->functions[name] = *fn;
+    return GQ_OK;
+}
+
+int gq_launch_sweep(gq_module* module, const char* name, unsigned int grid, unsigned int block,
+                    const gq_sweep_args* args, void* stream) {
+    if (!module || !name || !args) return fail(GQ_E_INVALID, "gq_launch_sweep: null argument");
+    if (grid == 0 || block == 0 || block > 1024)
+        return fail(GQ_E_INVALID, "gq_launch_sweep: bad launch shape");
+    hipFunction_t fn;
+    int rc = get_function(module, name, &fn);
+    if (rc != GQ_OK) return rc;
+    gq_sweep_args local = *args;
+    void* params[] = {&local};
+    hipError_t e = hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, 0, (hipStream_t)stream,
+                                         params, nullptr);
+    if (e != hipSuccess) return hip_fail(e, "hipModuleLaunchKernel");
+    return GQ_OK;
+}
+
+int gq_resident_groups(gq_module* module, const char* name, unsigned int block, int* groups) {
+    if (!module || !name || !groups) return fail(GQ_E_INVALID, "gq_resident_groups: null argument");
+    hipFunction_t fn;
+    int rc = get_function(module, name, &fn);
+    if (rc != GQ_OK) return rc;
+    int per_cu = 0;
+    hipError_t e = hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, (int)block, 0);
+    if (e != hipSuccess) return hip_fail(e, "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
+    *groups = per_cu * module->ctx->compute_units;
+    return GQ_OK;
+}
+
+void gq_module_release(gq_module* module) {
+    if (!module) return;
+    std::lock_guard<std::mutex> lock(module->ctx->mu);
+    if (module->refs > 1) module->refs--;   // the cache keeps its reference until ctx_destroy
+}
+
+int gq_event_create(void** event) {
+    if (!event) return fail(GQ_E_INVALID, "gq_event_create: null output");
+    hipEvent_t ev;
+    hipError_t e = hipEventCreate(&ev);
+    if (e != hipSuccess) return hip_fail(e, "hipEventCreate");
+    *event = (void*)ev;
+    return GQ_OK;
+}
+
+int gq_event_record(void* event, void* stream) {
+    hipError_t e = hipEventRecord((hipEvent_t)event, (hipStream_t)stream);
+    return e == hipSuccess ? GQ_OK : hip_fail(e, "hipEventRecord");
+}
+
+int gq_event_elapsed_ms(void* start, void* stop, float* ms) {
+    hipError_t e = hipEventSynchronize((hipEvent_t)stop);
+    if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize");
+    e = hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop);
+    return e == hipSuccess ? GQ_OK : hip_fail(e, "hipEventElapsedTime");
+}
+
+int gq_event_destroy(void* event) {
+    hipError_t e = hipEventDestroy((hipEvent_t)event);
+    return e == hipSuccess ? GQ_OK : hip_fail(e, "hipEventDestroy");
+}
+
+int gq_stream_synchronize(void* stream) {
+    hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+    return e == hipSuccess ? GQ_OK : hip_fail(e, "hipStreamSynchronize");
+}
+
+int gq_status_read(unsigned int* device_status, void* stream, unsigned int* flags, int clear) {
+    if (!device_status || !flags) return fail(GQ_E_INVALID, "gq_status_read: null argument");
+    unsigned int host = 0;
+    hipError_t e = hipMemcpyAsync(&host, device_status, sizeof(host), hipMemcpyDeviceToHost,
+                                  (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(status)");
+    e = hipStreamSynchronize((hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    *flags = host;
+    if (clear) {
+        e = hipMemsetAsync(device_status, 0, sizeof(unsigned int), (hipStream_t)stream);
+        if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync(status)");
+    }
+    return GQ_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,252 @@
+"""HIP source emission for gfx950 (hiprtc), from an ``RHSPlan``.
+
+The generated source is: the ABI header (argument structs), a precision prelude, the model
+functions printed from the plan (statics, time atoms, stage values, RHS, observable, initial
+state) and one hand-written solver template from ``kernels/``.  This replaces the reference's
+CUDA template assembly (``gqis/solver.py:2073-2233``, ``gqis/N_Level_Kernel.cu``); the
+reference's FP64 mode is a textual promotion of its FP32 kernel (``gqis/solver.py:2219-2232``),
+here the precision is a typedef.
+"""
+import os
+import re
+
+import sympy as sp
+from sympy.printing.c import C99CodePrinter
+from sympy.printing.precedence import precedence
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+KERNEL_DIR = os.path.join(_PKG, "kernels")
+
+
+def _header_path():
+    """The C-ABI header (include/gqis_hip.h): one struct definition for host and device."""
+    cands = [os.path.join(_PKG, "..", "..", "include", "gqis_hip.h"),
+             os.path.join(_PKG, "kernels", "gqis_hip.h")]
+    for c in cands:
+        if os.path.exists(c):
+            return os.path.abspath(c)
+    raise FileNotFoundError("gqis_hip.h not found next to the package")
+
+
+def read_kernel_template(name):
+    with open(os.path.join(KERNEL_DIR, name)) as fh:
+        return fh.read()
+
+
+class HipPrinter(C99CodePrinter):
+    """C++ printer for device code: literals as ``R(x)`` (the kernel's real type), integer
+    powers as products, symbol names mapped to kernel variables."""
+
+    def __init__(self, names, literal="R", settings=None):
+        super().__init__(settings or {})
+        self._names = names
+        self._lit = literal
+
+    def _lit_str(self, value):
+        return f"{self._lit}({float(value)!r})"
+
+    def _print_Symbol(self, expr):
+        name = self._names.get(expr.name)
+        if name is None:
+            raise ValueError(f"symbol {expr.name!r} has no kernel binding")
+        return name
+
+    def _print_Float(self, expr):
+        return self._lit_str(expr)
+
+    def _print_Integer(self, expr):
+        return self._lit_str(int(expr))
+
+    def _print_Rational(self, expr):
+        return self._lit_str(expr.p / expr.q)
+
+    def _print_NumberSymbol(self, expr):
+        return self._lit_str(expr.evalf(20))
+
+    def _print_Exp1(self, expr):
+        return self._lit_str(sp.E.evalf(20))
+
+    def _print_Pi(self, expr):
+        return self._lit_str(sp.pi.evalf(20))
+
+    def _print_ImaginaryUnit(self, expr):
+        raise ValueError("complex values cannot be printed into the real kernel")
+
+    def _print_Pow(self, expr):
+        base, exp = expr.base, expr.exp
+        b = self.parenthesize(base, precedence(expr))
+        if exp.is_Integer:
+            n = int(exp)
+            if n == 0:
+                return self._lit_str(1)
+            if 0 < n <= 8:
+                return "(" + "*".join([b] * n) + ")"
+            if -8 <= n < 0:
+                return f"({self._lit}(1.0)/(" + "*".join([b] * (-n)) + "))"
+        if exp == sp.Rational(1, 2):
+            return f"sqrt({self._print(base)})"
+        if exp == -sp.Rational(1, 2):
+            return f"({self._lit}(1.0)/sqrt({self._print(base)}))"
+        return f"pow({self._print(base)}, {self._print(exp)})"
+
+    def _print_Piecewise(self, expr):
+        out = None
+        for value, cond in reversed(expr.args):
+            v = self._print(value)
+            if cond == True:  # noqa: E712 (SymPy true)
+                out = v
+            else:
+                out = f"(({self._print(cond)}) ? ({v}) : ({out if out is not None else 'R(0.0)'}))"
+        return out
+
+    def _print_re(self, expr):
+        raise ValueError("re() left in a real kernel expression")
+
+    def _print_im(self, expr):
+        raise ValueError("im() left in a real kernel expression")
+
+    # math functions: the C++ overloads pick float or double by argument type
+    def _print_Function(self, expr):
+        name = expr.func.__name__
+        known = {"sin", "cos", "tan", "exp", "log", "sinh", "cosh", "tanh", "asin", "acos",
+                 "atan", "atan2", "Abs", "sign", "floor", "ceiling", "Min", "Max"}
+        if name not in known:
+            return super()._print_Function(expr)
+        args = ", ".join(self._print(a) for a in expr.args)
+        mapping = {"Abs": "fabs", "ceiling": "ceil", "Min": "fmin", "Max": "fmax",
+                   "sign": "gq_sign"}
+        return f"{mapping.get(name, name)}({args})"
+
+    _print_sin = _print_cos = _print_tan = _print_exp = _print_log = _print_Function
+    _print_sinh = _print_cosh = _print_tanh = _print_asin = _print_acos = _print_Function
+    _print_atan = _print_atan2 = _print_Abs = _print_sign = _print_floor = _print_Function
+    _print_ceiling = _print_Min = _print_Max = _print_Function
+
+
+def _names_for(plan):
+    """Kernel variable names of every symbol the plan can use."""
+    names = {"ParX": "px", "ParY": "py", "t": "t_now"}
+    for i, s in enumerate(plan.states):
+        names[s.name] = f"y[{i}]"
+    for i, s in enumerate(plan.const_syms):
+        names[s.name] = f"C[{i}]"
+    for sym, _ in plan.statics:
+        names[sym.name] = f"S.{sym.name}"
+    for i, (sym, _) in enumerate(plan.time_atoms):
+        names[sym.name] = f"TT[{i}]"
+    for sym, _ in plan.stage:
+        names[sym.name] = f"G.{sym.name}"
+    for sym, _ in plan.rhs_temps:
+        names[sym.name] = sym.name
+    for sym, _ in plan.obs_temps:
+        names[sym.name] = sym.name
+    return names
+
+
+def emit_model(plan, rho0_exprs=None):
+    """Device functions of one model (the part the reference prints from SymPy,
+    ``gqis/solver.py:633-700,1649-1692``)."""
+    names = _names_for(plan)
+    pr = HipPrinter(names)
+    lines = []
+    n_static = len(plan.statics)
+    n_stage = len(plan.stage)
+    n_atoms = len(plan.time_atoms)
+    lines.append(f"#define GQ_D {plan.state_size}")
+    lines.append(f"#define GQ_NATOMS {n_atoms}")
+    lines.append(f"#define GQ_NATOMS_ALLOC {max(1, n_atoms)}")
+    lines.append("struct gq_static_t {")
+    lines += [f"    real {s.name};" for s, _ in plan.statics] or ["    real unused_;"]
+    lines.append("};")
+    lines.append("struct gq_stage_t {")
+    lines += [f"    real {s.name};" for s, _ in plan.stage] or ["    real unused_;"]
+    lines.append("};")
+    # statics: once per parameter point
+    lines.append("__device__ __forceinline__ void gq_statics(const real px, const real py, "
+                 "const real* __restrict__ C, gq_static_t& S) {")
+    lines.append("    (void)px; (void)py; (void)C;")
+    lines += [f"    S.{s.name} = {pr.doprint(e)};" for s, e in plan.statics]
+    lines.append("}")
+    # time atoms in FP64 (tables, adaptive FP64 time) and in the kernel's real type
+    for fname, rtype, lit in (("gq_atoms_f64", "double", "DBL"), ("gq_atoms_real", "real", "R")):
+        anames = dict(names)
+        anames["t"] = "t_now"
+        for i, s in enumerate(plan.const_syms):
+            anames[s.name] = f"({rtype})C[{i}]"
+        apr = HipPrinter(anames, literal=lit)
+        lines.append(f"__device__ __forceinline__ void {fname}(const {rtype} t_now, "
+                     f"const real* __restrict__ C, {rtype}* __restrict__ TT) {{")
+        lines.append("    (void)t_now; (void)C; (void)TT;")
+        lines += [f"    TT[{i}] = {apr.doprint(e)};" for i, (_, e) in enumerate(plan.time_atoms)]
+        lines.append("}")
+    # stage values: once per distinct stage time
+    lines.append("__device__ __forceinline__ void gq_stage(const real px, const real py, "
+                 "const real* __restrict__ C, const gq_static_t& S, const real* __restrict__ TT, "
+                 "gq_stage_t& G) {")
+    lines.append("    (void)px; (void)py; (void)C; (void)S; (void)TT;")
+    lines += [f"    G.{s.name} = {pr.doprint(e)};" for s, e in plan.stage]
+    lines.append("}")
+    # right-hand side
+    lines.append("__device__ __forceinline__ void gq_rhs(const real* __restrict__ y, const real px, "
+                 "const real py, const real* __restrict__ C, const gq_static_t& S, "
+                 "const gq_stage_t& G, real* __restrict__ dy) {")
+    lines.append("    (void)px; (void)py; (void)C; (void)S; (void)G;")
+    lines += [f"    const real {s.name} = {pr.doprint(e)};" for s, e in plan.rhs_temps]
+    lines += [f"    dy[{i}] = {pr.doprint(e)};" for i, e in enumerate(plan.rhs_out)]
+    lines.append("}")
+    # observable
+    lines.append("__device__ __forceinline__ void gq_obs(const real* __restrict__ y, const real px, "
+                 "const real py, const real* __restrict__ C, const gq_static_t& S, "
+                 "const gq_stage_t& G, real& o_re, real& o_im) {")
+    lines.append("    (void)y; (void)px; (void)py; (void)C; (void)S; (void)G;")
+    lines += [f"    const real {s.name} = {pr.doprint(e)};" for s, e in plan.obs_temps]
+    lines.append(f"    o_re = {pr.doprint(plan.obs_out[0])};")
+    lines.append(f"    o_im = {pr.doprint(plan.obs_out[1])};")
+    lines.append("}")
+    # initial state from expressions (None: |0><0| or zero for ODEs handled by the template)
+    lines.append("__device__ __forceinline__ void gq_init_expr(real* __restrict__ y, const real px, "
+                 "const real py, const real* __restrict__ C) {")
+    lines.append("    (void)px; (void)py; (void)C;")
+    if rho0_exprs is None:
+        lines.append("    for (int i = 0; i < GQ_D; ++i) y[i] = R(0.0);")
+        lines.append("    y[0] = R(1.0);")
+    else:
+        ipr = HipPrinter(names)
+        lines += [f"    y[{i}] = {ipr.doprint(sp.sympify(e))};" for i, e in enumerate(rho0_exprs)]
+    lines.append("}")
+    return "\n".join(lines)
+
+
+def prelude(fp64, acc64, block_size, defines):
+    real = "double" if fp64 else "float"
+    acc = "double" if (fp64 or acc64) else "float"
+    out = [f"typedef {real} real;",
+           f"typedef {acc} acc_t;",
+           f"typedef {real}2 real2;",
+           "#define R(x) ((real)(x))",
+           "#define DBL(x) ((double)(x))",
+           f"#define GQ_FP64 {1 if fp64 else 0}",
+           f"#define GQ_BLOCK {int(block_size)}",
+           "__device__ __forceinline__ real gq_sign(real x) { return (x > R(0)) - (x < R(0)); }"]
+    for k, v in defines.items():
+        out.append(f"#define {k} {v}")
+    return "\n".join(out)
+
+
+def assemble(plan, templates, *, fp64, acc64=True, block_size=256, defines=None,
+             rho0_exprs=None):
+    """Full hiprtc source: ABI header + prelude + model + solver templates."""
+    with open(_header_path()) as fh:
+        header = fh.read()
+    parts = [header, prelude(fp64, acc64, block_size, defines or {}), emit_model(plan, rho0_exprs)]
+    parts += [read_kernel_template(t) for t in (["common.hip"] + list(templates))]
+    return "\n".join(parts)
+
+
+HIPRTC_BASE_OPTIONS = ("-O3", "-std=c++17", "-ffp-contract=fast",
+                       # packed FP32 instructions are off for every kernel of this library
+                       "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops")
+
+
+def strip_comments(src):
+    return re.sub(r"//[^\n]*", "", src)

@@ -1,0 +1,135 @@
+"""Device side of one sweep call: buffers (torch), hiprtc module, launches, status, layouts.
+
+Replaces the reference's CuPy upload / launch / copy-back block (``gqis/solver.py:2268-2362``).
+Outputs are written by the kernels in coalesced point-minor layouts and returned in the
+reference's shapes: ``(nx, ny)`` complex, ``(nx, ny, D)`` real, traces ``(nx, ny, n[, D])``.
+"""
+import time
+
+import numpy as np
+
+from . import codegen, runtime
+
+
+def _axis(values, dtype, device):
+    import torch
+    if isinstance(values, torch.Tensor):
+        return values.to(device=device, dtype=dtype).reshape(-1).contiguous()
+    return torch.as_tensor(np.asarray(values, dtype=np.float64).reshape(-1), dtype=dtype,
+                           device=device)
+
+
+def _initial_states(rho0_values, nx, ny, size, dtype, device):
+    import torch
+    if isinstance(rho0_values, torch.Tensor):
+        r = rho0_values.to(device=device, dtype=dtype)
+    else:
+        arr = np.asarray(rho0_values)
+        if np.iscomplexobj(arr):
+            raise ValueError("rho0_values must be real.")
+        r = torch.as_tensor(arr.astype(np.float64), dtype=dtype, device=device)
+    if tuple(r.shape) == (nx, size) and ny == 1:
+        r = r.reshape(nx, 1, size)
+    if tuple(r.shape) != (nx, ny, size):
+        raise ValueError(f"rho0_values must have shape ({nx}, {ny}, {size}) "
+                         f"or ({nx}, {size}) when num_Y=1; got {tuple(r.shape)}.")
+    return r.reshape(nx * ny, size).contiguous()
+
+
+def run_sweep(*, source, variant, spec, compiled, mode, var_arrays, rho0_values, const_vals,
+              fp64, dt, t_in, num_steps, warmup_steps, num_trace, trace_stride, trace_offset,
+              state_trace, time_table, block_size, options, reuse, collect):
+    import torch
+    setup_start = time.time()
+    ctx = runtime.Context.get()
+    dev = torch.device("cuda", ctx.device)
+    rdt = torch.float64 if fp64 else torch.float32
+    cdt = torch.complex128 if fp64 else torch.complex64
+    axes = list(var_arrays.values())
+    px = _axis(axes[0], rdt, dev)
+    py = _axis(axes[1], rdt, dev) if len(axes) > 1 else torch.zeros(1, dtype=rdt, device=dev)
+    nx, ny = int(px.numel()), int(py.numel())
+    npts = nx * ny
+    size = compiled.state_size
+
+    module, cached, build_s = ctx.compile(source, codegen.HIPRTC_BASE_OPTIONS + tuple(options))
+    rho0 = (_initial_states(rho0_values, nx, ny, size, rdt, dev)
+            if rho0_values is not None else None)
+    consts = torch.as_tensor(np.asarray(const_vals if const_vals else [0.0], np.float64),
+                             dtype=rdt, device=dev)
+    if mode in ("final_rho", "mean_state"):
+        result = torch.empty((size, max(npts, 1)), dtype=rdt, device=dev)
+    else:
+        result = torch.empty(max(npts, 1), dtype=cdt, device=dev)
+    trace = None
+    if num_trace:
+        trace = (torch.empty((num_trace, size, npts), dtype=rdt, device=dev) if state_trace
+                 else torch.empty((num_trace, npts), dtype=cdt, device=dev))
+    status = torch.zeros(1, dtype=torch.int32, device=dev)
+    cursor = torch.zeros(1, dtype=torch.int32, device=dev)
+    natoms = len(compiled.plan.time_atoms)
+    table = None
+    use_table = bool(time_table) and not spec.adaptive and natoms > 0
+    rows = spec.stage_times_per_step * num_steps + 1
+    if use_table:
+        table = torch.empty(rows * natoms, dtype=rdt, device=dev)
+    steps_out = (torch.zeros((npts, 2), dtype=torch.int32, device=dev)
+                 if spec.adaptive and spec.options.get("count_steps") else None)
+
+    a = runtime.SweepArgs()
+    a.dt = float(dt)
+    a.t_in = float(t_in)
+    a.num_x, a.num_y = nx, ny
+    a.point_begin, a.point_count = 0, npts
+    a.num_steps, a.warmup_steps = int(num_steps), int(warmup_steps)
+    a.trace_stride, a.trace_offset, a.num_trace = int(trace_stride), int(trace_offset), int(num_trace)
+    a.trace_point_stride = npts
+    a.par_x, a.par_y, a.consts = px.data_ptr(), py.data_ptr(), consts.data_ptr()
+    a.rho0 = rho0.data_ptr() if rho0 is not None else None
+    a.time_table = table.data_ptr() if table is not None else None
+    for i, v in enumerate(spec.parameters[:runtime.MAX_SOLVER_PARAMS]):
+        a.solver_param[i] = float(v)
+    a.result = result.data_ptr()
+    a.trace = trace.data_ptr() if trace is not None else None
+    a.adaptive_steps = steps_out.data_ptr() if steps_out is not None else None
+    a.status = status.data_ptr()
+    a.work_cursor = cursor.data_ptr()
+
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    timer = runtime.EventTimer(ctx, stream) if collect else None
+    setup_s = time.time() - setup_start
+    if npts > 0 and num_steps >= 0:
+        if timer:
+            timer.record_start()
+        if use_table:
+            ctx.launch(module, table_kernel(spec), (rows + 255) // 256, 256, a, stream)
+        if spec.adaptive and spec.options.get("persistent", True):
+            grid = min(ctx.resident_groups(module, spec.kernel, block_size),
+                       (npts + block_size - 1) // block_size)
+        else:
+            grid = (npts + block_size - 1) // block_size
+        ctx.launch(module, spec.kernel, max(grid, 1), block_size, a, stream)
+        if timer:
+            timer.record_stop()
+    flags = ctx.read_status(status, stream)
+    gpu_s = timer.elapsed_ms() / 1000.0 if timer else 0.0
+    if timer:
+        timer.close()
+
+    if mode in ("final_rho", "mean_state"):
+        out = result[:, :npts].t().reshape(nx, ny, size).contiguous()
+    else:
+        out = result[:npts].reshape(nx, ny)
+    tr = None
+    if trace is not None:
+        if state_trace:
+            tr = trace.permute(2, 0, 1).reshape(nx, ny, num_trace, size).contiguous()
+        else:
+            tr = trace.t().reshape(nx, ny, num_trace).contiguous()
+    steps = steps_out.reshape(nx, ny, 2).cpu().numpy() if steps_out is not None else None
+    return {"result": out, "trace": tr, "status": flags, "gpu_s": gpu_s, "build_s": build_s,
+            "cached_kernel": cached, "setup_s": setup_s, "adaptive_steps": steps}
+
+
+def table_kernel(spec):
+    return "gq_rk4_time_table" if spec.template == "fixed_rk4.hip" else "gq_fixed_time_table"

@@ -1,0 +1,187 @@
+// Classical RK4 over a uniform time grid, one parameter point per lane, the reduced state in
+// registers.  Semantics of the reference's RK4 body (gqis/cuda_solvers.py:470-517): stage times
+// from the integer step index (t_mid = (step + 1/2) dt, t_end = t_mid + dt/2), k1 of a step is
+// the RHS at the end of the previous one, the mean is a right-endpoint average over the steps
+// with index >= warmup (gqis/solver.py:2144-2152), traces are stored after steps
+// offset + k*stride (gqis/solver.py:2177-2193).
+//
+// Time atoms (the time-only parts of the drives) come from a table built in FP64 once per
+// launch when GQ_TIME_TABLE is 1: index 0 is t = 0, 2s+1 the midpoint and 2s+2 the end of step
+// s.  With GQ_TIME_TABLE 0 each lane evaluates them in its own precision from a time computed
+// as the reference does.
+
+__device__ __forceinline__ void gq_time_atoms(const gq_sweep_args& a, const real* __restrict__ C,
+                                              int stage_index, real t_local,
+                                              real* __restrict__ TT) {
+#if GQ_TIME_TABLE
+    (void)t_local;
+    const real* __restrict__ tab = (const real*)a.time_table;
+#pragma unroll
+    for (int k = 0; k < GQ_NATOMS; ++k) TT[k] = tab[stage_index * GQ_NATOMS + k];
+#else
+    (void)stage_index;
+    gq_atoms_real(t_local + (real)a.t_in, C, TT);
+#endif
+}
+
+// FP64 table of the time atoms at every RK4 stage time: 2 * num_steps + 1 rows.
+extern "C" __global__ void gq_rk4_time_table(const gq_sweep_args a) {
+    const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (s > 2 * a.num_steps) return;
+    const real* __restrict__ C = (const real*)a.consts;
+    double t_local = 0.0;
+    if (s > 0) {
+        const int step = (s - 1) >> 1;
+#if GQ_FP64
+        // the reference's FP64 arithmetic, bit for bit
+        double t_mid = ((double)step + 0.5) * a.dt;
+        if ((s & 1) == 0) t_mid += a.dt * 0.5;
+        t_local = t_mid;
+#else
+        t_local = ((double)step + 0.5) * a.dt + (((s & 1) == 0) ? 0.5 * a.dt : 0.0);
+#endif
+    }
+    double TT[GQ_NATOMS_ALLOC];
+    gq_atoms_f64(t_local + a.t_in, C, TT);
+    real* __restrict__ tab = (real*)a.time_table;
+#pragma unroll
+    for (int k = 0; k < GQ_NATOMS; ++k) tab[s * GQ_NATOMS + k] = (real)TT[k];
+}
+
+extern "C" __global__ __launch_bounds__(GQ_BLOCK) void gq_rk4(const gq_sweep_args a) {
+    const long long q = (long long)blockIdx.x * GQ_BLOCK + threadIdx.x;
+    if (q >= a.point_count) return;
+    const gq_point pt = gq_locate(a, q);
+    const real px = ((const real*)a.par_x)[pt.ix];
+    const real py = ((const real*)a.par_y)[pt.iy];
+    const real* __restrict__ C = (const real*)a.consts;
+
+    gq_static_t S;
+    gq_statics(px, py, C, S);
+    real y[GQ_D];
+    gq_init_state(y, a, pt, px, py, C);
+
+    const real dt = (real)a.dt;
+    const real dt2 = dt * R(0.5);
+    const real dt6 = dt / R(6.0);
+    const int num_steps = a.num_steps;
+
+    real TT[GQ_NATOMS_ALLOC];
+    gq_stage_t G;
+    gq_time_atoms(a, C, 0, R(0.0), TT);
+    gq_stage(px, py, C, S, TT, G);
+    real k[GQ_D];
+    gq_rhs(y, px, py, C, S, G, k);
+
+#if GQ_OUT_MEAN
+    acc_t avg_re = 0, avg_im = 0;
+#endif
+#if GQ_OUT_MEAN_STATE
+    acc_t savg[GQ_D];
+#pragma unroll
+    for (int i = 0; i < GQ_D; ++i) savg[i] = 0;
+#endif
+#if GQ_TRACE
+    bool trace_bad = false;
+#endif
+
+    for (int step = 0; step < num_steps; ++step) {
+        real t_mid = ((real)step + R(0.5)) * dt;
+        gq_time_atoms(a, C, 2 * step + 1, t_mid, TT);
+        gq_stage(px, py, C, S, TT, G);
+        real yt[GQ_D], acc[GQ_D];
+#pragma unroll
+        for (int i = 0; i < GQ_D; ++i) {
+            yt[i] = fma(dt2, k[i], y[i]);
+            acc[i] = k[i];
+        }
+        gq_rhs(yt, px, py, C, S, G, k);
+#pragma unroll
+        for (int i = 0; i < GQ_D; ++i) {
+            yt[i] = fma(dt2, k[i], y[i]);
+            acc[i] = fma(R(2.0), k[i], acc[i]);
+        }
+        gq_rhs(yt, px, py, C, S, G, k);
+#pragma unroll
+        for (int i = 0; i < GQ_D; ++i) {
+            yt[i] = fma(dt, k[i], y[i]);
+            acc[i] = fma(R(2.0), k[i], acc[i]);
+        }
+        t_mid += dt2;
+        gq_time_atoms(a, C, 2 * step + 2, t_mid, TT);
+        gq_stage(px, py, C, S, TT, G);
+        gq_rhs(yt, px, py, C, S, G, k);
+#pragma unroll
+        for (int i = 0; i < GQ_D; ++i) {
+            acc[i] += k[i];
+            y[i] = fma(dt6, acc[i], y[i]);
+        }
+        if (step + 1 < num_steps) gq_rhs(y, px, py, C, S, G, k);
+
+#if GQ_OUT_MEAN
+        if (step >= a.warmup_steps) {
+            real o_re, o_im;
+            gq_obs(y, px, py, C, S, G, o_re, o_im);
+            avg_re += (acc_t)o_re;
+            avg_im += (acc_t)o_im;
+        }
+#endif
+#if GQ_OUT_MEAN_STATE
+        if (step >= a.warmup_steps) {
+#pragma unroll
+            for (int i = 0; i < GQ_D; ++i) savg[i] += (acc_t)y[i];
+        }
+#endif
+#if GQ_TRACE
+        if (step >= a.trace_offset && (step - a.trace_offset) % a.trace_stride == 0) {
+            const int kk = (step - a.trace_offset) / a.trace_stride;
+            if (kk < a.num_trace) {
+                const long long ld = a.trace_point_stride;
+#if GQ_TRACE == 1
+                real o_re, o_im;
+                gq_obs(y, px, py, C, S, G, o_re, o_im);
+                trace_bad |= !(gq_finite(o_re) && gq_finite(o_im));
+                gq_store_complex(a.trace, (long long)kk * ld + q, o_re, o_im);
+#else
+#pragma unroll
+                for (int i = 0; i < GQ_D; ++i) {
+                    trace_bad |= !gq_finite(y[i]);
+                    ((real*)a.trace)[((long long)kk * GQ_D + i) * ld + q] = y[i];
+                }
+#endif
+            }
+        }
+#endif
+    }
+
+    bool bad = false;
+#if GQ_OUT_MEAN
+    const acc_t inv_t = (acc_t)1 / (acc_t)(num_steps - a.warmup_steps);
+    const real r_re = (real)(avg_re * inv_t), r_im = (real)(avg_im * inv_t);
+    bad = !(gq_finite(r_re) && gq_finite(r_im));
+    gq_store_complex(a.result, q, r_re, r_im);
+#elif GQ_OUT_FINAL
+    real r_re, r_im;
+    gq_obs(y, px, py, C, S, G, r_re, r_im);
+    bad = !(gq_finite(r_re) && gq_finite(r_im));
+    gq_store_complex(a.result, q, r_re, r_im);
+#elif GQ_OUT_FINAL_RHO
+#pragma unroll
+    for (int i = 0; i < GQ_D; ++i) {
+        bad |= !gq_finite(y[i]);
+        ((real*)a.result)[(long long)i * a.point_count + q] = y[i];
+    }
+#elif GQ_OUT_MEAN_STATE
+    const acc_t inv_n = (acc_t)1 / (acc_t)(num_steps - a.warmup_steps);
+#pragma unroll
+    for (int i = 0; i < GQ_D; ++i) {
+        const real v = (real)(savg[i] * inv_n);
+        bad |= !gq_finite(v);
+        ((real*)a.result)[(long long)i * a.point_count + q] = v;
+    }
+#endif
+    gq_flag(a.status, bad, GQ_STATUS_NONFINITE_RESULT);
+#if GQ_TRACE
+    gq_flag(a.status, trace_bad, GQ_STATUS_NONFINITE_TRACE);
+#endif
+}

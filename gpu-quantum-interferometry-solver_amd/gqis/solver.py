@@ -1,0 +1,537 @@
+"""``mesolve_2D`` and the shared sweep driver on the MI355X.
+
+Driver semantics follow the reference ``_solve_symbolic_system`` (``gqis/solver.py:1739-2391``):
+validation of ``tlist`` and options, warmup and trace bookkeeping, the process-local RHS and
+kernel caches, the finite check of the outputs and the timing dictionary.  The device work is
+HIP: the kernel source is emitted from the symbolic plan, compiled with hiprtc and launched
+through ``libgqis_hip.so``; torch provides the device arrays and the stream.
+"""
+from dataclasses import dataclass, field
+import os
+import time
+
+import numpy as np
+import sympy as sp
+
+from ._options import OPTION_GROUPS, apply_option_groups
+from . import codegen, symbolic
+from .solvers import get_solver_spec, normalize_solver_name  # noqa: F401 (re-export)
+from .symbolic import (PARX, PARY, TIME, build_independent_rho,  # noqa: F401 (re-export)
+                       build_reduced_lindblad_rhs, rho_matrix_to_independent_exprs)
+
+_RHS_CODE_CACHE = {}
+_KERNEL_CACHE = {}
+_TIMESTEP_CALIBRATION_CACHE = {}
+
+
+@dataclass(eq=False)
+class CompiledRHS:
+    """Prepared and optimised right-hand side, independent of the device code
+    (the reference's ``CompiledRHS``, ``gqis/solver.py:1344-1371``)."""
+    kind: str
+    state_size: int
+    sweep_symbols: tuple
+    state_symbols: tuple
+    equations: tuple
+    mean_re: object
+    mean_im: object
+    drive_map: object
+    rho0_exprs: object
+    runtime_const_syms: tuple
+    base_runtime_syms: tuple
+    derived_const_exprs: tuple
+    const_values: dict
+    plan: object
+    cache_key: tuple
+    timings: dict
+    ode: object = None
+    emitted: dict = field(default_factory=dict, repr=False)
+
+
+def _rename_time(expr):
+    """Any symbol named ``t`` is the time (the reference prints it as the kernel's ``t``)."""
+    expr = sp.sympify(expr)
+    subs = {s: TIME for s in expr.free_symbols if s.name == "t" and s != TIME}
+    return expr.xreplace(subs) if subs else expr
+
+
+def _prepare_model(H, Drive, Col_Ops, mean_operator, *, sweep_symbols, const_values,
+                   runtime_consts, keep_symbolic_consts, auto_runtime_consts, rho0, ode_size):
+    """Validate the model; map sweep symbols to ParX/ParY; fold or keep constants
+    (behaviour of ``gqis/solver.py:1374-1521``)."""
+    const_values = {} if const_values is None else dict(const_values)
+    runtime_consts = runtime_consts or {}
+    if not isinstance(runtime_consts, dict):
+        raise ValueError("runtime_consts must be a dict of {Symbol: value}")
+    n = H.shape[0]
+    state_size = n * n - 1 if ode_size is None else ode_size
+    rho0_exprs = None
+    if rho0 is not None:
+        rho0_exprs = (rho_matrix_to_independent_exprs(rho0) if ode_size is None
+                      else [sp.sympify(v) for v in rho0])
+        if len(rho0_exprs) != state_size:
+            raise ValueError("rho0 does not match Hamiltonian size.")
+    for op in Col_Ops:
+        if op.shape != (n, n):
+            raise ValueError(f"Collapse operator dimension {op.shape} does not match H {n, n}")
+    if ode_size is None and mean_operator.shape != (n, n):
+        raise ValueError(f"Mean operator dimension {mean_operator.shape} does not match H {n, n}")
+
+    sweep_map = {s: p for s, p in zip(sweep_symbols, (PARX, PARY))}
+    if keep_symbolic_consts is None:
+        keep = set(const_values) if auto_runtime_consts else set()
+    elif isinstance(keep_symbolic_consts, str):
+        if keep_symbolic_consts not in ("all", "auto", "const_values"):
+            raise ValueError("keep_symbolic_consts must be an iterable of Symbols, 'all', "
+                             "'auto', 'const_values' or None")
+        keep = set(const_values)
+    else:
+        keep = set(keep_symbolic_consts)
+    keep |= set(runtime_consts)
+    for s in keep:
+        if not isinstance(s, sp.Symbol):
+            raise ValueError("keep_symbolic_consts/runtime_consts keys must be SymPy Symbols")
+    folded = {k: sp.Float(float(v)) for k, v in const_values.items() if k not in keep}
+
+    def prep(e):
+        return _rename_time(sp.sympify(e).xreplace(sweep_map).xreplace(folded))
+
+    H = H.applyfunc(prep)
+    if isinstance(Drive, dict):
+        Drive = {k: prep(v) for k, v in Drive.items()}
+    else:
+        Drive = prep(Drive)
+    Col_Ops = [op.applyfunc(prep) for op in Col_Ops]
+    mean_operator = mean_operator.applyfunc(prep)
+    if rho0_exprs is not None:
+        rho0_exprs = [prep(e) for e in rho0_exprs]
+    used = set()
+    for e in list(H) + (list(Drive.values()) if isinstance(Drive, dict) else [Drive]) \
+            + [x for op in Col_Ops for x in op] + list(mean_operator) + list(rho0_exprs or []):
+        used |= sp.sympify(e).free_symbols
+    runtime_syms = tuple(sorted((s for s in keep if s in used), key=lambda s: s.name))
+    defaults = {s: v for s, v in {**const_values, **runtime_consts}.items() if s in runtime_syms}
+    return {"N": n, "state_size": state_size, "H": H, "Drive": Drive, "Col_Ops": Col_Ops,
+            "mean_operator": mean_operator, "rho0_exprs": rho0_exprs,
+            "runtime_const_syms": runtime_syms, "folded": folded, "defaults": defaults}
+
+
+def _drive_map(drive, default_symbol=sp.Symbol("Drive", real=True)):
+    if isinstance(drive, dict):
+        return dict(drive)
+    if drive is None:
+        return {}
+    return {default_symbol: drive}
+
+
+def _compile_rhs(H, Drive, Col_Ops, mean_operator, *, sweep_symbols, const_values=None,
+                 runtime_consts=None, keep_symbolic_consts=None, auto_runtime_consts=False,
+                 rho0=None, ode_size=None, pre_expand=True, collect_rho=False, factor_terms=False,
+                 cse_batch_size=None, cse_simplify=False, hoist_rho_independent=True, reuse=True,
+                 symbolic_workers="auto", max_symbolic_workers=None):
+    """Symbolic ODE and hoisting plan, cached per model; returns ``(CompiledRHS, status)``."""
+    start = time.time()
+    model = _prepare_model(H, Drive, Col_Ops, mean_operator, sweep_symbols=sweep_symbols,
+                           const_values=const_values, runtime_consts=runtime_consts,
+                           keep_symbolic_consts=keep_symbolic_consts,
+                           auto_runtime_consts=auto_runtime_consts, rho0=rho0, ode_size=ode_size)
+    drive = _drive_map(model["Drive"])
+    key = ("compiled_rhs", model["N"], ode_size, sp.srepr(model["H"]),
+           tuple((str(k), sp.srepr(v)) for k, v in drive.items()),
+           tuple(sp.srepr(op) for op in model["Col_Ops"]), sp.srepr(model["mean_operator"]),
+           tuple(sorted((str(k), float(v)) for k, v in model["folded"].items())),
+           tuple(s.name for s in model["runtime_const_syms"]),
+           None if model["rho0_exprs"] is None else tuple(sp.srepr(e) for e in model["rho0_exprs"]),
+           pre_expand, collect_rho, factor_terms, cse_batch_size, cse_simplify,
+           hoist_rho_independent)
+    if reuse and key in _RHS_CODE_CACHE:
+        return _RHS_CODE_CACHE[key], "hit"
+    if ode_size is None:
+        eqs, mre, mim, meta = build_reduced_lindblad_rhs(
+            model["N"], model["H"], model["Col_Ops"], model["mean_operator"],
+            pre_expand=pre_expand, collect_rho=collect_rho, factor_terms=factor_terms,
+            symbolic_workers=symbolic_workers, max_symbolic_workers=max_symbolic_workers)
+        states = meta["rho_syms"]
+    else:
+        states = symbolic.state_symbols(ode_size)
+        eqs = list(model["H"])
+        if pre_expand:
+            eqs = [sp.expand(e) for e in eqs]
+        if factor_terms:
+            eqs = [sp.factor_terms(e) for e in eqs]
+        if collect_rho:
+            eqs = [sp.collect(e, states) for e in eqs]
+        obs = sp.sympify(model["mean_operator"][0])
+        mre, mim = sp.re(obs), sp.im(obs)
+    prepared = time.time()
+    plan = symbolic.make_plan(eqs, mre, mim, states, drive, model["runtime_const_syms"],
+                              cse_simplify=cse_simplify)
+    compiled = CompiledRHS(
+        kind="lindblad" if ode_size is None else "ode", state_size=model["state_size"],
+        sweep_symbols=tuple(sweep_symbols), state_symbols=tuple(states), equations=tuple(eqs),
+        mean_re=mre, mean_im=mim, drive_map=model["Drive"],
+        rho0_exprs=None if model["rho0_exprs"] is None else tuple(model["rho0_exprs"]),
+        runtime_const_syms=tuple(model["runtime_const_syms"]),
+        base_runtime_syms=tuple(model["runtime_const_syms"]), derived_const_exprs=(),
+        const_values=model["defaults"], plan=plan, cache_key=key,
+        timings={"ode_prepare_s": prepared - start, "ode_optimize_s": time.time() - prepared})
+    if reuse:
+        _RHS_CODE_CACHE[key] = compiled
+    return compiled, "miss"
+
+
+def compile_lindblad_rhs(H, Drive, Col_Ops, mean_operator, sweep_symbols, *, const_values=None,
+                         runtime_consts=None, keep_symbolic_consts=None, auto_runtime_consts=False,
+                         rho0=None, pre_expand=True, collect_rho=False, factor_terms=False,
+                         cse_batch_size=None, cse_simplify=False, hoist_rho_independent=True,
+                         RHSreuse=True, symbolic_workers="auto", max_symbolic_workers=None):
+    """Prepare and optimise a Lindblad RHS without the GPU (``gqis/solver.py:1592-1627``)."""
+    sweep_symbols = tuple(sweep_symbols)
+    if not 1 <= len(sweep_symbols) <= 2:
+        raise ValueError("sweep_symbols must contain one or two symbols.")
+    compiled, _ = _compile_rhs(
+        H, Drive, Col_Ops, mean_operator, sweep_symbols=sweep_symbols, const_values=const_values,
+        runtime_consts=runtime_consts, keep_symbolic_consts=keep_symbolic_consts,
+        auto_runtime_consts=auto_runtime_consts, rho0=rho0, pre_expand=pre_expand,
+        collect_rho=collect_rho, factor_terms=factor_terms, cse_batch_size=cse_batch_size,
+        cse_simplify=cse_simplify, hoist_rho_independent=hoist_rho_independent, reuse=RHSreuse,
+        symbolic_workers=symbolic_workers, max_symbolic_workers=max_symbolic_workers)
+    return compiled
+
+
+def _runtime_const_values(compiled, runtime_consts, const_values, use_defaults):
+    if not compiled.runtime_const_syms:
+        return []
+    values = dict(compiled.const_values) if use_defaults else {}
+    values.update(const_values or {})
+    values.update(runtime_consts or {})
+    out = []
+    for s in compiled.runtime_const_syms:
+        if s not in values:
+            raise ValueError(f"Missing value for runtime constant symbol: {s}")
+        out.append(float(values[s]))
+    return out
+
+
+def mesolve_2D(H, Drive=None, Col_Ops=None, mean_operator=None, tlist=None,
+               var_arrays=None, const_values=None, kernel_template_file=None, *,
+               RHSreuse=True, runtime_consts=None, keep_symbolic_consts=None,
+               auto_runtime_consts=False, output_mode="mean", solver="rk4",
+               solver_frequency=None, adaptive_options=None, unroll=False, fp64=False,
+               pre_expand=True, collect_rho=False, factor_terms=False,
+               cse_batch_size=None, cse_simplify=False, hoist_rho_independent=True,
+               symbolic_workers="auto", max_symbolic_workers=None,
+               nvrtc_options=(), timings=False, return_timing_info=False, warmup_time=0.0,
+               rho0=None, rho0_var_arrays=None, rho0_values=None,
+               return_time_trace=False, time_trace_every=None,
+               time_trace_samples_per_period=None, solver_samples_per_period=None,
+               Actual_Kernel_Save=False, beep_on_error=False, ignore_non_finite_output=False,
+               t_in=0.0, return_device=False, calibrate_timesteps=None, consts=None, initial=None,
+               output=None, solver_options=None, codegen=None, diagnostics=None,
+               hiprtc_options=(), time_table=True, block_size=256):
+    """Lindblad master-equation sweep over one or two parameter axes on the GPU.
+
+    Same signature, defaults, return shapes and errors as the reference ``mesolve_2D``
+    (``gqis/solver.py:1712-1736``, ``GQIS_API.md:505-528``), plus three code-generation options:
+    ``hiprtc_options`` (extra compiler flags), ``time_table`` (FP64 table of the time-only drive
+    terms for fixed-step solvers; False evaluates them per point as the reference does) and
+    ``block_size`` (threads per work-group).  ``return_device=True`` returns torch tensors on
+    the GPU.  ``H`` may be a ``CompiledRHS`` from ``compile_lindblad_rhs``.
+
+    Returns ``(num_X, num_Y)`` complex for ``"mean"``/``"final"``, ``(num_X, num_Y, N*N-1)``
+    real for ``"final_rho"``; with ``return_time_trace`` also the ``(num_X, num_Y, num_trace)``
+    trace and its times; with ``return_timing_info`` the timing dictionary last.
+    """
+    args = dict(locals())
+    groups = {name: args.pop(name) for name in OPTION_GROUPS}
+    args, _ = apply_option_groups(mesolve_2D, args, groups)
+    if args["output_mode"] not in {"mean", "final", "final_rho"}:
+        raise ValueError("output_mode must be 'mean', 'final', or 'final_rho'.")
+    if isinstance(H, CompiledRHS):
+        if H.kind != "lindblad":
+            raise ValueError("This CompiledRHS comes from compile_ode_rhs; pass it to odesolve_2D.")
+        args.update(H=None, compiled_rhs=H)
+    return _solve_symbolic_system(**args)
+
+
+def _check_tlist(tlist):
+    """(num_steps, dt) of a uniform grid starting at zero (``gqis/solver.py:1951-1979``)."""
+    if tlist is None:
+        raise ValueError("tlist is required for fixed-step integration.")
+    n_t = int(len(tlist))
+    if n_t < 2:
+        raise ValueError("tlist must contain at least two time samples.")
+    t_in_arr = np.asarray(tlist)
+    t = np.asarray(t_in_arr, dtype=np.float64)
+    if t.ndim != 1 or not np.all(np.isfinite(t)):
+        raise ValueError("tlist must be a finite one-dimensional array.")
+    diffs = np.diff(t)
+    if np.any(diffs <= 0.0):
+        raise ValueError("tlist must be strictly increasing.")
+    ftype = t_in_arr.dtype if np.issubdtype(t_in_arr.dtype, np.floating) else np.dtype(np.float64)
+    tol = 8.0 * np.finfo(ftype).eps * max(1.0, float(np.max(np.abs(t))))
+    dt = float((t[-1] - t[0]) / (n_t - 1))
+    return n_t - 1, dt, t, diffs, tol
+
+
+def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=None,
+                           const_values=None, kernel_template_file=None, *, RHSreuse=True,
+                           runtime_consts=None, keep_symbolic_consts=None,
+                           auto_runtime_consts=False, output_mode="mean", solver="rk4",
+                           solver_frequency=None, adaptive_options=None, unroll=False, fp64=False,
+                           pre_expand=True, collect_rho=False, factor_terms=False,
+                           cse_batch_size=None, cse_simplify=False, hoist_rho_independent=True,
+                           symbolic_workers="auto", max_symbolic_workers=None, nvrtc_options=(),
+                           timings=False, return_timing_info=False, warmup_time=0.0, rho0=None,
+                           rho0_var_arrays=None, rho0_values=None, return_time_trace=False,
+                           time_trace_every=None, time_trace_samples_per_period=None,
+                           solver_samples_per_period=None, Actual_Kernel_Save=False,
+                           beep_on_error=False, ignore_non_finite_output=False, t_in=0.0,
+                           return_device=False, calibrate_timesteps=None, compiled_rhs=None,
+                           hiprtc_options=(), time_table=True, block_size=256,
+                           _ode_size=None, _state_trace=False, _time_trace_offset=0):
+    if calibrate_timesteps is not None and calibrate_timesteps is not False:
+        from .calibration import solve_with_timestep_calibration
+        return solve_with_timestep_calibration(dict(locals()))
+    if output_mode not in {"mean", "final", "final_rho", "mean_state"}:
+        raise ValueError(f"Unsupported output_mode '{output_mode}'. "
+                         f"Use one of: {sorted(['final', 'final_rho', 'mean', 'mean_state'])}")
+    if not isinstance(unroll, (bool, np.bool_)):
+        raise ValueError("unroll must be True or False.")
+    unroll = bool(unroll)
+    if kernel_template_file is not None:
+        raise ValueError("kernel_template_file: the HIP kernels are built from the package's "
+                         "solver templates; custom CUDA templates are not supported.")
+    t_in = float(t_in)
+    if not np.isfinite(t_in):
+        raise ValueError("t_in must be finite.")
+    block_size = int(block_size)
+    if block_size not in (64, 128, 256, 512, 1024):
+        raise ValueError("block_size must be 64, 128, 256, 512 or 1024.")
+
+    var_arrays = {} if var_arrays is None else dict(var_arrays)
+    for sym, arr in ({} if rho0_var_arrays is None else dict(rho0_var_arrays)).items():
+        if sym in var_arrays:
+            raise ValueError(f"Symbol {sym} is present in both var_arrays and rho0_var_arrays.")
+        var_arrays[sym] = arr
+    if rho0 is not None and rho0_values is not None:
+        raise ValueError("Use either rho0 or rho0_values, not both.")
+    uses_rho0_values = rho0_values is not None
+    if not var_arrays:
+        raise ValueError("var_arrays or rho0_var_arrays must contain at least one sweep array "
+                         "(ParX).")
+    if len(var_arrays) > 2:
+        raise ValueError("Only 2 total sweep arrays are supported across var_arrays and "
+                         "rho0_var_arrays.")
+    if compiled_rhs is None:
+        if H is None or Drive is None or Col_Ops is None or mean_operator is None:
+            raise TypeError("Give H, Drive, Col_Ops and mean_operator, or a CompiledRHS.")
+    else:
+        if any(x is not None for x in (H, Drive, Col_Ops, mean_operator)):
+            raise ValueError("With a CompiledRHS, do not pass the model (H, Drive, Col_Ops, "
+                             "mean_operator) again.")
+        if const_values or keep_symbolic_consts is not None or auto_runtime_consts or rho0 is not None:
+            raise ValueError("const_values, keep_symbolic_consts, auto_runtime_consts and rho0 are "
+                             "fixed when the RHS is compiled; give values of constants kept "
+                             "symbolic through runtime_consts.")
+        if tuple(var_arrays) != compiled_rhs.sweep_symbols:
+            raise ValueError(f"The sweep arrays must be for {compiled_rhs.sweep_symbols}, in the "
+                             "order given when the RHS was compiled.")
+
+    num_steps, dt, t_host, diffs, tol = _check_tlist(tlist)
+    spec = get_solver_spec(solver, frequency=solver_frequency, step_size=dt,
+                           adaptive_options=adaptive_options, fp64=fp64)
+    if not np.isclose(t_host[0], 0.0, rtol=0.0, atol=tol):
+        raise ValueError("tlist must begin at zero; the kernel derives stage times from the "
+                         "step index.")
+    if not np.allclose(diffs, dt, rtol=1.0e-6, atol=tol):
+        raise ValueError("tlist must be uniformly spaced because the GPU solver uses one "
+                         "fixed dt.")
+    warmup_time = float(warmup_time)
+    if warmup_time < 0.0 or warmup_time > 1.0:
+        raise ValueError("warmup_time must be within [0, 1].")
+    warmup_steps = num_steps if warmup_time >= 1.0 else int(np.floor(warmup_time * num_steps))
+    warmup_steps = max(0, min(num_steps, warmup_steps))
+    return_time_trace = bool(return_time_trace)
+    if time_trace_every is not None and time_trace_samples_per_period is not None:
+        raise ValueError("Use either time_trace_every or time_trace_samples_per_period, not both.")
+    if time_trace_samples_per_period is not None:
+        if solver_samples_per_period is None:
+            raise ValueError("solver_samples_per_period is required when "
+                             "time_trace_samples_per_period is used.")
+        sspp, tspp = int(solver_samples_per_period), int(time_trace_samples_per_period)
+        if sspp <= 0 or tspp <= 0:
+            raise ValueError("solver_samples_per_period and time_trace_samples_per_period must "
+                             "be positive.")
+        trace_stride = max(1, int(round(sspp / tspp)))
+    elif time_trace_every is not None:
+        trace_stride = int(time_trace_every)
+        if trace_stride <= 0:
+            raise ValueError("time_trace_every must be a positive integer.")
+    else:
+        trace_stride = 1
+    trace_offset = int(_time_trace_offset)
+    num_trace = (int(np.ceil((num_steps - trace_offset) / trace_stride))
+                 if return_time_trace else 0)
+    mode = output_mode
+    if output_mode == "mean" and warmup_steps >= num_steps:
+        mode = "final"
+    if output_mode == "mean_state" and warmup_steps >= num_steps:
+        mode = "final_rho"
+    if timings not in (False, None, True, "compact", "full"):
+        raise ValueError("timings must be False, True, 'compact' or 'full'.")
+    collect = bool(timings or return_timing_info)
+    total_start = time.time()
+
+    # 1) symbolic ODE and plan
+    if compiled_rhs is None:
+        compiled, rhs_status = _compile_rhs(
+            H, Drive, Col_Ops, mean_operator, sweep_symbols=tuple(var_arrays),
+            const_values=const_values, runtime_consts=runtime_consts,
+            keep_symbolic_consts=keep_symbolic_consts, auto_runtime_consts=auto_runtime_consts,
+            rho0=rho0, ode_size=_ode_size, pre_expand=pre_expand, collect_rho=collect_rho,
+            factor_terms=factor_terms, cse_batch_size=cse_batch_size, cse_simplify=cse_simplify,
+            hoist_rho_independent=hoist_rho_independent, reuse=RHSreuse,
+            symbolic_workers=symbolic_workers, max_symbolic_workers=max_symbolic_workers)
+    else:
+        compiled, rhs_status = compiled_rhs, "provided"
+    stage_s = (dict(compiled.timings) if rhs_status == "miss"
+               else {"ode_prepare_s": 0.0, "ode_optimize_s": 0.0})
+    const_vals = _runtime_const_values(compiled, runtime_consts, const_values,
+                                       use_defaults=compiled_rhs is not None)
+
+    # 2) HIP source of this solver / output variant
+    variant = _kernel_variant(compiled, spec, mode, uses_rho0_values, return_time_trace,
+                              _state_trace, fp64, time_table, block_size, unroll,
+                              tuple(hiprtc_options) + _as_tuple(nvrtc_options))
+    emit_start = time.time()
+    source = _KERNEL_CACHE.get(("source",) + variant) if RHSreuse else None
+    emit_s = 0.0
+    if source is None:
+        source = _emit_source(compiled, spec, mode, uses_rho0_values, return_time_trace,
+                              _state_trace, fp64, time_table, block_size, unroll)
+        emit_s = time.time() - emit_start
+        if RHSreuse:
+            _KERNEL_CACHE[("source",) + variant] = source
+    if rhs_status != "miss":
+        emit_s = 0.0
+    stage_s["emit_s"] = emit_s
+    if Actual_Kernel_Save:
+        path = (Actual_Kernel_Save if isinstance(Actual_Kernel_Save, str)
+                else os.path.splitext(os.path.abspath(__import__("sys").argv[0] or "gqis"))[0]
+                + "_Kernel.hip")
+        with open(path, "w") as fh:
+            fh.write(source)
+        print(f"Generated kernel saved to: {path}")
+
+    # 3) device work
+    from . import device
+    outputs = device.run_sweep(
+        source=source, variant=variant, spec=spec, compiled=compiled, mode=mode,
+        var_arrays=var_arrays, rho0_values=rho0_values, const_vals=const_vals, fp64=fp64,
+        dt=dt, t_in=t_in, num_steps=num_steps, warmup_steps=warmup_steps,
+        num_trace=num_trace, trace_stride=trace_stride, trace_offset=trace_offset,
+        state_trace=_state_trace, time_table=time_table, block_size=block_size,
+        options=tuple(hiprtc_options) + _as_tuple(nvrtc_options), reuse=RHSreuse,
+        collect=collect)
+    flags = outputs["status"]
+    if flags & 1 and not ignore_non_finite_output:
+        raise RuntimeError(f"Non-finite values detected in "
+                           f"{'mesolve_2D' if _ode_size is None else 'odesolve_2D'} output. "
+                           "Try reducing dt (increase time samples) or check model "
+                           "parameters/RHS.")
+    if flags & 2 and not ignore_non_finite_output:
+        raise RuntimeError(f"Non-finite values detected in "
+                           f"{'mesolve_2D' if _ode_size is None else 'odesolve_2D'} time trace. "
+                           "Try reducing dt (increase time samples) or check model "
+                           "parameters/RHS.")
+    result = outputs["result"]
+    trace = outputs.get("trace")
+    if not return_device:
+        result = result.cpu().numpy()
+        trace = trace.cpu().numpy() if trace is not None else None
+
+    info = None
+    if collect:
+        prep = stage_s["ode_prepare_s"] + stage_s["ode_optimize_s"] + stage_s["emit_s"]
+        info = {"rhs_stage_s": outputs["setup_s"] + prep, "gpu_kernel_s": outputs["gpu_s"],
+                "rhs_codegen_s": prep, "total_s": time.time() - total_start,
+                "ode_prepare_s": stage_s["ode_prepare_s"],
+                "ode_optimize_s": stage_s["ode_optimize_s"], "code_emit_s": stage_s["emit_s"],
+                "kernel_build_s": outputs["build_s"], "cached_rhs": rhs_status,
+                "cached_kernel": "hit" if outputs["cached_kernel"] else "miss",
+                "solver": spec.name, "solver_frequency": solver_frequency, "unroll": unroll}
+        if outputs.get("adaptive_steps") is not None:
+            counts = outputs["adaptive_steps"]
+            info["adaptive_steps"] = {"accepted": counts[..., 0], "rejected": counts[..., 1]}
+        if timings:
+            _print_timings("mesolve_2D" if _ode_size is None else "odesolve_2D", info, timings)
+    if return_time_trace:
+        steps = np.arange(num_trace, dtype=np.float64 if fp64 else np.float32) * trace_stride
+        trace_t = t_in + (steps + 1.0 + trace_offset) * dt
+        out = (result, trace, trace_t)
+        return (*out, info) if return_timing_info else out
+    if return_timing_info:
+        return result, info
+    return result
+
+
+def _as_tuple(options):
+    if options is None:
+        return ()
+    if isinstance(options, str):
+        return (options,)
+    return tuple(options)
+
+
+def _kernel_variant(compiled, spec, mode, uses_rho0_values, trace, state_trace, fp64, time_table,
+                    block_size, unroll, options):
+    adaptive_key = tuple(sorted((k, v) for k, v in spec.options.items()
+                                if k in ("count_steps", "state_fp64", "persistent")))
+    return (compiled.cache_key, spec.name, adaptive_key, mode, bool(uses_rho0_values),
+            bool(trace), bool(state_trace), bool(fp64), bool(time_table), int(block_size),
+            bool(unroll), options)
+
+
+def _emit_source(compiled, spec, mode, uses_rho0_values, trace, state_trace, fp64, time_table,
+                 block_size, unroll):
+    defines = {
+        "GQ_OUT_MEAN": int(mode == "mean"),
+        "GQ_OUT_FINAL": int(mode == "final"),
+        "GQ_OUT_FINAL_RHO": int(mode == "final_rho"),
+        "GQ_OUT_MEAN_STATE": int(mode == "mean_state"),
+        "GQ_TRACE": (2 if state_trace else 1) if trace else 0,
+        "GQ_RHO0_VALUES": int(bool(uses_rho0_values)),
+        "GQ_TIME_TABLE": int(bool(time_table) and not spec.adaptive),
+    }
+    templates = [spec.template]
+    if spec.adaptive:
+        from .solvers import TSIT5A_DEFAULTS  # noqa: F401
+        defines["GQ_METHOD_DOP853"] = int(spec.name == "dop853a")
+        defines["GQ_COUNT_STEPS"] = int(bool(spec.options.get("count_steps")))
+        state64 = spec.options.get("state_fp64")
+        defines["GQ_STATE_FP64"] = int(bool(fp64 or state64))
+        defines["GQ_PERSISTENT"] = int(bool(spec.options.get("persistent", True)))
+    elif spec.template == "fixed_tableau.hip":
+        from .tableaux import tableau_defines
+        defines.update(tableau_defines(spec.name))
+    return codegen.assemble(compiled.plan, templates, fp64=fp64, acc64=True,
+                            block_size=block_size, defines=defines,
+                            rho0_exprs=compiled.rho0_exprs)
+
+
+def _print_timings(api_name, info, mode):
+    ode = {"miss": "new ODE", "hit": "reused ODE", "provided": "compiled ODE"}[info["cached_rhs"]]
+    kernel = "new kernel" if info["cached_kernel"] == "miss" else "reused kernel"
+
+    def show(sec):
+        return f"{sec * 1e3:.1f} ms" if sec < 1.0 else f"{sec:.2f} s"
+    if mode != "full":
+        print(f"{api_name}: preparation {show(info['rhs_stage_s'])} ({ode}, {kernel}), "
+              f"GPU {show(info['gpu_kernel_s'])}, total {show(info['total_s'])}")
+        return
+    print(f"{api_name} timings ({info['solver']}):")
+    for name, key in (("symbolic ODE", "ode_prepare_s"), ("optimization", "ode_optimize_s"),
+                      ("HIP code", "code_emit_s"), ("kernel build", "kernel_build_s"),
+                      ("GPU calculation", "gpu_kernel_s"), ("total", "total_s")):
+        print(f"  {name:<18}{show(info[key]):>10}")

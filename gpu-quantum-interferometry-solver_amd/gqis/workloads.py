@@ -1,0 +1,98 @@
+"""Symbolic definitions of the benchmark sweep workloads (inputs for ``mesolve_2D`` /
+``odesolve_2D``), with the sizes they are quoted at.
+
+* ``lzs_two_level``: two-level Landau-Zener-Stuckelberg qubit, 64 x 64 grid, 2000 RK4 steps.
+* ``four_level``: four-level interferometer, 2048 x 2048 grid, t in [0, 100], dt = 0.05.
+* ``random_n_level``: N-level scaling model (seeded GUE H0, random diagonal H1/H2/observable).
+* ``duffing``: forced Duffing oscillator, 1024 x 1024 grid over (gamma, x0), t in [0, 500].
+"""
+import numpy as np
+import sympy as sp
+
+EPS, AMP, T, DRIVE = sp.symbols("eps A t Drive", real=True)
+
+
+def lzs_two_level(n_eps=64, n_amp=64, n_t=2001, t_end=200.0):
+    """H = (eps/2) sz + (Delta/2) sx, Delta = 1; drive A cos(t) sz/2; collapse sqrt(0.01) s-
+    (s- = [[0,1],[0,0]]) and sqrt(0.005) sz; observable sz; rho0 = |0><0|."""
+    sz = sp.Matrix([[1, 0], [0, -1]])
+    sx = sp.Matrix([[0, 1], [1, 0]])
+    sm = sp.Matrix([[0, 1], [0, 0]])
+    h = EPS / 2 * sz + sp.Rational(1, 2) * sx + DRIVE * sz / 2
+    col = [sp.sqrt(sp.Rational(1, 100)) * sm, sp.sqrt(sp.Rational(5, 1000)) * sz]
+    return {
+        "H": h, "Drive": {DRIVE: AMP * sp.cos(T)}, "Col_Ops": col, "mean_operator": sz,
+        "tlist": np.linspace(0.0, t_end, n_t),
+        "var_arrays": {EPS: np.linspace(-10.0, 10.0, n_eps), AMP: np.linspace(0.0, 20.0, n_amp)},
+    }
+
+
+def four_level(n_eps=2048, n_amp=2048, n_t=2001, t_end=100.0, tunnel=sp.Rational(1, 2)):
+    """H = diag(eps, -eps, eps+2, -eps-2) + nearest-neighbour tunnelling 0.5
+    + A cos(t) diag(1,-1,1,-1)/2; collapse sqrt(0.02)|i><i+1| (i = 0..2) and
+    sqrt(0.01) diag(1,-1,1,-1); observable |0><0|."""
+    zz = sp.diag(1, -1, 1, -1)
+    h = sp.diag(EPS, -EPS, EPS + 2, -EPS - 2) + DRIVE * zz / 2
+    for i in range(3):
+        h[i, i + 1] = h[i + 1, i] = tunnel
+    col = []
+    for i in range(3):
+        op = sp.zeros(4)
+        op[i, i + 1] = sp.sqrt(sp.Rational(2, 100))
+        col.append(op)
+    col.append(sp.sqrt(sp.Rational(1, 100)) * zz)
+    proj = sp.zeros(4)
+    proj[0, 0] = 1
+    return {
+        "H": h, "Drive": {DRIVE: AMP * sp.cos(T)}, "Col_Ops": col, "mean_operator": proj,
+        "tlist": np.linspace(0.0, t_end, n_t),
+        "var_arrays": {EPS: np.linspace(-15.0, 15.0, n_eps), AMP: np.linspace(0.0, 30.0, n_amp)},
+    }
+
+
+def random_n_level_operators(n, seed):
+    """Seeded numeric operators: H0 = (G + G^+)/2 with complex normal E|G_ij|^2 = 1;
+    H1, H2 and the observable random diagonal in [-1, 1]."""
+    rng = np.random.default_rng(seed)
+    g = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(2.0)
+    h0 = (g + g.conj().T) / 2.0
+    h1 = rng.uniform(-1.0, 1.0, n)
+    h2 = rng.uniform(-1.0, 1.0, n)
+    obs = rng.uniform(-1.0, 1.0, n)
+    return h0, h1, h2, obs
+
+
+def random_n_level(n, seed=2024, n_eps=256, n_amp=256, n_t=2001, t_end=20.0, rate=0.01):
+    h0, h1, h2, obs = random_n_level_operators(n, seed)
+
+    def num(z):
+        z = complex(z)
+        return sp.Float(z.real, 17) + sp.I * sp.Float(z.imag, 17) if z.imag else sp.Float(z.real, 17)
+    h = sp.Matrix(n, n, lambda i, j: num(h0[i, j]))
+    h += EPS * sp.diag(*[sp.Float(v, 17) for v in h1]) + DRIVE * sp.diag(*[sp.Float(v, 17) for v in h2])
+    col = []
+    for i in range(n - 1):
+        op = sp.zeros(n)
+        op[i, i + 1] = sp.sqrt(sp.Float(rate, 17))
+        col.append(op)
+    return {
+        "H": h, "Drive": {DRIVE: AMP * sp.cos(T)}, "Col_Ops": col,
+        "mean_operator": sp.diag(*[sp.Float(v, 17) for v in obs]),
+        "tlist": np.linspace(0.0, t_end, n_t),
+        "var_arrays": {EPS: np.linspace(-5.0, 5.0, n_eps), AMP: np.linspace(0.0, 5.0, n_amp)},
+    }
+
+
+X, V, GAMMA, X0 = sp.symbols("x v gamma x0", real=True)
+
+
+def duffing(n_gamma=1024, n_x0=1024, t_end=500.0, n_out=5001, delta=0.2, alpha=-1.0, beta=1.0,
+            w=1.2):
+    """x'' + delta x' + alpha x + beta x^3 = gamma cos(w t) as (x, v); sweep (gamma, x0),
+    v0 = 0; tlist is the output grid (dt_out = 0.1 for 5001 points over [0, 500])."""
+    rhs = [V, -delta * V - alpha * X - beta * X ** 3 + GAMMA * sp.cos(w * T)]
+    return {
+        "rhs": rhs, "state_symbols": [X, V], "y0": [X0, 0],
+        "tlist": np.linspace(0.0, t_end, n_out),
+        "var_arrays": {GAMMA: np.linspace(0.1, 0.6, n_gamma), X0: np.linspace(-2.0, 2.0, n_x0)},
+    }

@@ -1,0 +1,127 @@
+/*
+ * gqis_hip.h -- C ABI of the MI355X (gfx950) batched parameter-sweep integrator.
+ *
+ * The library replaces the reference's CuPy/NVRTC device layer for the sweep path:
+ *   compile  : cp.RawKernel(kernel_code, "time_evolution_kernel", options).compile()
+ *              (reference gqis/solver.py:2237-2241)
+ *   launch   : time_evolution_kernel(grid, block, args)            (gqis/solver.py:2340)
+ *   timing   : cp.cuda.Event() pair around the launch               (gqis/solver.py:2316-2345)
+ *   checks   : cp.all(cp.isfinite(results)) after the launch        (gqis/solver.py:2351-2362)
+ * Kernels are HIP source compiled at run time with hiprtc for the device's architecture; all
+ * state, axis and output buffers are caller-owned device memory.  Every function returns 0 on
+ * success and a negative GQ_E* code otherwise; gq_last_error() gives the message (per thread).
+ *
+ * This header is also prepended to every generated kernel source, so the argument structs
+ * below have one definition on the host and on the device.  It uses builtin types only.
+ */
+#ifndef GQIS_HIP_H
+#define GQIS_HIP_H
+
+#define GQ_OK 0
+#define GQ_E_INVALID (-1)     /* bad argument (maps to ValueError)                   */
+#define GQ_E_COMPILE (-2)     /* hiprtc compilation failed (log in gq_last_error)    */
+#define GQ_E_HIP (-3)         /* HIP runtime error                                   */
+#define GQ_E_NOT_FOUND (-4)   /* kernel name not in the module                       */
+
+/* status bits written by the kernels with one atomic OR per wavefront */
+#define GQ_STATUS_NONFINITE_RESULT 1u   /* a result value is NaN or Inf              */
+#define GQ_STATUS_NONFINITE_TRACE 2u    /* a time-trace value is NaN or Inf          */
+#define GQ_STATUS_SUBSTEP_OVERFLOW 4u   /* an adaptive point exceeded max_substeps   */
+
+#define GQ_MAX_SOLVER_PARAMS 8
+
+/*
+ * Arguments of one sweep launch (fixed-step and adaptive kernels).  Pointers are device
+ * addresses; element type "real" is float or double as the module was compiled.
+ * Flat point p = ix * num_y + iy; a launch covers p in [point_begin, point_begin+point_count)
+ * and writes its outputs at the local index q = p - point_begin:
+ *   complex results   result[q]                         (real2: re, im)
+ *   state results     result[i * point_count + q]       (i < state size)
+ *   observable trace  trace[k * point_count + q]        (real2)
+ *   state trace       trace[(k * D + i) * point_count + q]
+ *   initial states    rho0[q * D + i]                   (the reference's (nx, ny, D) order)
+ */
+typedef struct gq_sweep_args {
+    double dt;                 /* uniform tlist step, FP64 on the host                       */
+    double t_in;               /* absolute time of the initial state                         */
+    long long num_x;           /* sweep axis lengths                                        */
+    long long num_y;
+    long long point_begin;
+    long long point_count;
+    int num_steps;             /* tlist intervals                                            */
+    int warmup_steps;          /* steps excluded from the mean                               */
+    int trace_stride;
+    int trace_offset;
+    int num_trace;             /* 0: no trace                                                */
+    int trace_point_stride;    /* leading dimension of the trace arrays (>= point_count)     */
+    const void* par_x;         /* real[num_x]                                                */
+    const void* par_y;         /* real[num_y]                                                */
+    const void* consts;        /* real[]: runtime constants                                  */
+    const void* rho0;          /* real[point_count * D] or null                              */
+    const void* time_table;    /* real[stage times * time atoms] or null                     */
+    double solver_param[GQ_MAX_SOLVER_PARAMS];
+    void* result;
+    void* trace;
+    int* adaptive_steps;       /* int[2 * point_count] accepted/rejected, or null           */
+    unsigned int* status;      /* one word, OR-ed with GQ_STATUS_* bits                      */
+    unsigned int* work_cursor; /* one word for persistent kernels (zero before the launch)  */
+} gq_sweep_args;
+
+#ifndef __HIPCC_RTC__
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gq_ctx gq_ctx;
+typedef struct gq_module gq_module;
+
+/* One context per process and device: selects the device, reads its gcnArchName and compute
+ * unit count.  Replaces CuPy's implicit current device (gqis/solver.py:16, 2237). */
+int gq_ctx_create(int device_ordinal, gq_ctx** out);
+void gq_ctx_destroy(gq_ctx* ctx);
+/* Architecture name ("gfx950") and compute-unit count of the context's device. */
+int gq_ctx_info(gq_ctx* ctx, char* arch, int arch_len, int* compute_units);
+
+/* Compile HIP source with hiprtc for the context's architecture and load it.  Modules are
+ * cached in the context by a hash of source, options and architecture; a cache hit returns
+ * the loaded module (cached_out = 1).  Replaces cp.RawKernel(...).compile()
+ * (gqis/solver.py:2237-2241). */
+int gq_module_compile(gq_ctx* ctx, const char* source, const char* const* options, int num_options,
+                      gq_module** out, int* cached_out, double* build_seconds);
+/* Compile only (no device needed): returns the code-object size, or the log on failure.
+ * Used by the build check on machines without a GPU. */
+int gq_compile_check(const char* source, const char* arch, const char* const* options,
+                     int num_options, long long* code_size);
+/* Launch kernel `name` of a module over `grid` x `block` threads with one gq_sweep_args by
+ * value, on a HIP stream (null: the default stream).  Replaces
+ * time_evolution_kernel(grid, block, args) (gqis/solver.py:2322-2340). */
+int gq_launch_sweep(gq_module* module, const char* name, unsigned int grid, unsigned int block,
+                    const gq_sweep_args* args, void* stream);
+/* Number of work-groups of `block` threads of kernel `name` that fit on the device at once
+ * (occupancy x compute units): the grid of a persistent launch. */
+int gq_resident_groups(gq_module* module, const char* name, unsigned int block, int* groups);
+void gq_module_release(gq_module* module);
+
+/* Event pair timing on a stream (cp.cuda.Event, gqis/solver.py:2316-2345). */
+int gq_event_create(void** event);
+int gq_event_record(void* event, void* stream);
+int gq_event_elapsed_ms(void* start, void* stop, float* ms);
+int gq_event_destroy(void* event);
+int gq_stream_synchronize(void* stream);
+
+/* Read (and optionally clear) a device status word written by the kernels; synchronises the
+ * stream first.  Replaces the isfinite reduction (gqis/solver.py:2351-2362). */
+int gq_status_read(unsigned int* device_status, void* stream, unsigned int* flags, int clear);
+
+/* sizeof(gq_sweep_args) as compiled into the library (host-side layout check). */
+unsigned long long gq_sweep_args_size(void);
+
+/* Message of the last failing call on this thread. */
+const char* gq_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* __HIPCC_RTC__ */
+
+#endif /* GQIS_HIP_H */
