@@ -114,6 +114,15 @@ def main():
     log("FP64 reference sweep ...")
     ref64, info64, lat64 = run_sweeps(w, True, 2)
     ref64 = ref64.cpu().numpy()
+    # RK4 at dt = 0.05 is unstable in a corner of the grid (|2 eps + 2 + A cos t| dt beyond the
+    # stability limit): there the state leaves the set of density matrices and rounding
+    # differences are amplified without bound.  Those points are reported, not gated.
+    from gqis import mesolve_2D
+    fin = mesolve_2D(w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"],
+                     var_arrays=w["var_arrays"], fp64=True, output_mode="final_rho",
+                     return_device=True, ignore_non_finite_output=True)
+    stable = (fin.abs().amax(dim=-1) <= 1.0).cpu().numpy()
+    del fin
     fp64_kernel_s = info64[-1]["gpu_kernel_s"]
     plain = None
     if not args.no_plain:
@@ -132,9 +141,11 @@ def main():
     res32 = res32.cpu().numpy()
 
     finite = np.isfinite(ref64) & np.isfinite(res32)
+    gate = finite & stable
     err = np.abs(res32.astype(np.complex128) - ref64)
-    max_err = float(np.max(err[finite])) if finite.any() else float("nan")
-    frac_over = float(np.mean(err[finite] > 1e-6)) if finite.any() else float("nan")
+    max_err = float(np.max(err[gate])) if gate.any() else float("nan")
+    frac_over = float(np.mean(err[gate] > 1e-6)) if gate.any() else float("nan")
+    max_err_all = float(np.max(err[finite])) if finite.any() else float("nan")
     kernel_s = float(np.mean([i["gpu_kernel_s"] for i in info32]))
     point_steps = npts * nsteps
     achieved = flops_ps * point_steps / kernel_s / 1e12
@@ -150,7 +161,9 @@ def main():
                                f"2000 steps dt=0.05, mean of |0><0|",
                    "points": npts, "rk_steps": nsteps, "solver": "rk4",
                    "precision": "fp32 RHS/state, fp64 time table + mean accumulator"},
-        "error_vs_fp64": {"max_abs": max_err, "frac_points_over_1e-6": frac_over,
+        "error_vs_fp64": {"max_abs_stable": max_err, "frac_stable_points_over_1e-6": frac_over,
+                          "rk4_unstable_points": int(np.sum(~stable)),
+                          "max_abs_all_points": max_err_all,
                           "nonfinite_points": int(np.sum(~finite))},
         "latency_s": {"p50": float(np.percentile(lat32, 50)), "p99": float(np.percentile(lat32, 99)),
                       "samples": len(lat32)},

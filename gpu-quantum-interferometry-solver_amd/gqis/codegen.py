@@ -234,11 +234,12 @@ def prelude(fp64, acc64, block_size, defines):
 
 
 def assemble(plan, templates, *, fp64, acc64=True, block_size=256, defines=None,
-             rho0_exprs=None):
-    """Full hiprtc source: ABI header + prelude + model + solver templates."""
+             rho0_exprs=None, extra=""):
+    """Full hiprtc source: ABI header + prelude + model + solver constants + templates."""
     with open(_header_path()) as fh:
         header = fh.read()
-    parts = [header, prelude(fp64, acc64, block_size, defines or {}), emit_model(plan, rho0_exprs)]
+    parts = [header, prelude(fp64, acc64, block_size, defines or {}), emit_model(plan, rho0_exprs),
+             extra]
     parts += [read_kernel_template(t) for t in (["common.hip"] + list(templates))]
     return "\n".join(parts)
 

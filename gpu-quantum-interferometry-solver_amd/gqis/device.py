@@ -63,8 +63,10 @@ def run_sweep(*, source, variant, spec, compiled, mode, var_arrays, rho0_values,
         result = torch.empty(max(npts, 1), dtype=cdt, device=dev)
     trace = None
     if num_trace:
-        trace = (torch.empty((num_trace, size, npts), dtype=rdt, device=dev) if state_trace
-                 else torch.empty((num_trace, npts), dtype=cdt, device=dev))
+        # zero-filled: an adaptive point that fails stops writing its trace
+        alloc = torch.zeros if spec.adaptive else torch.empty
+        trace = (alloc((num_trace, size, npts), dtype=rdt, device=dev) if state_trace
+                 else alloc((num_trace, npts), dtype=cdt, device=dev))
     status = torch.zeros(1, dtype=torch.int32, device=dev)
     cursor = torch.zeros(1, dtype=torch.int32, device=dev)
     natoms = len(compiled.plan.time_atoms)

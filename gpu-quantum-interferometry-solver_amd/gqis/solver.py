@@ -505,19 +505,20 @@ def _emit_source(compiled, spec, mode, uses_rho0_values, trace, state_trace, fp6
         "GQ_TIME_TABLE": int(bool(time_table) and not spec.adaptive),
     }
     templates = [spec.template]
+    from . import tableaux
+    extra = ""
     if spec.adaptive:
-        from .solvers import TSIT5A_DEFAULTS  # noqa: F401
+        extra = tableaux.emit_arrays(tableaux.DOP853 if spec.name == "dop853a" else tableaux.TSIT5)
         defines["GQ_METHOD_DOP853"] = int(spec.name == "dop853a")
         defines["GQ_COUNT_STEPS"] = int(bool(spec.options.get("count_steps")))
         state64 = spec.options.get("state_fp64")
         defines["GQ_STATE_FP64"] = int(bool(fp64 or state64))
         defines["GQ_PERSISTENT"] = int(bool(spec.options.get("persistent", True)))
     elif spec.template == "fixed_tableau.hip":
-        from .tableaux import tableau_defines
-        defines.update(tableau_defines(spec.name))
+        extra = tableaux.emit_fixed(spec.name)
     return codegen.assemble(compiled.plan, templates, fp64=fp64, acc64=True,
                             block_size=block_size, defines=defines,
-                            rho0_exprs=compiled.rho0_exprs)
+                            rho0_exprs=compiled.rho0_exprs, extra=extra)
 
 
 def _print_timings(api_name, info, mode):

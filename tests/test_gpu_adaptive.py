@@ -1,0 +1,85 @@
+"""Adaptive kernels (tsit5a, dop853a) on the GPU: closed forms of the reference's tests and
+parity with the FP64 oracle's restatement of the same controller."""
+import numpy as np
+import pytest
+import sympy as sp
+
+from gqis import mesolve_2D, odesolve_2D, workloads
+from oracle import integrate, models
+
+pytestmark = pytest.mark.gpu
+DUMMY = sp.Symbol("dummy", real=True)
+
+
+def test_tsit5a_time_dependent_commuting_hamiltonian(gpu):
+    # reference tests/test_gpu_solver.py:269-288
+    drive, t = sp.Symbol("Drive", real=True), sp.Symbol("t", real=True)
+    final_time = 1.7
+    h = (1 + drive) * sp.Matrix([[0, 1], [1, 0]]) / 2
+    rho = mesolve_2D(h, {drive: sp.Rational(2, 5) * sp.sin(t)}, [], sp.eye(2),
+                     np.linspace(0.0, final_time, 5), var_arrays={DUMMY: np.zeros(1)},
+                     output_mode="final_rho", solver="tsit5a", fp64=True,
+                     adaptive_options={"step_rtol": 1e-12, "step_atol": 1e-12,
+                                       "max_substeps": 10000})
+    phase = final_time + 0.4 * (1.0 - np.cos(final_time))
+    assert abs(float(rho[0, 0, 0]) - np.cos(0.5 * phase) ** 2) < 2e-7
+
+
+@pytest.mark.parametrize("persistent", [True, False])
+def test_tsit5a_trapezoid_time_average(gpu, persistent):
+    # reference tests/test_gpu_solver.py:291-310
+    value, info = mesolve_2D(
+        sp.Matrix([[0, 1], [1, 0]]) / 2, sp.Integer(0), [], sp.diag(1, 0),
+        np.linspace(0.0, 10.0, 6), var_arrays={DUMMY: np.zeros(1)},
+        rho0=sp.Matrix([[1, 0], [0, 0]]), output_mode="mean", warmup_time=0.2, solver="tsit5a",
+        fp64=True, return_timing_info=True,
+        adaptive_options={"step_rtol": 1e-10, "step_atol": 1e-10, "count_steps": True,
+                          "max_substeps": 10000, "persistent": persistent})
+    expected = 0.5 + (np.sin(10.0) - np.sin(2.0)) / 16.0
+    assert abs(float(np.real(value[0, 0])) - expected) < 5e-5
+    assert info["adaptive_steps"]["accepted"][0, 0] > 5
+
+
+@pytest.mark.parametrize("solver", ["tsit5a", "dop853a"])
+def test_duffing_adaptive_matches_oracle(gpu, solver):
+    w = workloads.duffing(n_gamma=6, n_x0=5, t_end=20.0, n_out=201)
+    opts = {"step_rtol": 1e-8, "step_atol": 1e-10, "max_substeps": 1000, "count_steps": True}
+    res, tr, times, info = odesolve_2D(w["rhs"], w["state_symbols"], w["y0"], w["tlist"],
+                                       var_arrays=w["var_arrays"], solver=solver, fp64=True,
+                                       adaptive_options=opts, return_time_trace=True,
+                                       time_trace_every=20, return_timing_info=True)
+    g, x0 = w["var_arrays"].values()
+    d = models.Duffing(g, x0)
+    fn = integrate.tsit5a if solver == "tsit5a" else integrate.dop853a
+    ref, ref_tr, acc, rej = fn(d.f, d.y0(), 0.1, 200, rtol=1e-8, atol=1e-10, max_substeps=1000,
+                               trace_stride=20, n_trace=10, trace_kind="state")
+    assert res.shape == (6, 5, 2) and tr.shape == (6, 5, 10, 2)
+    np.testing.assert_array_equal(info["adaptive_steps"]["accepted"].reshape(-1), acc)
+    np.testing.assert_array_equal(info["adaptive_steps"]["rejected"].reshape(-1), rej)
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(res.reshape(-1, 2) - ref)) <= 1e-10 * scale
+    assert np.max(np.abs(tr.reshape(30, 10, 2) - ref_tr)) <= 1e-10 * scale
+
+
+def test_persistent_and_one_shot_launch_agree_bitwise(gpu):
+    w = workloads.duffing(n_gamma=16, n_x0=16, t_end=10.0, n_out=101)
+    out = []
+    for persistent in (True, False):
+        out.append(odesolve_2D(w["rhs"], w["state_symbols"], w["y0"], w["tlist"],
+                               var_arrays=w["var_arrays"], solver="tsit5a", fp64=True,
+                               adaptive_options={"step_rtol": 1e-9, "step_atol": 1e-11,
+                                                 "max_substeps": 500,
+                                                 "persistent": persistent}))
+    np.testing.assert_array_equal(out[0], out[1])
+
+
+def test_substep_overflow_is_reported(gpu):
+    w = workloads.duffing(n_gamma=2, n_x0=2, t_end=5.0, n_out=11)
+    with pytest.raises(RuntimeError, match="Non-finite"):
+        odesolve_2D(w["rhs"], w["state_symbols"], w["y0"], w["tlist"], var_arrays=w["var_arrays"],
+                    solver="tsit5a", fp64=True,
+                    adaptive_options={"step_rtol": 1e-14, "step_atol": 1e-14, "max_substeps": 2})
+    res = odesolve_2D(w["rhs"], w["state_symbols"], w["y0"], w["tlist"], var_arrays=w["var_arrays"],
+                      solver="tsit5a", fp64=True, ignore_non_finite_output=True,
+                      adaptive_options={"step_rtol": 1e-14, "step_atol": 1e-14, "max_substeps": 2})
+    assert np.all(np.isnan(res))

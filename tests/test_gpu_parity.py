@@ -27,8 +27,14 @@ def test_four_level_mean_matches_oracle(gpu, fp64):
     if fp64:
         assert _rel(got, ref) <= 1e-10
     else:
-        # FP32 RHS and state, FP64 time table and FP64 mean accumulation
-        assert np.max(np.abs(got - ref)) <= 2e-5
+        # RK4 at dt = 0.05 is unstable where |2 eps + 2 + A cos t| dt exceeds the stability
+        # limit (the eps = 15, A = 30 corner): there the state grows beyond a density matrix and
+        # any rounding difference is amplified.  Elsewhere: FP32 RHS and state with the FP64
+        # time table and mean accumulator stay within 5e-6 of the FP64 oracle.
+        final, _ = integrate.rk4(m.f, m.ground_state(), 0.05, 2000, obs=m.obs, mode="final_rho")
+        stable = (np.max(np.abs(final), axis=1) <= 1.0).reshape(16, 12)
+        assert stable.sum() >= 180
+        assert np.max(np.abs(got - ref)[stable]) <= 5e-6
 
 
 def test_two_level_final_rho_and_trace(gpu):
