@@ -42,6 +42,19 @@ class HipPrinter(C99CodePrinter):
         self._names = names
         self._lit = literal
 
+    def doprint(self, expr, assign_to=None):
+        """Fold exact irrational constants (sqrt(2), pi, E, log(2), ...) to literals first."""
+        expr = sp.sympify(expr)
+        consts = {}
+        walk = sp.preorder_traversal(expr)
+        for sub in walk:
+            if sub.is_number and not (sub.is_Rational or sub.is_Float) and sub.is_real:
+                consts[sub] = sp.Float(sub.evalf(20), 20)
+                walk.skip()
+        if consts:
+            expr = expr.xreplace(consts)
+        return super().doprint(expr, assign_to)
+
     def _lit_str(self, value):
         return f"{self._lit}({float(value)!r})"
 
@@ -217,6 +230,15 @@ def emit_model(plan, rho0_exprs=None):
     return "\n".join(lines)
 
 
+def kernel_suffix(fp64):
+    """Kernel names carry the precision (gq_rk4_f64, gq_adaptive_f32, ...)."""
+    return "f64" if fp64 else "f32"
+
+
+def kernel_name(base, fp64):
+    return f"{base}_{kernel_suffix(fp64)}"
+
+
 def prelude(fp64, acc64, block_size, defines):
     real = "double" if fp64 else "float"
     acc = "double" if (fp64 or acc64) else "float"
@@ -226,6 +248,10 @@ def prelude(fp64, acc64, block_size, defines):
            "#define R(x) ((real)(x))",
            "#define DBL(x) ((double)(x))",
            f"#define GQ_FP64 {1 if fp64 else 0}",
+           "#define GQ_CAT_(a, b) a##_##b",
+           "#define GQ_CAT(a, b) GQ_CAT_(a, b)",
+           f"#define GQ_KSUFFIX {kernel_suffix(fp64)}",
+           "#define GQ_KERNEL(name) GQ_CAT(name, GQ_KSUFFIX)",
            f"#define GQ_BLOCK {int(block_size)}",
            "__device__ __forceinline__ real gq_sign(real x) { return (x > R(0)) - (x < R(0)); }"]
     for k, v in defines.items():

@@ -103,14 +103,16 @@ def run_sweep(*, source, variant, spec, compiled, mode, var_arrays, rho0_values,
     if npts > 0 and num_steps >= 0:
         if timer:
             timer.record_start()
+        kname = codegen.kernel_name(spec.kernel, fp64)
         if use_table:
-            ctx.launch(module, table_kernel(spec), (rows + 255) // 256, 256, a, stream)
+            ctx.launch(module, codegen.kernel_name(table_kernel(spec), fp64), (rows + 255) // 256,
+                       256, a, stream)
         if spec.adaptive and spec.options.get("persistent", True):
-            grid = min(ctx.resident_groups(module, spec.kernel, block_size),
+            grid = min(ctx.resident_groups(module, kname, block_size),
                        (npts + block_size - 1) // block_size)
         else:
             grid = (npts + block_size - 1) // block_size
-        ctx.launch(module, spec.kernel, max(grid, 1), block_size, a, stream)
+        ctx.launch(module, kname, max(grid, 1), block_size, a, stream)
         if timer:
             timer.record_stop()
     flags = ctx.read_status(status, stream)

@@ -62,7 +62,7 @@ __device__ __forceinline__ void gq_observe(const st_t* __restrict__ y, real px, 
     gq_obs(yr, px, py, C, S, G, re, im);
 }
 
-extern "C" __global__ __launch_bounds__(GQ_BLOCK) void gq_adaptive(const gq_sweep_args a) {
+extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_adaptive)(const gq_sweep_args a) {
     const double dt_out = a.dt;
     const double rtol = a.solver_param[0], atol = a.solver_param[1];
     const int max_sub = (int)a.solver_param[2];

@@ -25,7 +25,7 @@ __device__ __forceinline__ void gq_time_atoms(const gq_sweep_args& a, const real
 }
 
 // FP64 table of the time atoms at every RK4 stage time: 2 * num_steps + 1 rows.
-extern "C" __global__ void gq_rk4_time_table(const gq_sweep_args a) {
+extern "C" __global__ void GQ_KERNEL(gq_rk4_time_table)(const gq_sweep_args a) {
     const int s = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (s > 2 * a.num_steps) return;
     const real* __restrict__ C = (const real*)a.consts;
@@ -48,7 +48,7 @@ extern "C" __global__ void gq_rk4_time_table(const gq_sweep_args a) {
     for (int k = 0; k < GQ_NATOMS; ++k) tab[s * GQ_NATOMS + k] = (real)TT[k];
 }
 
-extern "C" __global__ __launch_bounds__(GQ_BLOCK) void gq_rk4(const gq_sweep_args a) {
+extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_rk4)(const gq_sweep_args a) {
     const long long q = (long long)blockIdx.x * GQ_BLOCK + threadIdx.x;
     if (q >= a.point_count) return;
     const gq_point pt = gq_locate(a, q);

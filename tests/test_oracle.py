@@ -1,0 +1,201 @@
+"""Pins the FP64 oracle: closed forms, the reference's known answers, convergence order,
+SciPy and matrix exponentials.  A plausible slip (wrong stage time, dropped term, transposed
+generator, wrong mean window) fails at least one of these."""
+import numpy as np
+import pytest
+import scipy.linalg
+from scipy.integrate import solve_ivp
+
+from oracle import integrate, lindblad as lb, models
+
+SX = np.array([[0, 1], [1, 0]], complex)
+SZ = np.array([[1, 0], [0, -1]], complex)
+SM = np.array([[0, 1], [0, 0]], complex)
+
+
+def test_reduced_generator_is_the_bloch_equations():
+    # reference tests/test_package.py:63-85 (closed-form Bloch equations)
+    rng = np.random.default_rng(1)
+    for _ in range(5):
+        delta, drive, g1, g2 = rng.uniform(0.1, 2.0, 4)
+        m, b = lb.affine_generator(delta * SX / 2 + drive * SZ / 2,
+                                   [np.sqrt(g1) * SM, np.sqrt(g2) * SZ])
+        cd = g1 / 2 + 2 * g2
+        # y = (rho00, Re rho01, Im rho01)
+        m_ref = np.array([[-g1, 0, -delta], [0, -cd, drive], [delta, -drive, -cd]])
+        b_ref = np.array([g1, 0, -delta / 2])
+        np.testing.assert_allclose(m, m_ref, atol=1e-14)
+        np.testing.assert_allclose(b, b_ref, atol=1e-14)
+        o0, o = lb.observable_form(np.diag([0.0, 1.0]))
+        assert o0 == 1 and np.allclose(o, [-1, 0, 0])
+
+
+def test_reduced_layout_round_trip():
+    rng = np.random.default_rng(2)
+    for n in (2, 3, 4):
+        a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+        rho = a @ a.conj().T
+        rho /= np.trace(rho)
+        y = lb.rho_to_reduced(rho)
+        assert y.size == n * n - 1
+        np.testing.assert_allclose(lb.reduced_to_rho(y), rho, atol=1e-14)
+
+
+def test_generator_matches_full_lindblad_map():
+    rng = np.random.default_rng(3)
+    n = 3
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    h = (a + a.conj().T) / 2
+    ops = [rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)) for _ in range(2)]
+    m, b = lb.affine_generator(h, ops)
+    r = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    rho = r @ r.conj().T
+    rho /= np.trace(rho)
+    y = lb.rho_to_reduced(rho)
+    np.testing.assert_allclose(m @ y + b, lb.coords(lb.lindblad(h, ops, rho)), atol=1e-12)
+
+
+def _const_model(n=3, seed=4):
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    h = (a + a.conj().T) / 2
+    op = np.zeros((n, n), complex)
+    op[0, 1] = 0.3
+    m, b = lb.affine_generator(h, [op])
+    return m, b
+
+
+def test_rk4_fourth_order_against_matrix_exponential():
+    m, b = _const_model()
+    d = m.shape[0]
+    aug = np.zeros((d + 1, d + 1))
+    aug[:d, :d], aug[:d, d] = m, b
+    y0 = np.zeros(d)
+    y0[0] = 1.0
+    exact = (scipy.linalg.expm(aug * 2.0) @ np.append(y0, 1.0))[:d]
+    errs = []
+    for steps in (40, 80, 160):
+        res, _ = integrate.rk4(lambda t, y: m @ y + b[:, None], y0[:, None], 2.0 / steps, steps,
+                               mode="final_rho")
+        errs.append(np.max(np.abs(res[0] - exact)))
+    ratios = [errs[i] / errs[i + 1] for i in range(2)]
+    assert all(14.0 < r < 18.0 for r in ratios), ratios
+
+
+def test_rk4_stage_times_commuting_hamiltonian():
+    # reference tests/test_gpu_solver.py:34-54: H = (1 + 0.4 sin t) sx/2, FP64, 80 steps
+    final_time = 1.7
+    m, b = lb.affine_generator(SX / 2, [])
+
+    def f(t, y):
+        return (1 + 0.4 * np.sin(t)) * (m @ y + b[:, None])
+    y0 = np.array([[1.0], [0.0], [0.0]])
+    res, _ = integrate.rk4(f, y0, final_time / 80, 80, mode="final_rho")
+    phase = final_time + 0.4 * (1.0 - np.cos(final_time))
+    assert abs(res[0, 0] - np.cos(0.5 * phase) ** 2) < 2e-7
+
+
+def test_rk4_time_offset_and_right_endpoint_mean():
+    # x' = t is integrated exactly by RK4: x(t_k) = ((t_in + t_k)^2 - t_in^2)/2;
+    # the mean is the average over the step endpoints after the warmup.
+    grid = np.linspace(0.0, 1.0, 65)
+    dt = grid[1]
+
+    def f(t, y):
+        return np.full_like(y, t)
+    res, _ = integrate.rk4(f, np.zeros((1, 1)), dt, 64, t_in=2.0, mode="final_rho")
+    assert abs(res[0, 0] - 2.5) < 1e-12          # reference tests/test_ode_api.py:350-355
+    obs = lambda t, y: y[0] + 0j  # noqa: E731
+    mean, _ = integrate.rk4(f, np.zeros((1, 1)), dt, 64, warmup_steps=16, obs=obs)
+    xs = grid[1:] ** 2 / 2
+    assert abs(mean[0].real - xs[16:].mean()) < 1e-12
+    full, _ = integrate.rk4(f, np.zeros((1, 1)), dt, 64, obs=obs)
+    assert abs(full[0].real - xs.mean()) < 1e-12   # reference tests/test_ode_api.py:366-368
+
+
+def test_rk4_trace_sampling_and_state_mean():
+    f = lambda t, y: -y  # noqa: E731
+    res, tr = integrate.rk4(f, np.ones((1, 2)), 1 / 128, 128, mode="mean_state",
+                            trace_stride=16, n_trace=8, trace_kind="state")
+    t_samples = (np.arange(8) * 16 + 1) / 128
+    np.testing.assert_allclose(tr[0, :, 0], np.exp(-t_samples), atol=1e-9)
+    np.testing.assert_allclose(res[0, 0], np.exp(-np.arange(1, 129) / 128).mean(), atol=1e-9)
+
+
+def test_installation_known_answers():
+    # reference gqis/check_environment.py:302-327
+    # Rabi: H = (A/2) sx, P(excited at pi) = sin^2(A pi / 2)
+    for amp in (1.0, 2.0):
+        m = models.AffineSweep([(models.one, amp / 2 * SX, [])], np.diag([0.0, 1.0]), [0.0], [0.0])
+        res, _ = integrate.rk4(m.f, m.ground_state(), np.pi / 256, 256, obs=m.obs, mode="final")
+        assert abs(res[0].real - np.sin(amp * np.pi / 2) ** 2) < 1e-4
+    # dissipation: pump sqrt(gamma)|1><0|, P = 1 - exp(-gamma t)
+    for gamma in (0.5, 2.0):
+        pump = np.sqrt(gamma) * np.array([[0, 0], [1, 0]], complex)
+        m = models.AffineSweep([(models.one, np.zeros((2, 2)), [pump])], np.diag([0.0, 1.0]),
+                               [0.0], [0.0])
+        res, _ = integrate.rk4(m.f, m.ground_state(), 1 / 128, 128, obs=m.obs, mode="final")
+        assert abs(res[0].real - (1 - np.exp(-gamma))) < 1e-5
+
+
+def test_tsit5a_closed_forms():
+    # reference tests/test_gpu_solver.py:269-310
+    m, b = lb.affine_generator(SX / 2, [])
+
+    def f(t, y, idx):
+        return (1 + 0.4 * np.sin(t)) * (m @ y + b[:, None])
+    res, _, acc, rej = integrate.tsit5a(f, np.array([[1.0], [0.0], [0.0]]), 1.7 / 4, 4,
+                                        rtol=1e-12, atol=1e-12, max_substeps=10000)
+    phase = 1.7 + 0.4 * (1.0 - np.cos(1.7))
+    assert abs(res[0, 0] - np.cos(0.5 * phase) ** 2) < 2e-7
+
+    def g(t, y, idx):
+        return m @ y + b[:, None]
+    obs = lambda t, y, idx: y[0] + 0j  # noqa: E731
+    val, _, acc, _ = integrate.tsit5a(g, np.array([[1.0], [0.0], [0.0]]), 2.0, 5, rtol=1e-10,
+                                      atol=1e-10, max_substeps=10000, warmup_steps=1, obs=obs,
+                                      mode="mean")
+    assert abs(val[0].real - (0.5 + (np.sin(10.0) - np.sin(2.0)) / 16.0)) < 5e-5
+    assert acc[0] > 5
+
+
+@pytest.mark.parametrize("method", ["tsit5a", "dop853a"])
+def test_adaptive_oracles_against_scipy(method):
+    d = models.Duffing(np.array([0.2, 0.45]), np.array([-1.0, 0.7]))
+    fn = integrate.tsit5a if method == "tsit5a" else integrate.dop853a
+    res, _, acc, rej = fn(d.f, d.y0(), 0.1, 100, rtol=1e-11, atol=1e-13, max_substeps=5000)
+    for p in range(2):
+        sol = solve_ivp(lambda t, y: d.f(t, y[:, None], np.array([p]))[:, 0], (0, 10),
+                        d.y0()[:, p], method="DOP853", rtol=1e-13, atol=1e-15)
+        np.testing.assert_allclose(res[p], sol.y[:, -1], atol=5e-8)
+    assert np.all(acc > 100)
+
+
+def test_adaptive_substep_limit_gives_nan():
+    d = models.Duffing(np.array([0.3]), np.array([1.0]))
+    res, _, acc, rej = integrate.tsit5a(d.f, d.y0(), 0.5, 4, rtol=1e-14, atol=1e-14,
+                                        max_substeps=2)
+    assert np.all(np.isnan(res))
+
+
+def test_c_oracle_matches_numpy_oracle():
+    from oracle import cbridge
+    m = models.four_level_chain(np.linspace(-10, 10, 4), np.linspace(0, 20, 3))
+    ref, _ = integrate.rk4(m.f, m.ground_state(), 0.05, 400, obs=m.obs, warmup_steps=100)
+    got = cbridge.rk4_affine(m, 0.05, 400, warmup=100, threads=2)
+    np.testing.assert_allclose(got, ref, atol=1e-13)
+    fin, _ = integrate.rk4(m.f, m.ground_state(), 0.05, 400, mode="final_rho")
+    np.testing.assert_allclose(cbridge.rk4_affine(m, 0.05, 400, mode="final_rho"), fin, atol=1e-13)
+
+
+def test_product_dop853_tableau_is_hairers():
+    from gqis import tableaux
+    c, a, b, e5, e3 = integrate.dop853_tableau()
+    t = tableaux.DOP853
+    np.testing.assert_array_equal(np.array(t["c"]), c)
+    np.testing.assert_array_equal(np.array(t["b"]), b)
+    np.testing.assert_array_equal(np.array(t["e5"]), e5)
+    np.testing.assert_allclose(np.array(t["e3"]), e3, atol=1e-17)
+    for s in range(12):
+        np.testing.assert_array_equal(np.array(t["a"][s]), a[s, :len(t["a"][s])])
+    np.testing.assert_array_equal(np.array(integrate.TSIT5_B), np.array(tableaux.TSIT5["b"]))
