@@ -72,7 +72,8 @@ def run_sweep(*, source, variant, spec, compiled, mode, var_arrays, rho0_values,
     natoms = len(compiled.plan.time_atoms)
     table = None
     use_table = bool(time_table) and not spec.adaptive and natoms > 0
-    rows = spec.stage_times_per_step * num_steps + 1
+    from .tableaux import rows_per_step
+    rows = rows_per_step(spec.name) * num_steps + 1
     if use_table:
         table = torch.empty(rows * natoms, dtype=rdt, device=dev)
     steps_out = (torch.zeros((npts, 2), dtype=torch.int32, device=dev)
@@ -104,7 +105,7 @@ def run_sweep(*, source, variant, spec, compiled, mode, var_arrays, rho0_values,
         if timer:
             timer.record_start()
         kname = codegen.kernel_name(spec.kernel, fp64)
-        if use_table:
+        if use_table and rows > 0:
             ctx.launch(module, codegen.kernel_name(table_kernel(spec), fp64), (rows + 255) // 256,
                        256, a, stream)
         if spec.adaptive and spec.options.get("persistent", True):

@@ -45,3 +45,104 @@ __device__ __forceinline__ void gq_store_complex(void* base, long long idx, real
     v.y = im;
     ((real2*)base)[idx] = v;
 }
+
+// Per-point output state of the fixed-step kernels (mean, state mean, trace) and its epilogue
+// (gqis/solver.py:2126-2193 output modes).
+struct gq_outputs {
+#if GQ_OUT_MEAN
+    acc_t re, im;
+#endif
+#if GQ_OUT_MEAN_STATE
+    acc_t s[GQ_D];
+#endif
+    bool trace_bad;
+};
+
+__device__ __forceinline__ void gq_out_init(gq_outputs& o) {
+#if GQ_OUT_MEAN
+    o.re = 0;
+    o.im = 0;
+#endif
+#if GQ_OUT_MEAN_STATE
+#pragma unroll
+    for (int i = 0; i < GQ_D; ++i) o.s[i] = 0;
+#endif
+    o.trace_bad = false;
+}
+
+// After step `step` (state y, stage values G at the step's end time).
+__device__ __forceinline__ void gq_out_step(gq_outputs& o, const gq_sweep_args& a, int step,
+                                            long long q, const real* __restrict__ y, real px,
+                                            real py, const real* __restrict__ C,
+                                            const gq_static_t& S, const gq_stage_t& G) {
+#if GQ_OUT_MEAN
+    if (step >= a.warmup_steps) {
+        real o_re, o_im;
+        gq_obs(y, px, py, C, S, G, o_re, o_im);
+        o.re += (acc_t)o_re;
+        o.im += (acc_t)o_im;
+    }
+#endif
+#if GQ_OUT_MEAN_STATE
+    if (step >= a.warmup_steps) {
+#pragma unroll
+        for (int i = 0; i < GQ_D; ++i) o.s[i] += (acc_t)y[i];
+    }
+#endif
+#if GQ_TRACE
+    if (step >= a.trace_offset && (step - a.trace_offset) % a.trace_stride == 0) {
+        const int kk = (step - a.trace_offset) / a.trace_stride;
+        if (kk < a.num_trace) {
+            const long long ld = a.trace_point_stride;
+#if GQ_TRACE == 1
+            real o_re, o_im;
+            gq_obs(y, px, py, C, S, G, o_re, o_im);
+            o.trace_bad |= !(gq_finite(o_re) && gq_finite(o_im));
+            gq_store_complex(a.trace, (long long)kk * ld + q, o_re, o_im);
+#else
+#pragma unroll
+            for (int i = 0; i < GQ_D; ++i) {
+                o.trace_bad |= !gq_finite(y[i]);
+                ((real*)a.trace)[((long long)kk * GQ_D + i) * ld + q] = y[i];
+            }
+#endif
+        }
+    }
+#endif
+}
+
+__device__ __forceinline__ void gq_out_final(gq_outputs& o, const gq_sweep_args& a, long long q,
+                                             const real* __restrict__ y, real px, real py,
+                                             const real* __restrict__ C, const gq_static_t& S,
+                                             const gq_stage_t& G) {
+    bool bad = false;
+#if GQ_OUT_MEAN
+    const acc_t inv_t = (acc_t)1 / (acc_t)(a.num_steps - a.warmup_steps);
+    const real r_re = (real)(o.re * inv_t), r_im = (real)(o.im * inv_t);
+    bad = !(gq_finite(r_re) && gq_finite(r_im));
+    gq_store_complex(a.result, q, r_re, r_im);
+#elif GQ_OUT_FINAL
+    real r_re, r_im;
+    gq_obs(y, px, py, C, S, G, r_re, r_im);
+    bad = !(gq_finite(r_re) && gq_finite(r_im));
+    gq_store_complex(a.result, q, r_re, r_im);
+#elif GQ_OUT_FINAL_RHO
+#pragma unroll
+    for (int i = 0; i < GQ_D; ++i) {
+        bad |= !gq_finite(y[i]);
+        ((real*)a.result)[(long long)i * a.point_count + q] = y[i];
+    }
+#elif GQ_OUT_MEAN_STATE
+    const acc_t inv_n = (acc_t)1 / (acc_t)(a.num_steps - a.warmup_steps);
+#pragma unroll
+    for (int i = 0; i < GQ_D; ++i) {
+        const real v = (real)(o.s[i] * inv_n);
+        bad |= !gq_finite(v);
+        ((real*)a.result)[(long long)i * a.point_count + q] = v;
+    }
+#endif
+    gq_flag(a.status, bad, GQ_STATUS_NONFINITE_RESULT);
+#if GQ_TRACE
+    gq_flag(a.status, o.trace_bad, GQ_STATUS_NONFINITE_TRACE);
+#endif
+}

@@ -73,17 +73,8 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_rk4)(const g
     real k[GQ_D];
     gq_rhs(y, px, py, C, S, G, k);
 
-#if GQ_OUT_MEAN
-    acc_t avg_re = 0, avg_im = 0;
-#endif
-#if GQ_OUT_MEAN_STATE
-    acc_t savg[GQ_D];
-#pragma unroll
-    for (int i = 0; i < GQ_D; ++i) savg[i] = 0;
-#endif
-#if GQ_TRACE
-    bool trace_bad = false;
-#endif
+    gq_outputs out;
+    gq_out_init(out);
 
     for (int step = 0; step < num_steps; ++step) {
         real t_mid = ((real)step + R(0.5)) * dt;
@@ -117,71 +108,8 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_rk4)(const g
             y[i] = fma(dt6, acc[i], y[i]);
         }
         if (step + 1 < num_steps) gq_rhs(y, px, py, C, S, G, k);
-
-#if GQ_OUT_MEAN
-        if (step >= a.warmup_steps) {
-            real o_re, o_im;
-            gq_obs(y, px, py, C, S, G, o_re, o_im);
-            avg_re += (acc_t)o_re;
-            avg_im += (acc_t)o_im;
-        }
-#endif
-#if GQ_OUT_MEAN_STATE
-        if (step >= a.warmup_steps) {
-#pragma unroll
-            for (int i = 0; i < GQ_D; ++i) savg[i] += (acc_t)y[i];
-        }
-#endif
-#if GQ_TRACE
-        if (step >= a.trace_offset && (step - a.trace_offset) % a.trace_stride == 0) {
-            const int kk = (step - a.trace_offset) / a.trace_stride;
-            if (kk < a.num_trace) {
-                const long long ld = a.trace_point_stride;
-#if GQ_TRACE == 1
-                real o_re, o_im;
-                gq_obs(y, px, py, C, S, G, o_re, o_im);
-                trace_bad |= !(gq_finite(o_re) && gq_finite(o_im));
-                gq_store_complex(a.trace, (long long)kk * ld + q, o_re, o_im);
-#else
-#pragma unroll
-                for (int i = 0; i < GQ_D; ++i) {
-                    trace_bad |= !gq_finite(y[i]);
-                    ((real*)a.trace)[((long long)kk * GQ_D + i) * ld + q] = y[i];
-                }
-#endif
-            }
-        }
-#endif
+        gq_out_step(out, a, step, q, y, px, py, C, S, G);
     }
 
-    bool bad = false;
-#if GQ_OUT_MEAN
-    const acc_t inv_t = (acc_t)1 / (acc_t)(num_steps - a.warmup_steps);
-    const real r_re = (real)(avg_re * inv_t), r_im = (real)(avg_im * inv_t);
-    bad = !(gq_finite(r_re) && gq_finite(r_im));
-    gq_store_complex(a.result, q, r_re, r_im);
-#elif GQ_OUT_FINAL
-    real r_re, r_im;
-    gq_obs(y, px, py, C, S, G, r_re, r_im);
-    bad = !(gq_finite(r_re) && gq_finite(r_im));
-    gq_store_complex(a.result, q, r_re, r_im);
-#elif GQ_OUT_FINAL_RHO
-#pragma unroll
-    for (int i = 0; i < GQ_D; ++i) {
-        bad |= !gq_finite(y[i]);
-        ((real*)a.result)[(long long)i * a.point_count + q] = y[i];
-    }
-#elif GQ_OUT_MEAN_STATE
-    const acc_t inv_n = (acc_t)1 / (acc_t)(num_steps - a.warmup_steps);
-#pragma unroll
-    for (int i = 0; i < GQ_D; ++i) {
-        const real v = (real)(savg[i] * inv_n);
-        bad |= !gq_finite(v);
-        ((real*)a.result)[(long long)i * a.point_count + q] = v;
-    }
-#endif
-    gq_flag(a.status, bad, GQ_STATUS_NONFINITE_RESULT);
-#if GQ_TRACE
-    gq_flag(a.status, trace_bad, GQ_STATUS_NONFINITE_TRACE);
-#endif
+    gq_out_final(out, a, q, y, px, py, C, S, G);
 }

@@ -46,7 +46,7 @@ _SPECS = {
     "lserk4": SolverSpec("lserk4", "LSRK4", 4, 5, "fixed_tableau.hip", "gq_fixed"),
     "lserk4f": SolverSpec("lserk4f", "LSRK4F", 4, 5, "fixed_tableau.hip", "gq_fixed"),
     "dp5": SolverSpec("dp5", "DP5", 5, 6, "fixed_tableau.hip", "gq_fixed"),
-    "ab5": SolverSpec("ab5", "AB5", 5, 1, "fixed_ab5.hip", "gq_ab5"),
+    "ab5": SolverSpec("ab5", "AB5", 5, 1, "fixed_tableau.hip", "gq_fixed"),
     "anas5": SolverSpec("anas5", "Anas5", 5, 6, "fixed_tableau.hip", "gq_fixed"),
     "tsit5": SolverSpec("tsit5", "Tsit5", 5, 6, "fixed_tableau.hip", "gq_fixed"),
     "alshina6": SolverSpec("alshina6", "Alshina6", 6, 7, "fixed_tableau.hip", "gq_fixed"),

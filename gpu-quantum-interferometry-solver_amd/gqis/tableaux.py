@@ -125,10 +125,20 @@ LSRK4 = {
           2006345519317 / 3224310063776, 2802321613138 / 2924317926251),
 }
 
-# Anas5(w): 6-stage tableau; a43 is replaced at run time by the fitted coefficient
-# (solver_param[0]) -- the default value is the w = 0 limit.
+# Anas5(w): 6-stage fitted tableau.  Row 6 depends linearly on the fitted coefficient p
+# (solver_param[0], evaluated on the host from w*dt): a6j = a0_6j + p * a1_6j.
 ANAS5 = {
-    "c": (0.0, 0.2, 0.4, 0.6, 0.8, 1.0),
+    "c": (0.0, 0.1, 1 / 3, 2 / 3, 0.9, 1.0),
+    "a": (
+        (),
+        (1 / 10,),
+        (-2 / 9, 5 / 9),
+        (28 / 9, -40 / 9, 2.0),
+        (-11277 / 8000, 171 / 80, -459 / 2000, 3213 / 8000),
+        (-4.0, 5.0, 0.0, 0.0, 0.0),
+    ),
+    "a1": {5: (-0.595, 0.0, 1.89, -2.295, 1.0)},
+    "b": (23 / 216, 0.0, 63 / 136, 9 / 56, 1000 / 3213, -1 / 24),
 }
 
 
@@ -155,9 +165,74 @@ def emit_arrays(tab, prefix="GQ"):
     return "\n".join(lines)
 
 
-FIXED = {"tsit5": TSIT5, "dp5": DP5, "alshina6": ALSHINA6, "dop853": DOP853}
+FIXED = {"tsit5": TSIT5, "dp5": DP5, "alshina6": ALSHINA6, "dop853": DOP853, "anas5": ANAS5}
+AB5_START_C = (0.5, 1.0)   # RK4 start-up rows: midpoint, end
 
 
-def tableau_defines(name):
-    """Preprocessor settings of the tableau-driven fixed-step template."""
-    return {"GQ_FIXED_METHOD": f'"{name}"'}
+def _rows(cvals):
+    """Distinct non-zero stage times of one step (first appearance), always ending with 1."""
+    rows = []
+    for c in cvals:
+        if c != 0.0 and all(abs(c - r) > 5e-14 for r in rows):
+            rows.append(c)
+    if all(abs(1.0 - r) > 5e-14 for r in rows):
+        rows.append(1.0)
+    return rows
+
+
+def _row_of(c, rows):
+    if c == 0.0:
+        return -1
+    return min(range(len(rows)), key=lambda i: abs(rows[i] - c))
+
+
+def emit_fixed(name):
+    """Constants of the fixed-step template for solver ``name`` (scheme, tableau, time rows)."""
+    lines = []
+    if name in FIXED:
+        tab = FIXED[name]
+        n = len(tab["c"])
+        rows = _rows(tab["c"])
+        lines.append("#define GQ_SCHEME 0")
+        lines.append(emit_arrays(tab))
+        a1 = tab.get("a1", {})
+        mat = []
+        for s in range(n):
+            row = list(a1.get(s, ())) + [0.0] * (n - len(a1.get(s, ())))
+            mat.append("{" + ", ".join(_fmt(v) for v in row) + "}")
+        lines.append(f"static constexpr double GQ_A1[{n}][{n}] = {{{', '.join(mat)}}};")
+        row_of = [_row_of(c, rows) for c in tab["c"]]
+    elif name in ("lserk4", "lserk4f"):
+        n = len(LSRK4["c"])
+        rows = _rows(LSRK4["c"])
+        lines.append("#define GQ_SCHEME 1")
+        lines.append(f"#define GQ_NSTAGES {n}")
+        lines.append(f"static constexpr double GQ_A[{n}][1] = "
+                     f"{{{', '.join('{' + _fmt(v) + '}' for v in LSRK4['a'])}}};")
+        lines.append(f"static constexpr double GQ_B[{n}] = {{{', '.join(_fmt(v) for v in LSRK4['b'])}}};")
+        row_of = [_row_of(c, rows) for c in LSRK4["c"]]
+    elif name == "ab5":
+        rows = list(AB5_START_C)
+        lines.append("#define GQ_SCHEME 2")
+        lines.append("#define GQ_NSTAGES 1")
+        row_of = [-1]
+    else:
+        raise ValueError(f"no fixed-step tableau for {name}")
+    end_row = _row_of(1.0, rows)
+    lines.append(f"#define GQ_NROWS {len(rows)}")
+    lines.append(f"#define GQ_END_ROW {end_row}")
+    lines.append(f"static constexpr double GQ_ROW_C[{len(rows)}] = {{{', '.join(_fmt(v) for v in rows)}}};")
+    lines.append(f"static constexpr int GQ_ROW_OF[{len(row_of)}] = {{{', '.join(str(v) for v in row_of)}}};")
+    return "\n".join(lines)
+
+
+def rows_per_step(name):
+    if name == "rk4":
+        return 2
+    if name in FIXED:
+        return len(_rows(FIXED[name]["c"]))
+    if name in ("lserk4", "lserk4f"):
+        return len(_rows(LSRK4["c"]))
+    if name == "ab5":
+        return len(AB5_START_C)
+    return 0

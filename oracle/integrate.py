@@ -317,3 +317,162 @@ def dop853a(f, y0, dt_out, num_steps, *, rtol=1e-8, atol=1e-10, max_substeps=409
     else:
         result = y.T.copy()
     return result, trace, accepted, rejected
+
+
+# ---------------------------------------------------------------------------------------------
+# Other fixed-step methods (reference gqis/cuda_solvers.py:48-702): explicit tableau RK with the
+# k1 of a step taken from the end of the previous one, the 2N-storage LSRK4 and Adams-Bashforth 5
+# with RK4 start-up.  Stage times t = step*dt + c*dt; outputs after each step as in rk4().
+def fixed_tableau(name, step_size=None, frequency=None):
+    """(c, a, b) of a fixed-step method; sources: SciPy (DP5 = RK45, DOP853), Tsitouras 2011,
+    Alshina et al. 2008 and Anastassi & Simos (fitted coefficient from w*dt)."""
+    if name == "dp5":
+        from scipy.integrate._ivp.rk import RK45
+        return RK45.C, RK45.A, RK45.B
+    if name == "dop853":
+        c, a, b, _, _ = dop853_tableau()
+        return c, a, b
+    if name == "tsit5":
+        a = np.zeros((6, 6))
+        for s, row in enumerate(TSIT5_A):
+            a[s, :len(row)] = row
+        return np.array(TSIT5_C), a, np.array(TSIT5_B)
+    if name == "alshina6":
+        c = np.array([0.0, 4 / 7, 5 / 7, 6 / 7, (5 - np.sqrt(5)) / 10, (5 + np.sqrt(5)) / 10, 1.0])
+        rows = [(), (4 / 7,), (115 / 112, -5 / 16), (589 / 630, 5 / 18, -16 / 45),
+                (229 / 1200 - 29 * np.sqrt(5) / 6000, 119 / 240 - 187 * np.sqrt(5) / 1200,
+                 -14 / 75 + 34 * np.sqrt(5) / 375, -3 * np.sqrt(5) / 100),
+                (71 / 2400 - 587 * np.sqrt(5) / 12000, 187 / 480 - 391 * np.sqrt(5) / 2400,
+                 -38 / 75 + 26 * np.sqrt(5) / 375, 27 / 80 - 3 * np.sqrt(5) / 400,
+                 (1 + np.sqrt(5)) / 4),
+                (-49 / 480 + 43 * np.sqrt(5) / 160, -425 / 96 + 51 * np.sqrt(5) / 32,
+                 52 / 15 - 4 * np.sqrt(5) / 5, -27 / 16 + 3 * np.sqrt(5) / 16,
+                 5 / 4 - 3 * np.sqrt(5) / 4, 5 / 2 - np.sqrt(5) / 2)]
+        a = np.zeros((7, 7))
+        for s, row in enumerate(rows):
+            a[s, :len(row)] = row
+        return c, a, np.array([1 / 12, 0, 0, 0, 5 / 12, 5 / 12, 1 / 12])
+    if name == "anas5":
+        p = anas5_parameter(frequency, step_size)
+        c = np.array([0.0, 0.1, 1 / 3, 2 / 3, 0.9, 1.0])
+        a = np.zeros((6, 6))
+        a[1, 0] = 0.1
+        a[2, :2] = (-2 / 9, 5 / 9)
+        a[3, :3] = (28 / 9, -40 / 9, 2.0)
+        a[4, :4] = (-11277 / 8000, 171 / 80, -459 / 2000, 3213 / 8000)
+        a[5, :5] = (-4.0 - 0.595 * p, 5.0, 1.89 * p, -2.295 * p, p)
+        return c, a, np.array([23 / 216, 0.0, 63 / 136, 9 / 56, 1000 / 3213, -1 / 24])
+    raise ValueError(name)
+
+
+def anas5_parameter(frequency, step_size):
+    """Fitted Anas5 coefficient from v = w*dt (reference gqis/cuda_solvers.py:971-990)."""
+    import mpmath as mp
+    w = 1.0 if frequency is None else float(frequency)
+    with mp.workdps(60):
+        v = mp.mpf(str(w)) * mp.mpf(str(step_size))
+        if v == 0:
+            return float(mp.mpf(-800) / 1071)
+        tn = mp.tan(v)
+        num = -2 * v ** 5 + 6 * tn * v ** 4 + 24 * v ** 3 - 72 * tn * v ** 2 - 144 * v + 144 * tn
+        den = v ** 5 * (2 * tn * v + 12 - 20)
+        return float((-mp.mpf(8000) / 1071) * num / den)
+
+
+LSRK4_A = (0.0, -567301805773 / 1357537059087, -2404267990393 / 2016746695238,
+           -3550918686646 / 2091501179385, -1275806237668 / 842570457699)
+LSRK4_B = (1432997174477 / 9575080441755, 5161836677717 / 13612068292357,
+           1720146321549 / 2090206949498, 3134564353537 / 4481467310338,
+           2277821191437 / 14882151754819)
+LSRK4_C = (0.0, 1432997174477 / 9575080441755, 2526269341429 / 6820363962896,
+           2006345519317 / 3224310063776, 2802321613138 / 2924317926251)
+AB5_BETA = (1901 / 720, -1387 / 360, 109 / 30, -637 / 360, 251 / 720)
+
+
+def fixed_step(name, f, y0, dt, num_steps, *, t_in=0.0, warmup_steps=0, obs=None, mode="mean",
+               trace_stride=1, trace_offset=0, n_trace=0, trace_kind="obs", frequency=None):
+    """Any fixed-step solver name of the reference (rk4 goes through rk4())."""
+    if name == "rk4":
+        return rk4(f, y0, dt, num_steps, t_in=t_in, warmup_steps=warmup_steps, obs=obs, mode=mode,
+                   trace_stride=trace_stride, trace_offset=trace_offset, n_trace=n_trace,
+                   trace_kind=trace_kind)
+    y = np.array(y0, dtype=np.float64, copy=True)
+    npts = y.shape[1]
+    if mode == "mean" and warmup_steps >= num_steps:
+        mode = "final"
+    if mode == "mean_state" and warmup_steps >= num_steps:
+        mode = "final_rho"
+    avg = np.zeros(npts, complex)
+    state_avg = np.zeros_like(y)
+    trace = None
+    if n_trace:
+        shape = (npts, n_trace) if trace_kind == "obs" else (npts, n_trace, y.shape[0])
+        trace = np.zeros(shape, complex if trace_kind == "obs" else np.float64)
+    t_end = 0.0
+    hist = []
+    if name not in ("lserk4", "lserk4f"):
+        k1 = f(t_in, y)
+        hist = [k1]
+    if name in FIXED_RK:
+        c, a, b = fixed_tableau(name, step_size=dt, frequency=frequency)
+    for step in range(num_steps):
+        t = step * dt
+        t_end = t + dt
+        if name in FIXED_RK:
+            ks = [k1]
+            for s in range(1, len(c)):
+                ys = y + dt * sum(a[s, j] * ks[j] for j in range(s) if a[s, j] != 0.0)
+                ks.append(f(t_in + t + c[s] * dt, ys))
+            y = y + dt * sum(b[j] * ks[j] for j in range(len(c)) if b[j] != 0.0)
+            if step + 1 < num_steps:
+                k1 = f(t_in + t_end, y)
+        elif name in ("lserk4", "lserk4f"):
+            r = None
+            for s in range(5):
+                k = f(t_in + t + LSRK4_C[s] * dt, y) if s else f(t_in + (t if step else 0.0), y)
+                r = dt * k if s == 0 else LSRK4_A[s] * r + dt * k
+                y = y + LSRK4_B[s] * r
+        elif name == "ab5":
+            if step < 4:
+                k = hist[0]
+                tmp = y + 0.5 * dt * k
+                acc = k.copy()
+                k = f(t_in + t + 0.5 * dt, tmp)
+                tmp = y + 0.5 * dt * k
+                acc = acc + 2 * k
+                k = f(t_in + t + 0.5 * dt, tmp)
+                tmp = y + dt * k
+                acc = acc + 2 * k
+                k = f(t_in + t_end, tmp)
+                acc = acc + k
+                y = y + dt / 6 * acc
+            else:
+                y = y + dt * sum(beta * hist[j] for j, beta in enumerate(AB5_BETA))
+            if step + 1 < num_steps:
+                hist = [f(t_in + t_end, y)] + hist[:4]
+        else:
+            raise ValueError(name)
+        if step >= warmup_steps:
+            if mode == "mean":
+                avg = avg + obs(t_in + t_end, y)
+            elif mode == "mean_state":
+                state_avg = state_avg + y
+        if trace is not None:
+            slot = _trace_slot(step, trace_stride, trace_offset, n_trace)
+            if slot is not None:
+                if trace_kind == "obs":
+                    trace[:, slot] = obs(t_in + t_end, y)
+                else:
+                    trace[:, slot, :] = y.T
+    if mode == "mean":
+        result = avg * (1.0 / (num_steps - warmup_steps))
+    elif mode == "final":
+        result = obs(t_in + t_end, y)
+    elif mode == "final_rho":
+        result = y.T.copy()
+    else:
+        result = (state_avg / (num_steps - warmup_steps)).T.copy()
+    return result, trace
+
+
+FIXED_RK = ("dp5", "tsit5", "alshina6", "dop853", "anas5")

@@ -1,0 +1,63 @@
+"""Every fixed-step solver name on the GPU: the reference's closed-form tests and parity with the
+FP64 oracle."""
+import numpy as np
+import pytest
+import sympy as sp
+
+from gqis import mesolve_2D, odesolve_2D, workloads
+from oracle import integrate, models
+
+pytestmark = pytest.mark.gpu
+DUMMY = sp.Symbol("dummy", real=True)
+FIXED = ("rk4", "lserk4", "lserk4f", "dp5", "ab5", "anas5", "tsit5", "alshina6", "dop853")
+
+
+@pytest.mark.parametrize("solver", FIXED + ("tsit5a", "dop853a"))
+def test_pi_rotation(gpu, solver):
+    # reference tests/test_gpu_solver.py:16-31 (FP32)
+    rho = mesolve_2D(sp.Matrix([[0, sp.Rational(1, 2)], [sp.Rational(1, 2), 0]]), sp.Integer(0),
+                     [], sp.eye(2), np.linspace(0.0, np.pi, 101, dtype=np.float32),
+                     var_arrays={DUMMY: np.zeros(1, np.float32)},
+                     rho0=sp.Matrix([[1, 0], [0, 0]]), output_mode="final_rho", solver=solver)
+    assert abs(float(rho[0, 0, 0])) < 2.0e-5
+
+
+@pytest.mark.parametrize("solver", FIXED)
+def test_commuting_time_dependent_hamiltonian(gpu, solver):
+    # reference tests/test_gpu_solver.py:34-54 (FP64)
+    drive, t = sp.Symbol("Drive", real=True), sp.Symbol("t", real=True)
+    rho = mesolve_2D((1 + drive) * sp.Matrix([[0, 1], [1, 0]]) / 2,
+                     {drive: sp.Rational(2, 5) * sp.sin(t)}, [], sp.eye(2),
+                     np.linspace(0.0, 1.7, 81), var_arrays={DUMMY: np.zeros(1)},
+                     output_mode="final_rho", solver=solver, fp64=True)
+    phase = 1.7 + 0.4 * (1.0 - np.cos(1.7))
+    assert abs(float(rho[0, 0, 0]) - np.cos(0.5 * phase) ** 2) < 2e-7
+
+
+@pytest.mark.parametrize("solver", FIXED)
+def test_ode_absolute_time_and_analytic(gpu, solver):
+    # reference tests/test_ode_api.py:350-365 (FP32)
+    x, y, t = sp.symbols("x y t", real=True)
+    res = odesolve_2D([t], [x], [0], np.linspace(0, 1, 65), t_in=2.0, solver=solver)
+    np.testing.assert_allclose(res, 2.5, atol=2e-5)
+    grid = np.linspace(0, 1, 129)
+    final, trace, times = odesolve_2D([-x, t], [x, y], [1, 0], grid, solver=solver,
+                                      return_time_trace=True, time_trace_every=16)
+    np.testing.assert_allclose(final[0, 0], [np.exp(-1), 0.5], atol=2e-5)
+    np.testing.assert_allclose(trace[0, 0, :, 0], np.exp(-times), atol=2e-5)
+    np.testing.assert_allclose(trace[0, 0, :, 1], times ** 2 / 2, atol=2e-5)
+
+
+@pytest.mark.parametrize("solver", FIXED)
+@pytest.mark.parametrize("time_table", [True, False])
+def test_four_level_parity_fp64(gpu, solver, time_table):
+    w = workloads.four_level(n_eps=6, n_amp=5, n_t=801, t_end=40.0)
+    got = mesolve_2D(w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"],
+                     var_arrays=w["var_arrays"], fp64=True, solver=solver, warmup_time=0.25,
+                     time_table=time_table, solver_frequency=1.0)
+    eps_v, amp_v = w["var_arrays"].values()
+    m = models.four_level_chain(eps_v, amp_v)
+    ref, _ = integrate.fixed_step(solver, m.f, m.ground_state(), 0.05, 800, obs=m.obs,
+                                  warmup_steps=200, frequency=1.0)
+    ref = ref.reshape(6, 5)
+    assert np.max(np.abs(got - ref)) <= 1e-10 * np.max(np.abs(ref))

@@ -199,3 +199,38 @@ def test_product_dop853_tableau_is_hairers():
     for s in range(12):
         np.testing.assert_array_equal(np.array(t["a"][s]), a[s, :len(t["a"][s])])
     np.testing.assert_array_equal(np.array(integrate.TSIT5_B), np.array(tableaux.TSIT5["b"]))
+
+
+@pytest.mark.parametrize("name,order", [("rk4", 4), ("lserk4", 4), ("dp5", 5), ("tsit5", 5),
+                                        ("alshina6", 6), ("ab5", 5), ("anas5", 5)])
+def test_fixed_step_oracles_have_their_order(name, order):
+    """Convergence order on a time-dependent dissipative model pins each tableau and the
+    stage-time semantics (a wrong coefficient or stage time drops the order)."""
+    m, b = _const_model()
+    y0 = np.zeros((m.shape[0], 1))
+    y0[0] = 1.0
+
+    def f(t, y):
+        return (1 + 0.3 * np.sin(t)) * (m @ y + b[:, None])
+    sol = solve_ivp(lambda t, y: (1 + 0.3 * np.sin(t)) * (m @ y + b), (0, 2), y0[:, 0],
+                    method="DOP853", rtol=1e-13, atol=1e-15)
+    errs = []
+    for steps in (20, 40, 80):
+        r, _ = integrate.fixed_step(name, f, y0, 2.0 / steps, steps, mode="final_rho")
+        errs.append(np.max(np.abs(r[0] - sol.y[:, -1])))
+    observed = np.log2(errs[1] / errs[2])
+    assert order - 0.3 < observed < order + 1.0, (name, errs)
+
+
+def test_dop853_fixed_is_eighth_order():
+    m, b = _const_model()
+    f = lambda t, y: (m @ y + b[:, None])  # noqa: E731
+    d = m.shape[0]
+    aug = np.zeros((d + 1, d + 1))
+    aug[:d, :d], aug[:d, d] = m, b
+    y0 = np.zeros(d)
+    y0[0] = 1.0
+    exact = (scipy.linalg.expm(aug * 4.0) @ np.append(y0, 1.0))[:d]
+    errs = [np.max(np.abs(integrate.fixed_step("dop853", f, y0[:, None], 4.0 / n, n,
+                                               mode="final_rho")[0][0] - exact)) for n in (4, 8)]
+    assert np.log2(errs[0] / errs[1]) > 7.0
