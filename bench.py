@@ -65,6 +65,18 @@ def run_sweeps(w, fp64, k, time_table=True):
     return res, info, lat
 
 
+def measured_traffic():
+    """HBM bytes per launch of the headline kernel from the committed rocprofv3 counter runs
+    (profiles/pmc_gq_rk4_four_level_2048_fp64.json: FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB
+    copy), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_gq_rk4_four_level_2048_fp64.json")
+    try:
+        with open(path) as fh:
+            return json.load(fh)["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline(seconds_budget=15.0):
     """FP64 C oracle (RK4, OpenMP over points) on a bounded sample of the same workload:
     whole eps columns (2048 A values each) at the full 2000 steps, as many columns as fit the
@@ -99,6 +111,11 @@ def main():
     ap.add_argument("--grid", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--profile", choices=["fp64", "fp32", "plain"], default=None,
+                    help="run only `--steps` sweeps of one form (for rocprofv3), no JSON line")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="with --profile: also run one device copy of this many bytes "
+                         "(the counter calibration kernel)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the sweep shards by parameter rows with no "
@@ -114,6 +131,19 @@ def main():
     nsteps = len(w["tlist"]) - 1
     compiled, _ = _compile_rhs(w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"],
                                sweep_symbols=tuple(w["var_arrays"]))
+    if args.profile:
+        fp64 = args.profile != "fp32"
+        table = args.profile != "plain"
+        run_sweeps(w, fp64, 1, time_table=table)          # compile + warm
+        _, info, _ = run_sweeps(w, fp64, args.steps, time_table=table)
+        log(f"profile {args.profile}: kernel {np.mean([i['gpu_kernel_s'] for i in info]):.6f} s")
+        if args.copy_bytes:
+            src = torch.empty(args.copy_bytes // 4, dtype=torch.float32, device="cuda")
+            dst = torch.empty_like(src)
+            dst.copy_(src)
+            torch.cuda.synchronize()
+            log(f"copy kernel: {args.copy_bytes} bytes read and written")
+        return
     flops_ps, rhs_flops = algorithmic_flops_per_point_step(compiled)
 
     log("FP64 reference sweep ...")
@@ -184,7 +214,8 @@ def main():
                       "nonfinite_points": int(np.sum(~finite)),
                       "meets_1e-6": bool(max_err <= 1e-6)},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS["fp64"],
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["fp64"], "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["fp64"],
+                     "traffic": measured_traffic() if args.grid == 2048 else None,
                      "flops_per_point_step": flops_ps, "kernel": "gq_rk4_f64"},
     }
     if not args.no_cpu:

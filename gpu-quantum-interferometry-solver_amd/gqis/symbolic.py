@@ -192,8 +192,9 @@ class RHSPlan:
 class _Hoister:
     """Replaces state-free sub-expressions by static or stage symbols (deduplicated)."""
 
-    def __init__(self, dynamic):
+    def __init__(self, dynamic, force=()):
         self.dynamic = set(dynamic)     # symbols that make an expression non-hoistable
+        self.force = set(force)         # bare symbols that still need a hoisted value (time)
         self.values = {}                # expr -> symbol
         self.order = []                 # [(symbol, expr)]
         self.count = 0
@@ -209,7 +210,9 @@ class _Hoister:
 
     def __call__(self, expr, prefix):
         if not (expr.free_symbols & self.dynamic):
-            if expr.is_Number or expr.is_Symbol or not expr.free_symbols:
+            if expr.is_Number or not expr.free_symbols:
+                return expr
+            if expr.is_Symbol and expr not in self.force:
                 return expr
             return self._new(expr, prefix)
         if expr.is_Add or expr.is_Mul:
@@ -218,7 +221,8 @@ class _Hoister:
             new_dyn = [self(a, prefix) for a in dyn]
             if free:
                 lump = expr.func(*free)
-                if not (lump.is_Number or lump.is_Symbol or not lump.free_symbols):
+                if not (lump.is_Number or not lump.free_symbols
+                        or (lump.is_Symbol and lump not in self.force)):
                     lump = self._new(lump, prefix)
                 return expr.func(lump, *new_dyn)
             return expr.func(*new_dyn)
@@ -309,7 +313,7 @@ def make_plan(equations, obs_re, obs_im, states, drive_map, const_syms, *,
                                                           for k, c in coeffs.items()]))
 
     # 2. Hoist state-free parts: they become "H" values (static or stage, decided below).
-    hoist = _Hoister(stset)
+    hoist = _Hoister(stset, force={time_symbol})
     rhs = [hoist(e, "H") for e in decomposed]
     obs_h = [hoist(e, "H") for e in obs_dec]
 

@@ -176,3 +176,15 @@ def test_hiprtc_compiles_generated_kernels_for_gfx950():
             src = _emit_source(c, get_solver_spec(name, step_size=0.1), mode, False, True, False,
                                fp64, True, 256, False)
             assert runtime.compile_check(src, codegen.HIPRTC_BASE_OPTIONS) > 1000
+
+
+def test_bare_time_in_an_equation_becomes_a_time_atom():
+    x, t = sp.symbols("x t", real=True)
+    from gqis.gqis_ode_api import _prepare_ode_model
+    from gqis.solver import _compile_rhs
+    model = _prepare_ode_model([t], [x], [0], None, None, None, None, None, None)
+    c, _ = _compile_rhs(model["H"], {}, [], model["mean"], sweep_symbols=tuple(model["sweeps"]),
+                        rho0=model["initial"], ode_size=1)
+    assert [str(e) for _, e in c.plan.time_atoms] == ["t"]
+    got, _ = _eval_plan(c.plan, 0.0, 0.0, 2.25, [0.3])
+    assert got[0] == 2.25
