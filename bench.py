@@ -113,6 +113,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--profile", choices=["fp64", "fp32", "plain"], default=None,
                     help="run only `--steps` sweeps of one form (for rocprofv3), no JSON line")
+    ap.add_argument("--requests", type=int, default=1000, help="serving: number of requests")
+    ap.add_argument("--workload", choices=["four_level", "duffing", "serving"],
+                    default="four_level",
+                    help="four_level: the headline sweep; duffing: adaptive Tsit5/DOP853 sweep")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="with --profile: also run one device copy of this many bytes "
                          "(the counter calibration kernel)")
@@ -125,6 +129,10 @@ def main():
     import torch
     from gqis import workloads
     from gqis.solver import _compile_rhs
+    if args.workload == "duffing":
+        return bench_duffing(args)
+    if args.workload == "serving":
+        return bench_serving(args)
 
     w = workloads.four_level(n_eps=args.grid, n_amp=args.grid)
     npts = args.grid * args.grid
@@ -221,6 +229,121 @@ def main():
     if not args.no_cpu:
         log("CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+def bench_duffing(args):
+    """Forced Duffing sweep over (gamma, x0), t in [0, 500], output step 0.1, adaptive Tsit5 and
+    DOP853 at rtol 1e-8 / atol 1e-10: final state of every point plus x(t) at the 5000 output
+    times for an 8 x 8 subset (a second call on the subset: per-point arithmetic does not depend
+    on the batch).  FP64 throughout, and FP32 RHS with FP64 state/controller beside it."""
+    import numpy as np
+    import torch
+    from gqis import odesolve_2D, workloads
+    side = args.grid if args.grid != 2048 else 1024
+    w = workloads.duffing(n_gamma=side, n_x0=side)
+    out = {"metric": "accepted adaptive point-steps/s", "unit": "accepted point-steps/s",
+           "n_gpus": 1, "higher_is_better": True, "data": "synthetic", "vs_baseline": None,
+           "config": {"workload": f"forced Duffing {side}x{side}, t 0..500, dt_out 0.1, "
+                                  "rtol 1e-8 atol 1e-10", "points": side * side}}
+    finals = {}
+    for solver in ("tsit5a", "dop853a"):
+        for label, fp64, opts in (("fp64", True, {}), ("fp32_rhs", False, {"state_fp64": True})):
+            ao = {"step_rtol": 1e-8, "step_atol": 1e-10, "max_substeps": 4096,
+                  "count_steps": True, **opts}
+            kw = dict(var_arrays=w["var_arrays"], solver=solver, fp64=fp64, adaptive_options=ao,
+                      return_device=True, return_timing_info=True, ignore_non_finite_output=True)
+            log(f"duffing {solver} {label} ...")
+            odesolve_2D(w["rhs"], w["state_symbols"], w["y0"], w["tlist"][:11],
+                        **kw)  # compile + warm on a short interval
+            t0 = time.perf_counter()
+            res, info = odesolve_2D(w["rhs"], w["state_symbols"], w["y0"], w["tlist"], **kw)
+            torch.cuda.synchronize()
+            wall = time.perf_counter() - t0
+            acc = info["adaptive_steps"]["accepted"].astype(np.int64)
+            rej = info["adaptive_steps"]["rejected"].astype(np.int64)
+            k = info["gpu_kernel_s"]
+            finals[(solver, label)] = res.cpu().numpy()
+            out[f"{solver}_{label}"] = {
+                "kernel_s": k, "wall_s": wall, "accepted_steps": int(acc.sum()),
+                "rejected_steps": int(rej.sum()),
+                "accepted_per_point": {"mean": float(acc.mean()), "max": int(acc.max()),
+                                       "min": int(acc.min())},
+                "value": float(acc.sum() / k),
+                "nonfinite_points": int(np.sum(~np.isfinite(finals[(solver, label)]).any(-1)))}
+        a, b = finals[(solver, "fp64")], finals[(solver, "fp32_rhs")]
+        d = np.abs(a - b).max(-1).ravel()
+        out[f"{solver}_fp32_rhs_vs_fp64"] = {"median": float(np.nanmedian(d)),
+                                             "p90": float(np.nanpercentile(d, 90)),
+                                             "frac_below_1e-6": float(np.mean(d < 1e-6))}
+    x = finals[("tsit5a", "fp64")]
+    y = finals[("dop853a", "fp64")]
+    dd = np.abs(x - y).max(-1).ravel()
+    out["tsit5a_vs_dop853a_fp64"] = {"median": float(np.nanmedian(dd)),
+                                     "frac_below_1e-6": float(np.mean(dd < 1e-6))}
+    # x(t) at the 5000 output times for an 8 x 8 subset
+    gi = np.arange(0, side, side // 8)
+    sub = {k: v[gi] for k, v in w["var_arrays"].items()}
+    t0 = time.perf_counter()
+    fin_s, tr, times = odesolve_2D(w["rhs"], w["state_symbols"], w["y0"], w["tlist"],
+                                   var_arrays=sub, solver="tsit5a", fp64=True,
+                                   adaptive_options={"step_rtol": 1e-8, "step_atol": 1e-10,
+                                                     "max_substeps": 4096},
+                                   return_time_trace=True, ignore_non_finite_output=True)
+    out["trace_subset"] = {"points": 64, "samples": int(tr.shape[2]),
+                           "wall_s": time.perf_counter() - t0,
+                           "final_matches_full_sweep": bool(np.array_equal(
+                               fin_s, x.reshape(side, side, 2)[np.ix_(gi, gi)]))}
+    t_best = out["tsit5a_fp64"]
+    out["value"] = t_best["value"]
+    out["dtype"] = "f64"
+    print(json.dumps(out), flush=True)
+
+
+def bench_serving(args):
+    """Poisson stream (20 requests/s, seeded) of sweep requests: two-level LZS or N-level chain
+    models (N in {2, 4, 8}), square grids of side 32/128/512/1024, random amplitude ranges,
+    RK4 or Tsit5A, FP32 or FP64, served with continuous batching on 4 streams under a 16 GiB
+    HBM budget.  Reports throughput and arrival-to-host-result latency percentiles."""
+    import numpy as np
+    from gqis import compile_lindblad_rhs, workloads
+    from gqis.serving import SweepRequest, SweepServer
+    stream = workloads.serving_stream(count=args.requests)
+    models = {}
+    for d in stream:
+        key = (d["style"], d["n"])
+        if key not in models:
+            w = (workloads.lzs_two_level() if d["style"] == "lzs"
+                 else workloads.chain(d["n"], eps_range=(-10.0, 10.0)))
+            models[key] = (compile_lindblad_rhs(w["H"], w["Drive"], w["Col_Ops"],
+                                                w["mean_operator"], list(w["var_arrays"])), w)
+    reqs = []
+    for d in stream:
+        c, w = models[(d["style"], d["n"])]
+        eps = np.linspace(-10.0, 10.0, d["side"])
+        amp = np.linspace(0.0, d["amp_max"], d["side"])
+        reqs.append(SweepRequest(c, w["tlist"], (eps, amp), solver=d["solver"], fp64=d["fp64"],
+                                 arrival=d["arrival"]))
+    srv = SweepServer(streams=4, hbm_budget_bytes=16 << 30)
+    log(f"serving: compiling {len(models)} models x solver/precision variants ...")
+    srv.warm(reqs)
+    log(f"serving {len(reqs)} requests ...")
+    t0 = time.perf_counter()
+    done = srv.serve(reqs, realtime=True)
+    wall = time.perf_counter() - t0
+    lat = np.array([r.done_at - r.arrival for r in done])
+    pts = sum(r.shape[0] * r.shape[1] * (len(r.tlist) - 1) for r in done)
+    out = {"metric": "sweep-request latency (arrival -> host result) under a Poisson stream",
+           "unit": "s", "n_gpus": 1, "data": "synthetic", "vs_baseline": None,
+           "config": {"workload": "Poisson 20 req/s, LZS / N-level chain (N 2,4,8), sides "
+                                  "32..1024, rk4/tsit5a, fp32/fp64", "requests": len(done)},
+           "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+           "mean_s": float(lat.mean()), "max_s": float(lat.max()),
+           "requests_per_s": len(done) / wall, "param_point_steps_per_s": pts / wall,
+           "wall_s": wall, "launches": srv.launches,
+           "mean_batch_size": float(np.mean([r.batch_size for r in done])),
+           "peak_hbm_bytes": srv.peak_in_use,
+           "nonfinite_requests": int(sum(1 for r in done if r.status))}
     print(json.dumps(out), flush=True)
 
 

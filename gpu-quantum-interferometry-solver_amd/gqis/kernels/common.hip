@@ -9,8 +9,13 @@ __device__ __forceinline__ gq_point gq_locate(const gq_sweep_args& a, long long 
     gq_point pt;
     pt.q = q;
     pt.p = a.point_begin + q;
-    pt.ix = pt.p / a.num_y;
-    pt.iy = pt.p - pt.ix * a.num_y;
+    if (a.param_mode == 1) {          // per-point parameter lists (merged request batches)
+        pt.ix = q;
+        pt.iy = q;
+    } else {
+        pt.ix = pt.p / a.num_y;
+        pt.iy = pt.p - pt.ix * a.num_y;
+    }
     return pt;
 }
 

@@ -32,6 +32,8 @@ class SweepArgs(ctypes.Structure):
         ("trace_offset", ctypes.c_int),
         ("num_trace", ctypes.c_int),
         ("trace_point_stride", ctypes.c_int),
+        ("param_mode", ctypes.c_int),
+        ("reserved", ctypes.c_int),
         ("par_x", ctypes.c_void_p),
         ("par_y", ctypes.c_void_p),
         ("consts", ctypes.c_void_p),

@@ -96,3 +96,45 @@ def duffing(n_gamma=1024, n_x0=1024, t_end=500.0, n_out=5001, delta=0.2, alpha=-
         "tlist": np.linspace(0.0, t_end, n_out),
         "var_arrays": {GAMMA: np.linspace(0.1, 0.6, n_gamma), X0: np.linspace(-2.0, 2.0, n_x0)},
     }
+
+
+def chain(n, n_eps=64, n_amp=64, eps_range=(-10.0, 10.0), amp_max=20.0, n_t=2001, t_end=100.0):
+    """N-level generalisation of the four-level interferometer: detunings
+    (-1)^i (eps + 2 floor(i/2)), nearest-neighbour tunnelling 0.5, drive
+    A cos(t) diag((-1)^i)/2, relaxation sqrt(0.02)|i><i+1|, dephasing sqrt(0.01) diag((-1)^i),
+    observable |0><0| (n = 4 is ``four_level``)."""
+    signs = [(-1) ** i for i in range(n)]
+    h = sp.diag(*[s * (EPS + 2 * (i // 2)) for i, s in enumerate(signs)])
+    h += DRIVE * sp.diag(*signs) / 2
+    for i in range(n - 1):
+        h[i, i + 1] = h[i + 1, i] = sp.Rational(1, 2)
+    col = []
+    for i in range(n - 1):
+        op = sp.zeros(n)
+        op[i, i + 1] = sp.sqrt(sp.Rational(2, 100))
+        col.append(op)
+    col.append(sp.sqrt(sp.Rational(1, 100)) * sp.diag(*signs))
+    proj = sp.zeros(n)
+    proj[0, 0] = 1
+    return {
+        "H": h, "Drive": {DRIVE: AMP * sp.cos(T)}, "Col_Ops": col, "mean_operator": proj,
+        "tlist": np.linspace(0.0, t_end, n_t),
+        "var_arrays": {EPS: np.linspace(*eps_range, n_eps), AMP: np.linspace(0.0, amp_max, n_amp)},
+    }
+
+
+def serving_stream(count=1000, rate=20.0, seed=7, sides=(32, 128, 512, 1024)):
+    """Seeded Poisson stream of sweep-request descriptions: model (two-level LZS or an N-level
+    chain, N in {2, 4, 8}), square grid side, amplitude range, solver (rk4 / tsit5a) and
+    precision (fp32 / fp64).  Returns a list of dicts with the arrival time in seconds."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, count))
+    out = []
+    for i in range(count):
+        style = rng.choice(["lzs", "chain"])
+        n = 2 if style == "lzs" else int(rng.choice([2, 4, 8]))
+        out.append({"arrival": float(t[i]), "style": str(style), "n": n,
+                    "side": int(rng.choice(sides)), "amp_max": float(rng.uniform(5.0, 30.0)),
+                    "solver": str(rng.choice(["rk4", "tsit5a"])),
+                    "fp64": bool(rng.integers(0, 2))})
+    return out

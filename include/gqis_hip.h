@@ -34,6 +34,7 @@
  * Arguments of one sweep launch (fixed-step and adaptive kernels).  Pointers are device
  * addresses; element type "real" is float or double as the module was compiled.
  * Flat point p = ix * num_y + iy; a launch covers p in [point_begin, point_begin+point_count)
+ * (param_mode 1: the point parameters are given per launch point instead)
  * and writes its outputs at the local index q = p - point_begin:
  *   complex results   result[q]                         (real2: re, im)
  *   state results     result[i * point_count + q]       (i < state size)
@@ -54,6 +55,10 @@ typedef struct gq_sweep_args {
     int trace_offset;
     int num_trace;             /* 0: no trace                                                */
     int trace_point_stride;    /* leading dimension of the trace arrays (>= point_count)     */
+    int param_mode;            /* 0: par_x/par_y are the sweep axes (p = ix*num_y + iy);      */
+                               /* 1: they hold one value per launch point (q), for batches of */
+                               /*    requests merged into one launch                          */
+    int reserved;
     const void* par_x;         /* real[num_x]                                                */
     const void* par_y;         /* real[num_y]                                                */
     const void* consts;        /* real[]: runtime constants                                  */
