@@ -341,6 +341,10 @@ def bench_serving(args):
            "mean_s": float(lat.mean()), "max_s": float(lat.max()),
            "requests_per_s": len(done) / wall, "param_point_steps_per_s": pts / wall,
            "wall_s": wall, "launches": srv.launches,
+           "gpu_kernel_s_sum": srv.gpu_kernel_s, "host_launch_s": srv.host_launch_s,
+           "gpu_by_kind": {f"D={k[0]} {k[1]} {k[2]}": {"launches": v[0], "points": v[1],
+                                                        "gpu_s": v[2]}
+                           for k, v in sorted(srv.gpu_by_kind.items())},
            "mean_batch_size": float(np.mean([r.batch_size for r in done])),
            "peak_hbm_bytes": srv.peak_in_use,
            "nonfinite_requests": int(sum(1 for r in done if r.status))}

@@ -256,6 +256,7 @@ def prelude(fp64, acc64, block_size, defines):
            "__device__ __forceinline__ real gq_sign(real x) { return (x > R(0)) - (x < R(0)); }"]
     for k, v in defines.items():
         out.append(f"#define {k} {v}")
+    out += ["#ifndef GQ_HILBERT", "#define GQ_HILBERT 0", "#endif"]
     return "\n".join(out)
 
 

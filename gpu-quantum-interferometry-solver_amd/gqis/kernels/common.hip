@@ -19,6 +19,7 @@ __device__ __forceinline__ gq_point gq_locate(const gq_sweep_args& a, long long 
     return pt;
 }
 
+#if !GQ_HILBERT
 // Initial state: explicit per-point values, else the model's expressions (|0><0| by default).
 __device__ __forceinline__ void gq_init_state(real* __restrict__ y, const gq_sweep_args& a,
                                               const gq_point& pt, real px, real py,
@@ -31,6 +32,8 @@ __device__ __forceinline__ void gq_init_state(real* __restrict__ y, const gq_swe
     gq_init_expr(y, px, py, C);
 #endif
 }
+
+#endif
 
 __device__ __forceinline__ bool gq_finite(real v) { return isfinite(v); }
 
@@ -51,6 +54,7 @@ __device__ __forceinline__ void gq_store_complex(void* base, long long idx, real
     ((real2*)base)[idx] = v;
 }
 
+#if !GQ_HILBERT
 // Per-point output state of the fixed-step kernels (mean, state mean, trace) and its epilogue
 // (gqis/solver.py:2126-2193 output modes).
 struct gq_outputs {
@@ -151,3 +155,4 @@ __device__ __forceinline__ void gq_out_final(gq_outputs& o, const gq_sweep_args&
     gq_flag(a.status, o.trace_bad, GQ_STATUS_NONFINITE_TRACE);
 #endif
 }
+#endif  // !GQ_HILBERT

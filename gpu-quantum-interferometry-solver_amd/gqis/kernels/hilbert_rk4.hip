@@ -1,0 +1,257 @@
+// RK4 on the full density matrix with the commutator as one complex GEMM on MFMA (FP32).
+//
+// Model structure (checked on the host): H(t) = H0 + diag(v(ParX, ParY, t)) with H0 constant,
+// collapse operators diagonal or single-entry, observable diagonal.  With
+// H_eff = H0 - (i/2) sum L^+L (constant, complex) and Y = -i H_eff rho - i diag(v) rho,
+//     d rho/dt = Y + Y^+ + sum_k d_k d_k^T o rho  (diagonal L = diag(d_k))
+//              + sum |l|^2 rho_bb |a><a|           (single-entry L = l |a><b|).
+// X = H_eff rho is a real GEMM Hbig x Rbig with Hbig = [[Re, -Im], [Im, Re]] (2N x 2N) and
+// Rbig = [Re rho; Im rho] (2N x N).  One wavefront holds P = 32/N points side by side as the 32
+// columns of Rbig and runs v_mfma_f32_32x32x2f32 over K = 2N in N steps.  The K order is
+// permuted so that lane (c, h) feeds at step s the s-th value it holds in the accumulator
+// layout (rows 32t + 8q + 4h + r of column c): the output of one RHS is the operand of the
+// next with no data movement; the matching permutation of Hbig is packed on the host
+// (GQ_APACK, T x N x 64 floats, staged in LDS).  Y^+ needs Y transposed: a per-wavefront LDS
+// tile with padded rows (conflict-free).  RK4 stage times, k1 pipelining and output
+// semantics are the reference's (gqis/cuda_solvers.py:470-517, gqis/solver.py:2126-2152).
+//
+// Lane (c, h): point p = c / N, column j = c % N; it owns the N/2 complex elements
+// (i_m, j), i_m = 8 (m / 4) + 4 h + (m % 4), m < N/2, with Re in register m and Im in m + N/2.
+
+typedef float gq_f32x16 __attribute__((ext_vector_type(16)));
+
+#define HN GQ_HN                 // Hilbert dimension N (8, 16 or 32)
+#define HP (32 / HN)             // points per wavefront
+#define HT ((2 * HN + 31) / 32)  // 32x32 output tiles
+#define HM (HN / 2)              // complex elements per lane
+#define HWAVES (GQ_BLOCK / 64)
+#define HLD (HN + 1)             // padded LDS row
+
+__device__ __forceinline__ int gq_row_of(int m, int h) { return 8 * (m >> 2) + 4 * h + (m & 3); }
+
+extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4)(const gq_sweep_args a) {
+    __shared__ float s_apack[HT * HN * 64];
+    __shared__ float s_y[HWAVES][2][HP * HN * HLD];
+    __shared__ float s_diag[HWAVES][HP * HN];
+    __shared__ float s_dep[HN * HN];          // dephasing weights sum_k d_k(i) d_k(j)
+    __shared__ float s_jw[HN * GQ_JMAX];      // population transfer into a: weights |l|^2 ...
+    __shared__ int s_jb[HN * GQ_JMAX];        // ... from the populations b (sparse rows)
+    __shared__ double s_red[HWAVES][64];
+
+    const float* __restrict__ cst = (const float*)a.consts;   // packed operator constants
+    for (int e = threadIdx.x; e < HT * HN * 64; e += GQ_BLOCK) s_apack[e] = cst[GQ_OFF_APACK + e];
+    for (int e = threadIdx.x; e < HN * HN; e += GQ_BLOCK) s_dep[e] = cst[GQ_OFF_DEP + e];
+    for (int e = threadIdx.x; e < HN * GQ_JMAX; e += GQ_BLOCK) {
+        s_jw[e] = cst[GQ_OFF_JW + e];
+        s_jb[e] = (int)cst[GQ_OFF_JB + e];
+    }
+    __syncthreads();
+
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int c = lane & 31, h = lane >> 5;
+    const int pw = c / HN, j = c % HN;
+    const long long q = ((long long)blockIdx.x * HWAVES + wave) * HP + pw;
+    const bool valid = q < a.point_count;
+    const long long qc = valid ? q : 0;
+    const gq_point pt = gq_locate(a, qc);
+    const float px = ((const float*)a.par_x)[pt.ix];
+    const float py = ((const float*)a.par_y)[pt.iy];
+    float* __restrict__ ybuf_r = s_y[wave][0] + pw * HN * HLD;
+    float* __restrict__ ybuf_i = s_y[wave][1] + pw * HN * HLD;
+    float* __restrict__ dbuf = s_diag[wave] + pw * HN;
+
+    // ---- initial state: rho0 values (reduced layout) or |0><0| ------------------------------
+    float y[HN];
+#pragma unroll
+    for (int m = 0; m < HM; ++m) {
+        const int i = gq_row_of(m, h);
+        float re = 0.0f, im = 0.0f;
+#if GQ_RHO0_VALUES
+        const float* __restrict__ r0 = (const float*)a.rho0 + qc * (HN * HN - 1);
+        if (i == j) {
+            if (i < HN - 1) {
+                re = r0[i];
+            } else {
+                float s = 0.0f;
+                for (int k = 0; k < HN - 1; ++k) s += r0[k];
+                re = 1.0f - s;
+            }
+        } else {
+            const int lo = i < j ? i : j, hi = i < j ? j : i;
+            const int u = lo * HN - (lo * (lo + 1)) / 2 + (hi - lo - 1);
+            re = r0[HN - 1 + 2 * u];
+            im = (i < j ? 1.0f : -1.0f) * r0[HN - 1 + 2 * u + 1];
+        }
+#else
+        re = (i == 0 && j == 0) ? 1.0f : 0.0f;
+#endif
+        y[m] = re;
+        y[m + HM] = im;
+    }
+
+    // per-row diagonal drive coefficients v_i = sum_k coef[k][i] g_k(px, py, t)
+    float vco[GQ_NBASIS][HM];
+#pragma unroll
+    for (int k = 0; k < GQ_NBASIS; ++k)
+#pragma unroll
+        for (int m = 0; m < HM; ++m) vco[k][m] = cst[GQ_OFF_VCOEF + k * HN + gq_row_of(m, h)];
+    float odiag[HM];
+#pragma unroll
+    for (int m = 0; m < HM; ++m) odiag[m] = cst[GQ_OFF_OBS + gq_row_of(m, h)];
+
+    const float dt = (float)a.dt;
+    const float dt2 = dt * 0.5f;
+    const float dt6 = dt / 6.0f;
+    const int num_steps = a.num_steps;
+    float v[HM];
+
+    auto stage_values = [&](int row) {
+        const float* __restrict__ tab = (const float*)a.time_table;
+        float TT[GQ_NATOMS_ALLOC];
+#pragma unroll
+        for (int k = 0; k < GQ_NATOMS; ++k) TT[k] = tab[row * GQ_NATOMS + k];
+        float g[GQ_NBASIS];
+        gq_basis(px, py, TT, cst, g);
+#pragma unroll
+        for (int m = 0; m < HM; ++m) {
+            float s = 0.0f;
+#pragma unroll
+            for (int k = 0; k < GQ_NBASIS; ++k) s = fmaf(vco[k][m], g[k], s);
+            v[m] = s;
+        }
+    };
+
+    auto rhs = [&](const float* __restrict__ x, float* __restrict__ k) {
+        gq_f32x16 acc[HT];
+#pragma unroll
+        for (int t = 0; t < HT; ++t)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[t][e] = 0.0f;
+#pragma unroll
+        for (int s = 0; s < HN; ++s) {
+#pragma unroll
+            for (int t = 0; t < HT; ++t) {
+                const float av = s_apack[(t * HN + s) * 64 + lane];
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, x[s], acc[t], 0, 0, 0);
+            }
+        }
+        float yr[HM], yi[HM];
+#pragma unroll
+        for (int m = 0; m < HM; ++m) {
+            const float xr = acc[m >> 4][m & 15];
+            const float xi = acc[(m + HM) >> 4][(m + HM) & 15];
+            // Y = -i X - i v rho
+            yr[m] = fmaf(v[m], x[m + HM], xi);
+            yi[m] = -fmaf(v[m], x[m], xr);
+            const int i = gq_row_of(m, h);
+            ybuf_r[i * HLD + j] = yr[m];
+            ybuf_i[i * HLD + j] = yi[m];
+            if (i == j) dbuf[i] = x[m];
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+#pragma unroll
+        for (int m = 0; m < HM; ++m) {
+            const int i = gq_row_of(m, h);
+            const float dep = s_dep[i * HN + j];
+            float dr = yr[m] + ybuf_r[j * HLD + i];
+            float di = yi[m] - ybuf_i[j * HLD + i];
+            dr = fmaf(dep, x[m], dr);
+            di = fmaf(dep, x[m + HM], di);
+            if (i == j) {
+                float sum = 0.0f;
+#pragma unroll
+                for (int e = 0; e < GQ_JMAX; ++e)
+                    sum = fmaf(s_jw[i * GQ_JMAX + e], dbuf[s_jb[i * GQ_JMAX + e]], sum);
+                dr += sum;
+            }
+            k[m] = dr;
+            k[m + HM] = di;
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    };
+
+    stage_values(0);
+    float k[HN];
+    rhs(y, k);
+    double avg = 0.0;
+    for (int step = 0; step < num_steps; ++step) {
+        stage_values(2 * step + 1);
+        float yt[HN], acc[HN];
+#pragma unroll
+        for (int i = 0; i < HN; ++i) {
+            yt[i] = fmaf(dt2, k[i], y[i]);
+            acc[i] = k[i];
+        }
+        rhs(yt, k);
+#pragma unroll
+        for (int i = 0; i < HN; ++i) {
+            yt[i] = fmaf(dt2, k[i], y[i]);
+            acc[i] = fmaf(2.0f, k[i], acc[i]);
+        }
+        rhs(yt, k);
+#pragma unroll
+        for (int i = 0; i < HN; ++i) {
+            yt[i] = fmaf(dt, k[i], y[i]);
+            acc[i] = fmaf(2.0f, k[i], acc[i]);
+        }
+        stage_values(2 * step + 2);
+        rhs(yt, k);
+#pragma unroll
+        for (int i = 0; i < HN; ++i) {
+            acc[i] += k[i];
+            y[i] = fmaf(dt6, acc[i], y[i]);
+        }
+        if (step + 1 < num_steps) rhs(y, k);
+#if GQ_OUT_MEAN
+        if (step >= a.warmup_steps) {
+#pragma unroll
+            for (int m = 0; m < HM; ++m)
+                if (gq_row_of(m, h) == j) avg += (double)(odiag[m] * y[m]);
+        }
+#endif
+    }
+
+    bool bad = false;
+#if GQ_OUT_MEAN || GQ_OUT_FINAL
+#if GQ_OUT_FINAL
+    avg = 0.0;
+#pragma unroll
+    for (int m = 0; m < HM; ++m)
+        if (gq_row_of(m, h) == j) avg += (double)(odiag[m] * y[m]);
+#endif
+    // sum the partial traces of the 2N lanes of each point
+    s_red[wave][lane] = avg;
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    if (h == 0 && j == 0) {
+        double s = 0.0;
+        for (int l = 0; l < HN; ++l) s += s_red[wave][pw * HN + l] + s_red[wave][32 + pw * HN + l];
+#if GQ_OUT_MEAN
+        s /= (double)(num_steps - a.warmup_steps);
+#endif
+        if (valid) {
+            const float r = (float)s;
+            bad = !gq_finite(r);
+            gq_store_complex(a.result, q, r, 0.0f);
+        }
+    }
+#elif GQ_OUT_FINAL_RHO
+    if (valid) {
+#pragma unroll
+        for (int m = 0; m < HM; ++m) {
+            const int i = gq_row_of(m, h);
+            int idx = -1;
+            if (i == j && i < HN - 1) {
+                ((float*)a.result)[(long long)i * a.point_count + q] = y[m];
+                bad |= !gq_finite(y[m]);
+            } else if (i < j) {
+                idx = HN - 1 + 2 * (i * HN - (i * (i + 1)) / 2 + (j - i - 1));
+                ((float*)a.result)[(long long)idx * a.point_count + q] = y[m];
+                ((float*)a.result)[(long long)(idx + 1) * a.point_count + q] = y[m + HM];
+                bad |= !(gq_finite(y[m]) && gq_finite(y[m + HM]));
+            }
+        }
+    }
+#endif
+    gq_flag(a.status, bad, GQ_STATUS_NONFINITE_RESULT);
+}

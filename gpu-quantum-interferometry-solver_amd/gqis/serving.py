@@ -54,10 +54,11 @@ class SweepRequest:
 
 class _Launch:
     def __init__(self, requests, stream, event, result, host, status_dev, status_host, nbytes,
-                 offsets, mode, size):
+                 offsets, mode, size, start_event, stop_event, host_key):
         self.requests, self.stream, self.event = requests, stream, event
         self.result, self.host, self.status_dev, self.status_host = result, host, status_dev, status_host
         self.nbytes, self.offsets, self.mode, self.size = nbytes, offsets, mode, size
+        self.start_event, self.stop_event, self.host_key = start_event, stop_event, host_key
 
 
 class SweepServer:
@@ -79,6 +80,10 @@ class SweepServer:
         self.modules = {}
         self.peak_in_use = 0
         self.launches = 0
+        self.gpu_kernel_s = 0.0       # sum of launch durations (they may overlap)
+        self.host_launch_s = 0.0
+        self.host_pool = {}           # pinned host buffers by (bytes class)
+        self.gpu_by_kind = {}         # (N, solver, precision) -> [launches, points, GPU seconds]
 
     # -- preparation ---------------------------------------------------------------------------
     def _key(self, r):
@@ -153,23 +158,33 @@ class SweepServer:
         r0 = batch[0]
         rdt = torch.float64 if r0.fp64 else torch.float32
         cdt = torch.complex128 if r0.fp64 else torch.complex64
-        px, py, offsets = [], [], []
+        h0 = time.perf_counter()
+        offsets, xs, ys = [], [], []
         off = 0
         for r in batch:
             x = np.asarray(r.axes[0], np.float64)
             y = np.asarray(r.axes[1], np.float64) if len(r.axes) > 1 else np.zeros(1)
-            px.append(np.repeat(x, len(y)))
-            py.append(np.tile(y, len(x)))
+            xs.append(x)
+            ys.append(y)
             offsets.append(off)
             off += len(x) * len(y)
-        px = np.concatenate(px)
-        py = np.concatenate(py)
         stream = self.free_streams.popleft()
         size = r0.compiled.state_size
         mode = r0.output_mode
         with torch.cuda.stream(stream):
-            pxd = torch.as_tensor(px, dtype=rdt).to(self.dev, non_blocking=True)
-            pyd = torch.as_tensor(py, dtype=rdt).to(self.dev, non_blocking=True)
+            start_ev = torch.cuda.Event(enable_timing=True)
+            stop_ev = torch.cuda.Event(enable_timing=True)
+            # axes go up as they are (small); the per-point lists are expanded on the device
+            ax = torch.as_tensor(np.concatenate(xs + ys), dtype=rdt).to(self.dev, non_blocking=True)
+            parts_x, parts_y, pos = [], [], 0
+            for x, y in zip(xs, ys):
+                parts_x.append(ax[pos:pos + len(x)].repeat_interleave(len(y)))
+                pos += len(x)
+            for x, y in zip(xs, ys):
+                parts_y.append(ax[pos:pos + len(y)].repeat(len(x)))
+                pos += len(y)
+            pxd = torch.cat(parts_x)
+            pyd = torch.cat(parts_y)
             consts = torch.as_tensor(np.asarray(key[-1] or (0.0,), np.float64), dtype=rdt,
                                      device=self.dev)
             if mode in ("final_rho", "mean_state"):
@@ -195,6 +210,7 @@ class SweepServer:
                 a.solver_param[i] = float(v)
             a.result, a.status, a.work_cursor = result.data_ptr(), status.data_ptr(), cursor.data_ptr()
             sid = stream.cuda_stream
+            start_ev.record(stream)
             if table is not None:
                 self.ctx.launch(mod, codegen.kernel_name(
                     "gq_rk4_time_table" if spec.template == "fixed_rk4.hip" else "gq_fixed_time_table",
@@ -204,15 +220,22 @@ class SweepServer:
             if spec.adaptive and spec.options.get("persistent", True):
                 grid = min(grid, self.ctx.resident_groups(mod, kname, self.block))
             self.ctx.launch(mod, kname, max(1, grid), self.block, a, sid)
-            host = torch.empty(result.shape, dtype=result.dtype, pin_memory=True)
+            stop_ev.record(stream)
+            nb = result.numel() * result.element_size()
+            host_key = (1 << max(12, (nb - 1).bit_length()))
+            pool = self.host_pool.setdefault(host_key, [])
+            buf = pool.pop() if pool else torch.empty(host_key, dtype=torch.uint8, pin_memory=True)
+            host = buf[:nb].view(result.dtype).view(result.shape)
             host.copy_(result, non_blocking=True)
             status_host = torch.empty(1, dtype=torch.int32, pin_memory=True)
             status_host.copy_(status, non_blocking=True)
             event = torch.cuda.Event()
             event.record(stream)
         # keep device tensors alive until the launch completes
-        launch = _Launch(batch, stream, event, (result, pxd, pyd, consts, status, cursor, table),
-                         host, status, status_host, nbytes, offsets, mode, size)
+        launch = _Launch(batch, stream, event, (result, pxd, pyd, consts, status, cursor, table, ax),
+                         host, status, status_host, nbytes, offsets, mode, size, start_ev, stop_ev,
+                         (host_key, buf))
+        self.host_launch_s += time.perf_counter() - h0
         self.running.append(launch)
         self.in_use += nbytes
         self.peak_in_use = max(self.peak_in_use, self.in_use)
@@ -228,7 +251,16 @@ class SweepServer:
             self.free_streams.append(l.stream)
             t = now_fn()
             flags = int(l.status_host[0])
-            host = l.host.numpy()
+            ks = l.start_event.elapsed_time(l.stop_event) / 1000.0
+            self.gpu_kernel_s += ks
+            r0 = l.requests[0]
+            kind = (r0.compiled.state_size, r0.solver, "fp64" if r0.fp64 else "fp32")
+            acc = self.gpu_by_kind.setdefault(kind, [0, 0, 0.0])
+            acc[0] += 1
+            acc[1] += sum(r.shape[0] * r.shape[1] for r in l.requests)
+            acc[2] += ks
+            host = l.host.numpy().copy()
+            self.host_pool.setdefault(l.host_key[0], []).append(l.host_key[1])
             for r, off in zip(l.requests, l.offsets):
                 nx, ny = r.shape
                 if l.mode in ("final_rho", "mean_state"):

@@ -228,14 +228,16 @@ def mesolve_2D(H, Drive=None, Col_Ops=None, mean_operator=None, tlist=None,
                Actual_Kernel_Save=False, beep_on_error=False, ignore_non_finite_output=False,
                t_in=0.0, return_device=False, calibrate_timesteps=None, consts=None, initial=None,
                output=None, solver_options=None, codegen=None, diagnostics=None,
-               hiprtc_options=(), time_table=True, block_size=256):
+               hiprtc_options=(), time_table=True, block_size=256, liouville="auto"):
     """Lindblad master-equation sweep over one or two parameter axes on the GPU.
 
     Same signature, defaults, return shapes and errors as the reference ``mesolve_2D``
     (``gqis/solver.py:1712-1736``, ``GQIS_API.md:505-528``), plus three code-generation options:
     ``hiprtc_options`` (extra compiler flags), ``time_table`` (FP64 table of the time-only drive
     terms for fixed-step solvers; False evaluates them per point as the reference does) and
-    ``block_size`` (threads per work-group).  ``return_device=True`` returns torch tensors on
+    ``block_size`` (threads per work-group) and ``liouville`` ("auto", "lanes" or "mfma": the
+    matrix-core path for N = 8, 16, 32, used automatically from N = 16 in FP32 RK4 sweeps whose
+    operators fit it).  ``return_device=True`` returns torch tensors on
     the GPU.  ``H`` may be a ``CompiledRHS`` from ``compile_lindblad_rhs``.
 
     Returns ``(num_X, num_Y)`` complex for ``"mean"``/``"final"``, ``(num_X, num_Y, N*N-1)``
@@ -289,7 +291,8 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
                            beep_on_error=False, ignore_non_finite_output=False, t_in=0.0,
                            return_device=False, calibrate_timesteps=None, compiled_rhs=None,
                            hiprtc_options=(), time_table=True, block_size=256,
-                           _ode_size=None, _state_trace=False, _time_trace_offset=0):
+                           liouville="auto", _ode_size=None, _state_trace=False,
+                           _time_trace_offset=0):
     if calibrate_timesteps is not None and calibrate_timesteps is not False:
         from .calibration import solve_with_timestep_calibration
         return solve_with_timestep_calibration(dict(locals()))
@@ -383,6 +386,22 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
     collect = bool(timings or return_timing_info)
     total_start = time.time()
 
+    if liouville not in ("auto", "lanes", "mfma"):
+        raise ValueError("liouville must be 'auto', 'lanes' or 'mfma'.")
+    hilbert_out = None
+    if compiled_rhs is None and _ode_size is None and liouville != "lanes":
+        hilbert_out = _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values,
+                                   runtime_consts, keep_symbolic_consts, auto_runtime_consts,
+                                   rho0, rho0_values, spec, mode, fp64, return_time_trace,
+                                   liouville, dt, t_in, num_steps, warmup_steps, block_size,
+                                   collect, tuple(hiprtc_options) + _as_tuple(nvrtc_options))
+    if hilbert_out is not None:
+        outputs, stage_s, rhs_status = hilbert_out
+        return _finish(outputs, stage_s, rhs_status, spec, solver_frequency, unroll, collect,
+                       timings, return_device, return_time_trace, return_timing_info,
+                       ignore_non_finite_output, _ode_size, num_trace, trace_stride, trace_offset,
+                       fp64, t_in, dt, total_start)
+
     # 1) symbolic ODE and plan
     if compiled_rhs is None:
         compiled, rhs_status = _compile_rhs(
@@ -434,6 +453,16 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
         state_trace=_state_trace, time_table=time_table, block_size=block_size,
         options=tuple(hiprtc_options) + _as_tuple(nvrtc_options), reuse=RHSreuse,
         collect=collect)
+    return _finish(outputs, stage_s, rhs_status, spec, solver_frequency, unroll, collect, timings,
+                   return_device, return_time_trace, return_timing_info, ignore_non_finite_output,
+                   _ode_size, num_trace, trace_stride, trace_offset, fp64, t_in, dt, total_start)
+
+
+def _finish(outputs, stage_s, rhs_status, spec, solver_frequency, unroll, collect, timings,
+            return_device, return_time_trace, return_timing_info, ignore_non_finite_output,
+            _ode_size, num_trace, trace_stride, trace_offset, fp64, t_in, dt, total_start):
+    """Finite checks, host copies, timing dictionary and return tuple
+    (``gqis/solver.py:2347-2391``)."""
     flags = outputs["status"]
     if flags & 1 and not ignore_non_finite_output:
         raise RuntimeError(f"Non-finite values detected in "
@@ -476,6 +505,54 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
     return result
 
 
+def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, runtime_consts,
+                 keep_symbolic_consts, auto_runtime_consts, rho0, rho0_values, spec, mode, fp64,
+                 trace, liouville, dt, t_in, num_steps, warmup_steps, block_size, collect, options):
+    """Run the matrix-core (Hilbert-space) path when it applies; None otherwise."""
+    from . import hilbert
+    n = H.shape[0] if hasattr(H, "shape") else 0
+    wanted = liouville == "mfma" or (liouville == "auto" and n >= 16)
+    reasons = []
+    if n not in hilbert.SUPPORTED_N:
+        reasons.append(f"N = {n} (supported: {hilbert.SUPPORTED_N})")
+    if fp64:
+        reasons.append("fp64 (the matrix-core path is FP32)")
+    if spec.name != "rk4":
+        reasons.append(f"solver {spec.name} (RK4 only)")
+    if trace:
+        reasons.append("time traces")
+    if mode not in ("mean", "final", "final_rho"):
+        reasons.append(f"output mode {mode}")
+    if not wanted:
+        return None
+    if reasons:
+        if liouville == "mfma":
+            raise ValueError("liouville='mfma' does not apply: " + ", ".join(reasons))
+        return None
+    start = time.time()
+    model = _prepare_model(H, Drive, Col_Ops, mean_operator, sweep_symbols=tuple(var_arrays),
+                           const_values=const_values, runtime_consts=runtime_consts,
+                           keep_symbolic_consts=keep_symbolic_consts,
+                           auto_runtime_consts=auto_runtime_consts, rho0=rho0, ode_size=None)
+    hm = hilbert.extract(model, _drive_map(model["Drive"]))
+    if hm is None:
+        if liouville == "mfma":
+            raise ValueError("liouville='mfma' needs H = constant + diagonal parameter/time "
+                             "terms, constant diagonal or single-entry collapse operators and a "
+                             "diagonal observable.")
+        return None
+    values = dict(model["defaults"])
+    values.update(runtime_consts or {})
+    rt = [float(values[s]) for s in model["runtime_const_syms"]]
+    prep = time.time() - start
+    outputs = hilbert.run(hm, mode=mode, var_arrays=var_arrays, rho0_values=rho0_values,
+                          runtime_values=rt, dt=dt, t_in=t_in, num_steps=num_steps,
+                          warmup_steps=warmup_steps, block_size=block_size, collect=collect,
+                          options=options)
+    stage_s = {"ode_prepare_s": prep, "ode_optimize_s": 0.0, "emit_s": 0.0}
+    return outputs, stage_s, "miss"
+
+
 def _as_tuple(options):
     if options is None:
         return ()
@@ -504,7 +581,8 @@ def _emit_source(compiled, spec, mode, uses_rho0_values, trace, state_trace, fp6
         "GQ_RHO0_VALUES": int(bool(uses_rho0_values)),
         "GQ_TIME_TABLE": int(bool(time_table) and not spec.adaptive),
     }
-    templates = [spec.template]
+    templates = ([spec.template] if spec.template != "fixed_rk4.hip"
+                 else ["rk4_table.hip", "fixed_rk4.hip"])
     from . import tableaux
     extra = ""
     if spec.adaptive:
