@@ -114,7 +114,7 @@ def main():
     ap.add_argument("--profile", choices=["fp64", "fp32", "plain"], default=None,
                     help="run only `--steps` sweeps of one form (for rocprofv3), no JSON line")
     ap.add_argument("--requests", type=int, default=1000, help="serving: number of requests")
-    ap.add_argument("--workload", choices=["four_level", "duffing", "serving"],
+    ap.add_argument("--workload", choices=["four_level", "duffing", "serving", "scaling"],
                     default="four_level",
                     help="four_level: the headline sweep; duffing: adaptive Tsit5/DOP853 sweep")
     ap.add_argument("--copy-bytes", type=int, default=0,
@@ -133,6 +133,8 @@ def main():
         return bench_duffing(args)
     if args.workload == "serving":
         return bench_serving(args)
+    if args.workload == "scaling":
+        return bench_scaling(args)
 
     w = workloads.four_level(n_eps=args.grid, n_amp=args.grid)
     npts = args.grid * args.grid
@@ -297,6 +299,48 @@ def bench_duffing(args):
     t_best = out["tsit5a_fp64"]
     out["value"] = t_best["value"]
     out["dtype"] = "f64"
+    print(json.dumps(out), flush=True)
+
+
+def bench_scaling(args):
+    """Hilbert-space scaling: seeded random N-level models (N = 4, 8, 16, 32), 256 x 256 grid,
+    RK4 2000 steps (dt = 0.01).  N = 4 runs the per-point register kernel; N = 8, 16, 32 the
+    matrix-core path (FP32 MFMA), N = 8 also the register kernel.  Errors are against the FP64
+    GPU paths (per-point FP64 for N = 4 and 8, FP64 Hilbert-space path for N = 16, 32) on a
+    16 x 16 subset of the grid."""
+    import numpy as np
+    import torch
+    from gqis import mesolve_2D, workloads
+    side = 256 if args.grid == 2048 else args.grid
+    out = {"metric": "param-point RK steps/s (Hilbert-space scaling)",
+           "unit": "param-point RK steps/s", "n_gpus": 1, "data": "synthetic",
+           "config": {"workload": f"random N-level, {side}x{side}, RK4 2000 steps dt 0.01",
+                      "points": side * side}, "results": {}}
+    for n, forms in ((4, ["lanes"]), (8, ["lanes", "mfma"]), (16, ["mfma"]), (32, ["mfma"])):
+        w = workloads.random_n_level(n, n_eps=side, n_amp=side)
+        sub = {k: v[:: side // 16] for k, v in w["var_arrays"].items()}
+        args_m = (w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"])
+        ref = mesolve_2D(*args_m, var_arrays=sub, fp64=True,
+                         liouville="lanes" if n <= 8 else "mfma")
+        for form in forms:
+            kw = dict(var_arrays=w["var_arrays"], liouville=form, return_device=True,
+                      return_timing_info=True, ignore_non_finite_output=True)
+            log(f"scaling N={n} {form} ...")
+            mesolve_2D(*args_m, **kw)
+            _, info = mesolve_2D(*args_m, **kw)
+            torch.cuda.synchronize()
+            sub_res = mesolve_2D(*args_m, var_arrays=sub, liouville=form)
+            k = info["gpu_kernel_s"]
+            ps = side * side * 2000
+            d2 = n * n
+            flops = 4 * 8 * n ** 3 + 4 * 12 * d2 + 13 * (d2 - 1) if form == "mfma" else None
+            entry = {"kernel_s": k, "value": ps / k,
+                     "max_abs_err_vs_fp64": float(np.max(np.abs(sub_res - ref)))}
+            if flops:
+                ach = flops * ps / k / 1e12
+                entry.update({"flops_per_point_step": flops, "achieved_tflops": ach,
+                              "mfma_fp32_peak_frac": ach / 157.3})
+            out["results"][f"N={n} {form}"] = entry
     print(json.dumps(out), flush=True)
 
 

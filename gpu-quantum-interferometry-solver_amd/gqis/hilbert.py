@@ -157,8 +157,12 @@ def constants(m, runtime_values):
     for a in range(m.n):
         for e, b in enumerate(np.nonzero(m.jump[a])[0]):
             jw[a, e], jb[a, e] = m.jump[a, b], b
-    parts = {"RC": rc, "APACK": pack_a(m).ravel(), "DEP": m.dep.ravel(), "JW": jw.ravel(),
-             "JB": jb.ravel(), "VCOEF": m.vcoef.ravel(), "OBS": m.obs}
+    n = m.n
+    big = np.zeros((2 * n, 2 * n))
+    big[:n, :n], big[:n, n:] = m.h_eff.real, -m.h_eff.imag
+    big[n:, :n], big[n:, n:] = m.h_eff.imag, m.h_eff.real
+    parts = {"RC": rc, "APACK": pack_a(m).ravel(), "HBIG": big.ravel(), "DEP": m.dep.ravel(),
+             "JW": jw.ravel(), "JB": jb.ravel(), "VCOEF": m.vcoef.ravel(), "OBS": m.obs}
     offs, buf, pos = {}, [], 0
     for k, v in parts.items():
         offs[k] = pos
@@ -167,7 +171,12 @@ def constants(m, runtime_values):
     return np.concatenate(buf).astype(np.float64), offs
 
 
-def emit(m, offs, mode, block_size=256):
+def block_for(m, fp64, block_size=256):
+    """Work-group size: the FP64 N = 32 kernel keeps its LDS under 160 KB with two waves."""
+    return 128 if (fp64 and m.n == 32) else block_size
+
+
+def emit(m, offs, mode, block_size=256, fp64=False):
     """hiprtc source of the Hilbert-space RK4 kernel for model m."""
     atoms = {}
     names = {"ParX": "px", "ParY": "py"}
@@ -178,6 +187,7 @@ def emit(m, offs, mode, block_size=256):
     for i, s in enumerate(m.runtime_syms):
         names[s.name] = f"C[{i}]"
     pr = HipPrinter(names)
+    ftype = "double" if fp64 else "float"
     lines = [f"#define GQ_HN {m.n}", f"#define GQ_NBASIS {len(m.basis)}",
              f"#define GQ_JMAX {jump_width(m)}",
              f"#define GQ_NATOMS {len(atom_list)}", f"#define GQ_NATOMS_ALLOC {max(1, len(atom_list))}"]
@@ -193,9 +203,9 @@ def emit(m, offs, mode, block_size=256):
     lines.append("    (void)t_now; (void)C; (void)TT;")
     lines += [f"    TT[{i}] = {dpr.doprint(e)};" for i, (_, e) in enumerate(atom_list)]
     lines.append("}")
-    lines.append("__device__ __forceinline__ void gq_basis(const float px, const float py, "
-                 "const float* __restrict__ TT, const float* __restrict__ C, "
-                 "float* __restrict__ g) {")
+    lines.append(f"__device__ __forceinline__ void gq_basis(const {ftype} px, const {ftype} py, "
+                 f"const {ftype}* __restrict__ TT, const {ftype}* __restrict__ C, "
+                 f"{ftype}* __restrict__ g) {{")
     lines.append("    (void)px; (void)py; (void)TT; (void)C;")
     lines += [f"    g[{k}] = {pr.doprint(e)};" for k, e in enumerate(split)]
     lines.append("}")
@@ -205,7 +215,7 @@ def emit(m, offs, mode, block_size=256):
     defines = {"GQ_HILBERT": 1, "GQ_OUT_MEAN": int(mode == "mean"),
                "GQ_OUT_FINAL": int(mode == "final"), "GQ_OUT_FINAL_RHO": int(mode == "final_rho"),
                "GQ_OUT_MEAN_STATE": 0, "GQ_TRACE": 0, "GQ_RHO0_VALUES": "GQ_RHO0_FROM_ARGS"}
-    parts = [header, codegen.prelude(False, True, block_size, defines), model_src,
+    parts = [header, codegen.prelude(fp64, True, block_size, defines), model_src,
              codegen.read_kernel_template("common.hip"), codegen.read_kernel_template("rk4_table.hip"),
              codegen.read_kernel_template("hilbert_rk4.hip")]
     return "\n".join(parts)
@@ -215,7 +225,7 @@ _SOURCES = {}
 
 
 def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps, warmup_steps,
-        block_size=256, collect=False, options=()):
+        block_size=256, collect=False, options=(), fp64=False):
     """Launch the Hilbert-space kernel; returns the same dict as ``device.run_sweep``."""
     import torch
     from .device import _axis, _initial_states
@@ -224,29 +234,32 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
     dev = torch.device("cuda", ctx.device)
     consts, offs = constants(m, runtime_values)
     uses_rho0 = rho0_values is not None
-    skey = (m.key, mode, block_size, uses_rho0, tuple(options))
+    block_size = block_for(m, fp64, block_size)
+    skey = (m.key, mode, block_size, uses_rho0, tuple(options), bool(fp64))
     src = _SOURCES.get(skey)
     if src is None:
-        src = emit(m, offs, mode, block_size).replace(
+        src = emit(m, offs, mode, block_size, fp64).replace(
             "GQ_RHO0_FROM_ARGS", "1" if uses_rho0 else "0")
         _SOURCES[skey] = src
     module, cached, build_s = ctx.compile(src, codegen.HIPRTC_BASE_OPTIONS + tuple(options))
+    rdt = torch.float64 if fp64 else torch.float32
     axes = list(var_arrays.values())
-    px = _axis(axes[0], torch.float32, dev)
-    py = _axis(axes[1], torch.float32, dev) if len(axes) > 1 else torch.zeros(1, device=dev)
+    px = _axis(axes[0], rdt, dev)
+    py = _axis(axes[1], rdt, dev) if len(axes) > 1 else torch.zeros(1, dtype=rdt, device=dev)
     nx, ny = int(px.numel()), int(py.numel())
     npts = nx * ny
     d = m.n * m.n - 1
-    rho0 = _initial_states(rho0_values, nx, ny, d, torch.float32, dev) if uses_rho0 else None
-    cbuf = torch.as_tensor(consts, dtype=torch.float32, device=dev)
+    rho0 = _initial_states(rho0_values, nx, ny, d, rdt, dev) if uses_rho0 else None
+    cbuf = torch.as_tensor(consts, dtype=rdt, device=dev)
     if mode == "final_rho":
-        result = torch.empty((d, max(npts, 1)), dtype=torch.float32, device=dev)
+        result = torch.empty((d, max(npts, 1)), dtype=rdt, device=dev)
     else:
-        result = torch.empty(max(npts, 1), dtype=torch.complex64, device=dev)
+        result = torch.empty(max(npts, 1), dtype=torch.complex128 if fp64 else torch.complex64,
+                             device=dev)
     status = torch.zeros(1, dtype=torch.int32, device=dev)
     rows = 2 * num_steps + 1
     n_atoms = int(src.split("#define GQ_NATOMS ")[1].split()[0])
-    table = torch.empty(max(1, rows * max(1, n_atoms)), dtype=torch.float32, device=dev)
+    table = torch.empty(max(1, rows * max(1, n_atoms)), dtype=rdt, device=dev)
     a = runtime.SweepArgs()
     a.dt, a.t_in = float(dt), float(t_in)
     a.num_x, a.num_y = nx, ny
@@ -265,9 +278,10 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
         if timer:
             timer.record_start()
         if n_atoms > 0:
-            ctx.launch(module, "gq_rk4_time_table_f32", (rows + 255) // 256, 256, a, stream)
-        ctx.launch(module, "gq_hilbert_rk4_f32", (npts + per_block - 1) // per_block, block_size,
-                   a, stream)
+            ctx.launch(module, codegen.kernel_name("gq_rk4_time_table", fp64), (rows + 255) // 256,
+                       256, a, stream)
+        ctx.launch(module, codegen.kernel_name("gq_hilbert_rk4", fp64),
+                   (npts + per_block - 1) // per_block, block_size, a, stream)
         if timer:
             timer.record_stop()
     flags = ctx.read_status(status, stream)

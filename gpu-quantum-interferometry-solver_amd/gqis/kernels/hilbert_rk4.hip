@@ -1,4 +1,5 @@
-// RK4 on the full density matrix with the commutator as one complex GEMM on MFMA (FP32).
+// RK4 on the full density matrix with the commutator as one complex GEMM on MFMA (FP32), or
+// as VALU dot products over an LDS copy of the state (FP64, GQ_FP64 = 1).
 //
 // Model structure (checked on the host): H(t) = H0 + diag(v(ParX, ParY, t)) with H0 constant,
 // collapse operators diagonal or single-entry, observable diagonal.  With
@@ -30,16 +31,25 @@ typedef float gq_f32x16 __attribute__((ext_vector_type(16)));
 __device__ __forceinline__ int gq_row_of(int m, int h) { return 8 * (m >> 2) + 4 * h + (m & 3); }
 
 extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4)(const gq_sweep_args a) {
+#if GQ_FP64
+    __shared__ real s_hbig[2 * HN * 2 * HN];
+    __shared__ real s_x[HWAVES][HP * 2 * HN * HN];
+#else
     __shared__ float s_apack[HT * HN * 64];
-    __shared__ float s_y[HWAVES][2][HP * HN * HLD];
-    __shared__ float s_diag[HWAVES][HP * HN];
-    __shared__ float s_dep[HN * HN];          // dephasing weights sum_k d_k(i) d_k(j)
-    __shared__ float s_jw[HN * GQ_JMAX];      // population transfer into a: weights |l|^2 ...
+#endif
+    __shared__ real s_y[HWAVES][2][HP * HN * HLD];
+    __shared__ real s_diag[HWAVES][HP * HN];
+    __shared__ real s_dep[HN * HN];           // dephasing weights sum_k d_k(i) d_k(j)
+    __shared__ real s_jw[HN * GQ_JMAX];       // population transfer into a: weights |l|^2 ...
     __shared__ int s_jb[HN * GQ_JMAX];        // ... from the populations b (sparse rows)
     __shared__ double s_red[HWAVES][64];
 
-    const float* __restrict__ cst = (const float*)a.consts;   // packed operator constants
+    const real* __restrict__ cst = (const real*)a.consts;   // packed operator constants
+#if GQ_FP64
+    for (int e = threadIdx.x; e < 4 * HN * HN; e += GQ_BLOCK) s_hbig[e] = cst[GQ_OFF_HBIG + e];
+#else
     for (int e = threadIdx.x; e < HT * HN * 64; e += GQ_BLOCK) s_apack[e] = cst[GQ_OFF_APACK + e];
+#endif
     for (int e = threadIdx.x; e < HN * HN; e += GQ_BLOCK) s_dep[e] = cst[GQ_OFF_DEP + e];
     for (int e = threadIdx.x; e < HN * GQ_JMAX; e += GQ_BLOCK) {
         s_jw[e] = cst[GQ_OFF_JW + e];
@@ -55,74 +65,92 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4)
     const bool valid = q < a.point_count;
     const long long qc = valid ? q : 0;
     const gq_point pt = gq_locate(a, qc);
-    const float px = ((const float*)a.par_x)[pt.ix];
-    const float py = ((const float*)a.par_y)[pt.iy];
-    float* __restrict__ ybuf_r = s_y[wave][0] + pw * HN * HLD;
-    float* __restrict__ ybuf_i = s_y[wave][1] + pw * HN * HLD;
-    float* __restrict__ dbuf = s_diag[wave] + pw * HN;
+    const real px = ((const real*)a.par_x)[pt.ix];
+    const real py = ((const real*)a.par_y)[pt.iy];
+    real* __restrict__ ybuf_r = s_y[wave][0] + pw * HN * HLD;
+    real* __restrict__ ybuf_i = s_y[wave][1] + pw * HN * HLD;
+    real* __restrict__ dbuf = s_diag[wave] + pw * HN;
+#if GQ_FP64
+    real* __restrict__ xcol = s_x[wave] + pw * 2 * HN * HN;   // Rbig of this point, [row][j]
+#endif
 
     // ---- initial state: rho0 values (reduced layout) or |0><0| ------------------------------
-    float y[HN];
+    real y[HN];
 #pragma unroll
     for (int m = 0; m < HM; ++m) {
         const int i = gq_row_of(m, h);
-        float re = 0.0f, im = 0.0f;
+        real re = R(0.0), im = R(0.0);
 #if GQ_RHO0_VALUES
-        const float* __restrict__ r0 = (const float*)a.rho0 + qc * (HN * HN - 1);
+        const real* __restrict__ r0 = (const real*)a.rho0 + qc * (HN * HN - 1);
         if (i == j) {
             if (i < HN - 1) {
                 re = r0[i];
             } else {
-                float s = 0.0f;
+                real s = R(0.0);
                 for (int k = 0; k < HN - 1; ++k) s += r0[k];
-                re = 1.0f - s;
+                re = R(1.0) - s;
             }
         } else {
             const int lo = i < j ? i : j, hi = i < j ? j : i;
             const int u = lo * HN - (lo * (lo + 1)) / 2 + (hi - lo - 1);
             re = r0[HN - 1 + 2 * u];
-            im = (i < j ? 1.0f : -1.0f) * r0[HN - 1 + 2 * u + 1];
+            im = (i < j ? R(1.0) : R(-1.0)) * r0[HN - 1 + 2 * u + 1];
         }
 #else
-        re = (i == 0 && j == 0) ? 1.0f : 0.0f;
+        re = (i == 0 && j == 0) ? R(1.0) : R(0.0);
 #endif
         y[m] = re;
         y[m + HM] = im;
     }
 
     // per-row diagonal drive coefficients v_i = sum_k coef[k][i] g_k(px, py, t)
-    float vco[GQ_NBASIS][HM];
+    real vco[GQ_NBASIS][HM];
 #pragma unroll
     for (int k = 0; k < GQ_NBASIS; ++k)
 #pragma unroll
         for (int m = 0; m < HM; ++m) vco[k][m] = cst[GQ_OFF_VCOEF + k * HN + gq_row_of(m, h)];
-    float odiag[HM];
+    real odiag[HM];
 #pragma unroll
     for (int m = 0; m < HM; ++m) odiag[m] = cst[GQ_OFF_OBS + gq_row_of(m, h)];
 
-    const float dt = (float)a.dt;
-    const float dt2 = dt * 0.5f;
-    const float dt6 = dt / 6.0f;
+    const real dt = (real)a.dt;
+    const real dt2 = dt * R(0.5);
+    const real dt6 = dt / R(6.0);
     const int num_steps = a.num_steps;
-    float v[HM];
+    real v[HM];
 
     auto stage_values = [&](int row) {
-        const float* __restrict__ tab = (const float*)a.time_table;
-        float TT[GQ_NATOMS_ALLOC];
+        const real* __restrict__ tab = (const real*)a.time_table;
+        real TT[GQ_NATOMS_ALLOC];
 #pragma unroll
         for (int k = 0; k < GQ_NATOMS; ++k) TT[k] = tab[row * GQ_NATOMS + k];
-        float g[GQ_NBASIS];
+        real g[GQ_NBASIS];
         gq_basis(px, py, TT, cst, g);
 #pragma unroll
         for (int m = 0; m < HM; ++m) {
-            float s = 0.0f;
+            real s = R(0.0);
 #pragma unroll
-            for (int k = 0; k < GQ_NBASIS; ++k) s = fmaf(vco[k][m], g[k], s);
+            for (int k = 0; k < GQ_NBASIS; ++k) s = fma(vco[k][m], g[k], s);
             v[m] = s;
         }
     };
 
-    auto rhs = [&](const float* __restrict__ x, float* __restrict__ k) {
+    auto rhs = [&](const real* __restrict__ x, real* __restrict__ k) {
+        real X[HN];   // rows row(m, h) of H_eff rho in the Rbig layout, column c
+#if GQ_FP64
+#pragma unroll
+        for (int m = 0; m < HN; ++m) xcol[gq_row_of(m, h) * HN + j] = x[m];
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+#pragma unroll
+        for (int m = 0; m < HN; ++m) {
+            const real* __restrict__ hrow = s_hbig + gq_row_of(m, h) * 2 * HN;
+            real s = R(0.0);
+#pragma unroll 4
+            for (int kk = 0; kk < 2 * HN; ++kk) s = fma(hrow[kk], xcol[kk * HN + j], s);
+            X[m] = s;
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+#else
         gq_f32x16 acc[HT];
 #pragma unroll
         for (int t = 0; t < HT; ++t)
@@ -136,14 +164,17 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4)
                 acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, x[s], acc[t], 0, 0, 0);
             }
         }
-        float yr[HM], yi[HM];
+#pragma unroll
+        for (int m = 0; m < HN; ++m) X[m] = acc[m >> 4][m & 15];
+#endif
+        real yr[HM], yi[HM];
 #pragma unroll
         for (int m = 0; m < HM; ++m) {
-            const float xr = acc[m >> 4][m & 15];
-            const float xi = acc[(m + HM) >> 4][(m + HM) & 15];
+            const real xr = X[m];
+            const real xi = X[m + HM];
             // Y = -i X - i v rho
-            yr[m] = fmaf(v[m], x[m + HM], xi);
-            yi[m] = -fmaf(v[m], x[m], xr);
+            yr[m] = fma(v[m], x[m + HM], xi);
+            yi[m] = -fma(v[m], x[m], xr);
             const int i = gq_row_of(m, h);
             ybuf_r[i * HLD + j] = yr[m];
             ybuf_i[i * HLD + j] = yi[m];
@@ -153,16 +184,16 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4)
 #pragma unroll
         for (int m = 0; m < HM; ++m) {
             const int i = gq_row_of(m, h);
-            const float dep = s_dep[i * HN + j];
-            float dr = yr[m] + ybuf_r[j * HLD + i];
-            float di = yi[m] - ybuf_i[j * HLD + i];
-            dr = fmaf(dep, x[m], dr);
-            di = fmaf(dep, x[m + HM], di);
+            const real dep = s_dep[i * HN + j];
+            real dr = yr[m] + ybuf_r[j * HLD + i];
+            real di = yi[m] - ybuf_i[j * HLD + i];
+            dr = fma(dep, x[m], dr);
+            di = fma(dep, x[m + HM], di);
             if (i == j) {
-                float sum = 0.0f;
+                real sum = R(0.0);
 #pragma unroll
                 for (int e = 0; e < GQ_JMAX; ++e)
-                    sum = fmaf(s_jw[i * GQ_JMAX + e], dbuf[s_jb[i * GQ_JMAX + e]], sum);
+                    sum = fma(s_jw[i * GQ_JMAX + e], dbuf[s_jb[i * GQ_JMAX + e]], sum);
                 dr += sum;
             }
             k[m] = dr;
@@ -172,35 +203,35 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4)
     };
 
     stage_values(0);
-    float k[HN];
+    real k[HN];
     rhs(y, k);
     double avg = 0.0;
     for (int step = 0; step < num_steps; ++step) {
         stage_values(2 * step + 1);
-        float yt[HN], acc[HN];
+        real yt[HN], acc[HN];
 #pragma unroll
         for (int i = 0; i < HN; ++i) {
-            yt[i] = fmaf(dt2, k[i], y[i]);
+            yt[i] = fma(dt2, k[i], y[i]);
             acc[i] = k[i];
         }
         rhs(yt, k);
 #pragma unroll
         for (int i = 0; i < HN; ++i) {
-            yt[i] = fmaf(dt2, k[i], y[i]);
-            acc[i] = fmaf(2.0f, k[i], acc[i]);
+            yt[i] = fma(dt2, k[i], y[i]);
+            acc[i] = fma(R(2.0), k[i], acc[i]);
         }
         rhs(yt, k);
 #pragma unroll
         for (int i = 0; i < HN; ++i) {
-            yt[i] = fmaf(dt, k[i], y[i]);
-            acc[i] = fmaf(2.0f, k[i], acc[i]);
+            yt[i] = fma(dt, k[i], y[i]);
+            acc[i] = fma(R(2.0), k[i], acc[i]);
         }
         stage_values(2 * step + 2);
         rhs(yt, k);
 #pragma unroll
         for (int i = 0; i < HN; ++i) {
             acc[i] += k[i];
-            y[i] = fmaf(dt6, acc[i], y[i]);
+            y[i] = fma(dt6, acc[i], y[i]);
         }
         if (step + 1 < num_steps) rhs(y, k);
 #if GQ_OUT_MEAN
@@ -230,9 +261,9 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4)
         s /= (double)(num_steps - a.warmup_steps);
 #endif
         if (valid) {
-            const float r = (float)s;
+            const real r = (real)s;
             bad = !gq_finite(r);
-            gq_store_complex(a.result, q, r, 0.0f);
+            gq_store_complex(a.result, q, r, R(0.0));
         }
     }
 #elif GQ_OUT_FINAL_RHO
@@ -242,12 +273,12 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4)
             const int i = gq_row_of(m, h);
             int idx = -1;
             if (i == j && i < HN - 1) {
-                ((float*)a.result)[(long long)i * a.point_count + q] = y[m];
+                ((real*)a.result)[(long long)i * a.point_count + q] = y[m];
                 bad |= !gq_finite(y[m]);
             } else if (i < j) {
                 idx = HN - 1 + 2 * (i * HN - (i * (i + 1)) / 2 + (j - i - 1));
-                ((float*)a.result)[(long long)idx * a.point_count + q] = y[m];
-                ((float*)a.result)[(long long)(idx + 1) * a.point_count + q] = y[m + HM];
+                ((real*)a.result)[(long long)idx * a.point_count + q] = y[m];
+                ((real*)a.result)[(long long)(idx + 1) * a.point_count + q] = y[m + HM];
                 bad |= !(gq_finite(y[m]) && gq_finite(y[m + HM]));
             }
         }

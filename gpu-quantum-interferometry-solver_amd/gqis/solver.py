@@ -236,8 +236,8 @@ def mesolve_2D(H, Drive=None, Col_Ops=None, mean_operator=None, tlist=None,
     ``hiprtc_options`` (extra compiler flags), ``time_table`` (FP64 table of the time-only drive
     terms for fixed-step solvers; False evaluates them per point as the reference does) and
     ``block_size`` (threads per work-group) and ``liouville`` ("auto", "lanes" or "mfma": the
-    matrix-core path for N = 8, 16, 32, used automatically from N = 16 in FP32 RK4 sweeps whose
-    operators fit it).  ``return_device=True`` returns torch tensors on
+    Hilbert-space path for N = 8, 16, 32 -- matrix cores in FP32, VALU dot products in FP64 --
+    used automatically from N = 16 in RK4 sweeps whose operators fit it).  ``return_device=True`` returns torch tensors on
     the GPU.  ``H`` may be a ``CompiledRHS`` from ``compile_lindblad_rhs``.
 
     Returns ``(num_X, num_Y)`` complex for ``"mean"``/``"final"``, ``(num_X, num_Y, N*N-1)``
@@ -515,8 +515,6 @@ def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, run
     reasons = []
     if n not in hilbert.SUPPORTED_N:
         reasons.append(f"N = {n} (supported: {hilbert.SUPPORTED_N})")
-    if fp64:
-        reasons.append("fp64 (the matrix-core path is FP32)")
     if spec.name != "rk4":
         reasons.append(f"solver {spec.name} (RK4 only)")
     if trace:
@@ -548,7 +546,7 @@ def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, run
     outputs = hilbert.run(hm, mode=mode, var_arrays=var_arrays, rho0_values=rho0_values,
                           runtime_values=rt, dt=dt, t_in=t_in, num_steps=num_steps,
                           warmup_steps=warmup_steps, block_size=block_size, collect=collect,
-                          options=options)
+                          options=options, fp64=fp64)
     stage_s = {"ode_prepare_s": prep, "ode_optimize_s": 0.0, "emit_s": 0.0}
     return outputs, stage_s, "miss"
 

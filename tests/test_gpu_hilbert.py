@@ -50,3 +50,16 @@ def test_mfma_continuation_from_final_state(gpu):
     whole = mesolve_2D(*args, np.linspace(0.0, 2.0, 201), var_arrays=w["var_arrays"],
                        liouville="mfma", output_mode="final_rho")
     np.testing.assert_allclose(second, whole, atol=2e-5)
+
+
+@pytest.mark.parametrize("n,steps", [(8, 200), (16, 100), (32, 40)])
+def test_fp64_hilbert_path_parity(gpu, n, steps):
+    w = workloads.random_n_level(n, seed=SEED, n_eps=2, n_amp=2, n_t=steps + 1,
+                                 t_end=0.01 * steps)
+    eps_v, amp_v = w["var_arrays"].values()
+    got = mesolve_2D(w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"],
+                     var_arrays=w["var_arrays"], liouville="mfma", fp64=True)
+    m = models.random_n_level(n, SEED, eps_v, amp_v)
+    ref, _ = integrate.rk4(m.f, m.ground_state(), 0.01, steps, obs=m.obs)
+    assert got.dtype == np.complex128
+    assert np.max(np.abs(got.ravel() - ref)) <= 1e-10 * max(1.0, np.max(np.abs(ref)))
