@@ -114,6 +114,7 @@ def main():
     ap.add_argument("--profile", choices=["fp64", "fp32", "plain"], default=None,
                     help="run only `--steps` sweeps of one form (for rocprofv3), no JSON line")
     ap.add_argument("--requests", type=int, default=1000, help="serving: number of requests")
+    ap.add_argument("--rate", type=float, default=20.0, help="serving: Poisson rate (req/s)")
     ap.add_argument("--workload", choices=["four_level", "duffing", "serving", "scaling"],
                     default="four_level",
                     help="four_level: the headline sweep; duffing: adaptive Tsit5/DOP853 sweep")
@@ -316,20 +317,23 @@ def bench_scaling(args):
            "unit": "param-point RK steps/s", "n_gpus": 1, "data": "synthetic",
            "config": {"workload": f"random N-level, {side}x{side}, RK4 2000 steps dt 0.01",
                       "points": side * side}, "results": {}}
-    for n, forms in ((4, ["lanes"]), (8, ["lanes", "mfma"]), (16, ["mfma"]), (32, ["mfma"])):
+    for n, forms in ((4, ["lanes"]), (8, ["lanes", "mfma", "lanes64", "mfma64"]), (16, ["mfma"]),
+                     (32, ["mfma"])):
         w = workloads.random_n_level(n, n_eps=side, n_amp=side)
         sub = {k: v[:: side // 16] for k, v in w["var_arrays"].items()}
         args_m = (w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"])
         ref = mesolve_2D(*args_m, var_arrays=sub, fp64=True,
                          liouville="lanes" if n <= 8 else "mfma")
         for form in forms:
-            kw = dict(var_arrays=w["var_arrays"], liouville=form, return_device=True,
-                      return_timing_info=True, ignore_non_finite_output=True)
+            fp64 = form.endswith("64")
+            lv = form[:-2] if fp64 else form
+            kw = dict(var_arrays=w["var_arrays"], liouville=lv, return_device=True,
+                      return_timing_info=True, ignore_non_finite_output=True, fp64=fp64)
             log(f"scaling N={n} {form} ...")
             mesolve_2D(*args_m, **kw)
             _, info = mesolve_2D(*args_m, **kw)
             torch.cuda.synchronize()
-            sub_res = mesolve_2D(*args_m, var_arrays=sub, liouville=form)
+            sub_res = mesolve_2D(*args_m, var_arrays=sub, liouville=lv, fp64=fp64)
             k = info["gpu_kernel_s"]
             ps = side * side * 2000
             d2 = n * n
@@ -352,7 +356,7 @@ def bench_serving(args):
     import numpy as np
     from gqis import compile_lindblad_rhs, workloads
     from gqis.serving import SweepRequest, SweepServer
-    stream = workloads.serving_stream(count=args.requests)
+    stream = workloads.serving_stream(count=args.requests, rate=args.rate)
     models = {}
     for d in stream:
         key = (d["style"], d["n"])
@@ -379,8 +383,9 @@ def bench_serving(args):
     pts = sum(r.shape[0] * r.shape[1] * (len(r.tlist) - 1) for r in done)
     out = {"metric": "sweep-request latency (arrival -> host result) under a Poisson stream",
            "unit": "s", "n_gpus": 1, "data": "synthetic", "vs_baseline": None,
-           "config": {"workload": "Poisson 20 req/s, LZS / N-level chain (N 2,4,8), sides "
-                                  "32..1024, rk4/tsit5a, fp32/fp64", "requests": len(done)},
+           "config": {"workload": f"Poisson {args.rate:g} req/s, LZS / N-level chain (N 2,4,8), "
+                                  "sides 32..1024, rk4/tsit5a, fp32/fp64", "requests": len(done),
+                      "rate": args.rate},
            "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
            "mean_s": float(lat.mean()), "max_s": float(lat.max()),
            "requests_per_s": len(done) / wall, "param_point_steps_per_s": pts / wall,
@@ -390,6 +395,7 @@ def bench_serving(args):
                                                         "gpu_s": v[2]}
                            for k, v in sorted(srv.gpu_by_kind.items())},
            "mean_batch_size": float(np.mean([r.batch_size for r in done])),
+           "offered_point_steps_per_s": pts / max(r.arrival for r in done),
            "peak_hbm_bytes": srv.peak_in_use,
            "nonfinite_requests": int(sum(1 for r in done if r.status))}
     print(json.dumps(out), flush=True)

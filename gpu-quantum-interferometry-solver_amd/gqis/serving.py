@@ -97,17 +97,71 @@ class SweepServer:
                 float(dt), warm, consts)
 
     def _module(self, r, key):
+        """Kernel, constants and time table of a batch key, resident on the device."""
         m = self.modules.get(key)
-        if m is None:
-            num_steps, dt, _, _, _ = _check_tlist(r.tlist)
-            spec = get_solver_spec(r.solver, step_size=dt, adaptive_options=r.adaptive_options,
-                                   fp64=r.fp64)
+        if m is not None:
+            return m
+        torch = self.torch
+        num_steps, dt, _, _, _ = _check_tlist(r.tlist)
+        spec = get_solver_spec(r.solver, step_size=dt, adaptive_options=r.adaptive_options,
+                               fp64=r.fp64)
+        rdt = torch.float64 if r.fp64 else torch.float32
+        consts_vals = list(key[-1]) or [0.0]
+        hm = self._hilbert_model(r) if spec.name == "rk4" else None
+        if hm is not None:
+            from . import hilbert
+            cvals, offs = hilbert.constants(hm, list(key[-1]))
+            block = hilbert.block_for(hm, r.fp64, self.block)
+            src = hilbert.emit(hm, offs, r.output_mode, block, r.fp64).replace(
+                "GQ_RHO0_FROM_ARGS", "0")
+            kname = codegen.kernel_name("gq_hilbert_rk4", r.fp64)
+            natoms = int(src.split("#define GQ_NATOMS ")[1].split()[0])
+            per_block = (block // 64) * (32 // hm.n)
+            consts_vals = cvals
+            table_kernel = codegen.kernel_name("gq_rk4_time_table", r.fp64)
+            rows = 2 * num_steps + 1
+        else:
+            block = self.block
             src = _emit_source(r.compiled, spec, r.output_mode, False, False, False, r.fp64, True,
-                               self.block, False)
-            mod, _, _ = self.ctx.compile(src, codegen.HIPRTC_BASE_OPTIONS)
-            m = (mod, spec, num_steps, dt)
-            self.modules[key] = m
+                               block, False)
+            kname = codegen.kernel_name(spec.kernel, r.fp64)
+            natoms = len(r.compiled.plan.time_atoms)
+            per_block = block
+            table_kernel = codegen.kernel_name(
+                "gq_rk4_time_table" if spec.template == "fixed_rk4.hip" else "gq_fixed_time_table",
+                r.fp64)
+            rows = rows_per_step(spec.name) * num_steps + 1
+        mod, _, _ = self.ctx.compile(src, codegen.HIPRTC_BASE_OPTIONS)
+        consts = torch.as_tensor(np.asarray(consts_vals, np.float64), dtype=rdt, device=self.dev)
+        table = None
+        if not spec.adaptive and natoms:
+            table = torch.empty(rows * natoms, dtype=rdt, device=self.dev)
+            a = runtime.SweepArgs()
+            a.dt, a.t_in, a.num_steps = float(dt), 0.0, num_steps
+            a.consts, a.time_table = consts.data_ptr(), table.data_ptr()
+            self.ctx.launch(mod, table_kernel, (rows + 255) // 256, 256, a,
+                            torch.cuda.current_stream(self.dev).cuda_stream)
+            torch.cuda.synchronize(self.dev)
+        m = {"module": mod, "spec": spec, "num_steps": num_steps, "dt": dt, "kernel": kname,
+             "block": block, "per_block": per_block, "consts": consts, "table": table,
+             "hilbert": hm is not None}
+        self.modules[key] = m
         return m
+
+    def _hilbert_model(self, r):
+        """Matrix-core model of a Lindblad request with N >= 16 whose operators fit the
+        Hilbert-space kernel, else None (smaller models run faster on the per-point kernels)."""
+        if r.compiled.kind != "lindblad" or r.output_mode not in ("mean", "final", "final_rho"):
+            return None
+        n = int(round(np.sqrt(r.compiled.state_size + 1)))
+        if n < 16:
+            return None
+        src = getattr(r.compiled, "source_model", None)
+        if src is None:
+            return None
+        from . import hilbert
+        from .solver import _drive_map
+        return hilbert.extract(src, _drive_map(src["Drive"]))
 
     def warm(self, requests):
         """Compile every kernel variant the requests need (before the stream starts)."""
@@ -152,13 +206,21 @@ class SweepServer:
             started += 1
         return started
 
+    def _pinned(self, nbytes):
+        key = 1 << max(12, (int(nbytes) - 1).bit_length())
+        pool = self.host_pool.setdefault(key, [])
+        buf = pool.pop() if pool else self.torch.empty(key, dtype=self.torch.uint8, pin_memory=True)
+        return key, buf
+
     def _launch(self, key, batch, npts, nbytes):
         torch = self.torch
-        mod, spec, num_steps, dt = self._module(batch[0], key)
+        h0 = time.perf_counter()
+        ent = self._module(batch[0], key)
+        spec, num_steps, dt = ent["spec"], ent["num_steps"], ent["dt"]
         r0 = batch[0]
         rdt = torch.float64 if r0.fp64 else torch.float32
         cdt = torch.complex128 if r0.fp64 else torch.complex64
-        h0 = time.perf_counter()
+        npdt = np.float64 if r0.fp64 else np.float32
         offsets, xs, ys = [], [], []
         off = 0
         for r in batch:
@@ -168,14 +230,17 @@ class SweepServer:
             ys.append(y)
             offsets.append(off)
             off += len(x) * len(y)
+        axes_host = np.concatenate(xs + ys).astype(npdt)
+        stage_key, stage_buf = self._pinned(axes_host.nbytes)
+        stage = stage_buf[:axes_host.nbytes].view(rdt)
+        stage.numpy()[:] = axes_host
         stream = self.free_streams.popleft()
         size = r0.compiled.state_size
         mode = r0.output_mode
         with torch.cuda.stream(stream):
             start_ev = torch.cuda.Event(enable_timing=True)
             stop_ev = torch.cuda.Event(enable_timing=True)
-            # axes go up as they are (small); the per-point lists are expanded on the device
-            ax = torch.as_tensor(np.concatenate(xs + ys), dtype=rdt).to(self.dev, non_blocking=True)
+            ax = stage.to(self.dev, non_blocking=True)
             parts_x, parts_y, pos = [], [], 0
             for x, y in zip(xs, ys):
                 parts_x.append(ax[pos:pos + len(x)].repeat_interleave(len(y)))
@@ -185,18 +250,11 @@ class SweepServer:
                 pos += len(y)
             pxd = torch.cat(parts_x)
             pyd = torch.cat(parts_y)
-            consts = torch.as_tensor(np.asarray(key[-1] or (0.0,), np.float64), dtype=rdt,
-                                     device=self.dev)
             if mode in ("final_rho", "mean_state"):
                 result = torch.empty((size, npts), dtype=rdt, device=self.dev)
             else:
                 result = torch.empty(npts, dtype=cdt, device=self.dev)
-            status = torch.zeros(1, dtype=torch.int32, device=self.dev)
-            cursor = torch.zeros(1, dtype=torch.int32, device=self.dev)
-            natoms = len(r0.compiled.plan.time_atoms)
-            rows = rows_per_step(spec.name) * num_steps + 1
-            table = (torch.empty(rows * natoms, dtype=rdt, device=self.dev)
-                     if not spec.adaptive and natoms else None)
+            flags_dev = torch.zeros(2, dtype=torch.int32, device=self.dev)   # status, cursor
             a = runtime.SweepArgs()
             a.dt, a.t_in = float(dt), 0.0
             a.num_x, a.num_y = npts, 1
@@ -204,44 +262,41 @@ class SweepServer:
             a.num_steps, a.warmup_steps = num_steps, int(key[7])
             a.trace_stride, a.trace_offset, a.num_trace, a.trace_point_stride = 1, 0, 0, npts
             a.param_mode = 1
-            a.par_x, a.par_y, a.consts = pxd.data_ptr(), pyd.data_ptr(), consts.data_ptr()
-            a.time_table = table.data_ptr() if table is not None else None
+            a.par_x, a.par_y, a.consts = pxd.data_ptr(), pyd.data_ptr(), ent["consts"].data_ptr()
+            a.time_table = ent["table"].data_ptr() if ent["table"] is not None else None
             for i, v in enumerate(spec.parameters[:runtime.MAX_SOLVER_PARAMS]):
                 a.solver_param[i] = float(v)
-            a.result, a.status, a.work_cursor = result.data_ptr(), status.data_ptr(), cursor.data_ptr()
+            a.result = result.data_ptr()
+            a.status = flags_dev.data_ptr()
+            a.work_cursor = flags_dev.data_ptr() + 4
             sid = stream.cuda_stream
             start_ev.record(stream)
-            if table is not None:
-                self.ctx.launch(mod, codegen.kernel_name(
-                    "gq_rk4_time_table" if spec.template == "fixed_rk4.hip" else "gq_fixed_time_table",
-                    r0.fp64), (rows + 255) // 256, 256, a, sid)
-            kname = codegen.kernel_name(spec.kernel, r0.fp64)
-            grid = (npts + self.block - 1) // self.block
+            grid = (npts + ent["per_block"] - 1) // ent["per_block"]
             if spec.adaptive and spec.options.get("persistent", True):
-                grid = min(grid, self.ctx.resident_groups(mod, kname, self.block))
-            self.ctx.launch(mod, kname, max(1, grid), self.block, a, sid)
+                if "resident" not in ent:
+                    ent["resident"] = self.ctx.resident_groups(ent["module"], ent["kernel"],
+                                                               ent["block"])
+                grid = min(grid, ent["resident"])
+            self.ctx.launch(ent["module"], ent["kernel"], max(1, grid), ent["block"], a, sid)
             stop_ev.record(stream)
             nb = result.numel() * result.element_size()
-            host_key = (1 << max(12, (nb - 1).bit_length()))
-            pool = self.host_pool.setdefault(host_key, [])
-            buf = pool.pop() if pool else torch.empty(host_key, dtype=torch.uint8, pin_memory=True)
+            host_key, buf = self._pinned(nb + 8)
             host = buf[:nb].view(result.dtype).view(result.shape)
             host.copy_(result, non_blocking=True)
-            status_host = torch.empty(1, dtype=torch.int32, pin_memory=True)
-            status_host.copy_(status, non_blocking=True)
+            status_host = buf[nb:nb + 8].view(torch.int32)
+            status_host.copy_(flags_dev, non_blocking=True)
             event = torch.cuda.Event()
             event.record(stream)
-        # keep device tensors alive until the launch completes
-        launch = _Launch(batch, stream, event, (result, pxd, pyd, consts, status, cursor, table, ax),
-                         host, status, status_host, nbytes, offsets, mode, size, start_ev, stop_ev,
-                         (host_key, buf))
-        self.host_launch_s += time.perf_counter() - h0
+        launch = _Launch(batch, stream, event, (result, pxd, pyd, flags_dev, ax), host, flags_dev,
+                         status_host, nbytes, offsets, mode, size, start_ev, stop_ev,
+                         [(host_key, buf), (stage_key, stage_buf)])
         self.running.append(launch)
         self.in_use += nbytes
         self.peak_in_use = max(self.peak_in_use, self.in_use)
         self.launches += 1
         for r in batch:
             r.batch_size = len(batch)
+        self.host_launch_s += time.perf_counter() - h0
 
     def _complete(self, now_fn):
         done = [l for l in self.running if l.event.query()]
@@ -260,7 +315,8 @@ class SweepServer:
             acc[1] += sum(r.shape[0] * r.shape[1] for r in l.requests)
             acc[2] += ks
             host = l.host.numpy().copy()
-            self.host_pool.setdefault(l.host_key[0], []).append(l.host_key[1])
+            for k, b in l.host_key:
+                self.host_pool.setdefault(k, []).append(b)
             for r, off in zip(l.requests, l.offsets):
                 nx, ny = r.shape
                 if l.mode in ("final_rho", "mean_state"):

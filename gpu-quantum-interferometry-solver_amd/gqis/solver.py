@@ -46,6 +46,7 @@ class CompiledRHS:
     timings: dict
     ode: object = None
     emitted: dict = field(default_factory=dict, repr=False)
+    source_model: dict = field(default=None, repr=False)   # prepared operators (Lindblad)
 
 
 def _rename_time(expr):
@@ -174,7 +175,8 @@ def _compile_rhs(H, Drive, Col_Ops, mean_operator, *, sweep_symbols, const_value
         runtime_const_syms=tuple(model["runtime_const_syms"]),
         base_runtime_syms=tuple(model["runtime_const_syms"]), derived_const_exprs=(),
         const_values=model["defaults"], plan=plan, cache_key=key,
-        timings={"ode_prepare_s": prepared - start, "ode_optimize_s": time.time() - prepared})
+        timings={"ode_prepare_s": prepared - start, "ode_optimize_s": time.time() - prepared},
+        source_model=model if ode_size is None else None)
     if reuse:
         _RHS_CODE_CACHE[key] = compiled
     return compiled, "miss"
