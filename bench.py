@@ -349,7 +349,7 @@ def bench_scaling(args):
 
 
 def bench_serving(args):
-    """Poisson stream (20 requests/s, seeded) of sweep requests: two-level LZS or N-level chain
+    """Poisson stream (--rate requests/s, seeded) of sweep requests: two-level LZS or N-level chain
     models (N in {2, 4, 8}), square grids of side 32/128/512/1024, random amplitude ranges,
     RK4 or Tsit5A, FP32 or FP64, served with continuous batching on 4 streams under a 16 GiB
     HBM budget.  Reports throughput and arrival-to-host-result latency percentiles."""
@@ -377,7 +377,8 @@ def bench_serving(args):
     srv.warm(reqs)
     log(f"serving {len(reqs)} requests ...")
     t0 = time.perf_counter()
-    done = srv.serve(reqs, realtime=True)
+    done = srv.serve(reqs, realtime=True,
+                     progress=lambda f, s, t: log(f"  {f} finished, {s} submitted, {t:.1f} s"))
     wall = time.perf_counter() - t0
     lat = np.array([r.done_at - r.arrival for r in done])
     pts = sum(r.shape[0] * r.shape[1] * (len(r.tlist) - 1) for r in done)

@@ -331,10 +331,11 @@ class SweepServer:
         return len(done)
 
     # -- driving -------------------------------------------------------------------------------
-    def serve(self, requests, realtime=True, poll_s=0.0005):
+    def serve(self, requests, realtime=True, poll_s=0.0005, progress=None, progress_every=100):
         """Feed requests at their arrival times (seconds from start; all at once when
         ``realtime`` is False) and run to completion.  Returns the finished requests with
-        ``done_at`` relative to the start."""
+        ``done_at`` relative to the start.  ``progress(finished, submitted, t)`` is called each
+        time another ``progress_every`` requests have finished."""
         order = sorted(requests, key=lambda r: r.arrival)
         if not realtime:
             for r in order:
@@ -350,7 +351,12 @@ class SweepServer:
                 self.submit(order[i])
                 i += 1
             self._admit(t)
-            if not self._complete(now):
+            before = len(self.finished)
+            completed = self._complete(now)
+            if progress is not None and (len(self.finished) // progress_every
+                                         > before // progress_every):
+                progress(len(self.finished), i, now())
+            if not completed:
                 if realtime and i < len(order) and not self.running:
                     time.sleep(max(0.0, min(poll_s, order[i].arrival - now())))
                 else:

@@ -296,8 +296,9 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
                            liouville="auto", _ode_size=None, _state_trace=False,
                            _time_trace_offset=0):
     if calibrate_timesteps is not None and calibrate_timesteps is not False:
+        call = dict(locals())
         from .calibration import solve_with_timestep_calibration
-        return solve_with_timestep_calibration(dict(locals()))
+        return solve_with_timestep_calibration(call)
     if output_mode not in {"mean", "final", "final_rho", "mean_state"}:
         raise ValueError(f"Unsupported output_mode '{output_mode}'. "
                          f"Use one of: {sorted(['final', 'final_rho', 'mean', 'mean_state'])}")
