@@ -3,9 +3,13 @@
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
+#include <unistd.h>
+
+#include <cerrno>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <map>
 #include <mutex>
 #include <string>
@@ -40,6 +44,36 @@ struct Compiled {
     std::string log;
     int rc = GQ_OK;
 };
+
+// Code-object files of the on-disk cache: "<dir>/<arch>-<key>-<check>.hsaco", where key and
+// check are two independent 64-bit hashes of the same key text (compiler version, architecture,
+// options, source).  Written to a temporary name and renamed, so a reader never sees a partial
+// file; a file that fails to load is ignored and rebuilt.
+bool read_file(const std::string& path, std::vector<char>* out) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    f.seekg(0, std::ios::end);
+    std::streamoff n = f.tellg();
+    if (n <= 0) return false;
+    out->resize((size_t)n);
+    f.seekg(0, std::ios::beg);
+    return (bool)f.read(out->data(), n);
+}
+
+void write_file_atomic(const std::string& path, const std::vector<char>& data) {
+    std::string tmp = path + "." + std::to_string((long long)getpid()) + ".tmp";
+    {
+        std::ofstream f(tmp, std::ios::binary | std::ios::trunc);
+        if (!f) return;
+        f.write(data.data(), (std::streamsize)data.size());
+        if (!f) {
+            f.close();
+            std::remove(tmp.c_str());
+            return;
+        }
+    }
+    if (std::rename(tmp.c_str(), path.c_str()) != 0) std::remove(tmp.c_str());
+}
 
 Compiled hiprtc_compile(const char* source, const std::string& arch, const char* const* options,
                         int num_options) {
@@ -91,6 +125,7 @@ struct gq_ctx {
     int device = 0;
     std::string arch;
     int compute_units = 0;
+    std::string cache_dir;   // on-disk code-object cache; empty = disabled
     std::mutex mu;
     std::map<unsigned long long, gq_module*> modules;
 };
@@ -140,6 +175,14 @@ int gq_ctx_info(gq_ctx* ctx, char* arch, int arch_len, int* compute_units) {
     return GQ_OK;
 }
 
+int gq_ctx_set_cache_dir(gq_ctx* ctx, const char* dir) {
+    if (!ctx) return fail(GQ_E_INVALID, "gq_ctx_set_cache_dir: null context");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->cache_dir = dir ? dir : "";
+    while (ctx->cache_dir.size() > 1 && ctx->cache_dir.back() == '/') ctx->cache_dir.pop_back();
+    return GQ_OK;
+}
+
 int gq_compile_check(const char* source, const char* arch, const char* const* options,
                      int num_options, long long* code_size) {
     if (!source || !arch) return fail(GQ_E_INVALID, "gq_compile_check: null source or arch");
@@ -153,13 +196,17 @@ int gq_module_compile(gq_ctx* ctx, const char* source, const char* const* option
                       gq_module** out, int* cached_out, double* build_seconds) {
     if (!ctx || !source || !out) return fail(GQ_E_INVALID, "gq_module_compile: null argument");
     auto t0 = std::chrono::steady_clock::now();
-    std::string key_text = ctx->arch;
+    int rtc_major = 0, rtc_minor = 0;
+    (void)hiprtcVersion(&rtc_major, &rtc_minor);
+    std::string key_text = "hiprtc " + std::to_string(rtc_major) + "." +
+                           std::to_string(rtc_minor) + '\0' + ctx->arch;
     key_text.push_back('\0');
     for (int i = 0; i < num_options; ++i) {
         key_text += options[i];
         key_text.push_back('\0');
     }
     unsigned long long key = fnv1a(source, fnv1a(key_text));
+    std::string dir;
     {
         std::lock_guard<std::mutex> lock(ctx->mu);
         auto it = ctx->modules.find(key);
@@ -170,14 +217,34 @@ int gq_module_compile(gq_ctx* ctx, const char* source, const char* const* option
             if (build_seconds) *build_seconds = 0.0;
             return GQ_OK;
         }
+        dir = ctx->cache_dir;
     }
-    Compiled c = hiprtc_compile(source, ctx->arch, options, num_options);
-    if (c.rc != GQ_OK) return fail(c.rc, c.log);
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
     hipModule_t mod = nullptr;
-    e = hipModuleLoadData(&mod, c.code.data());
-    if (e != hipSuccess) return hip_fail(e, "hipModuleLoadData");
+    int cached = 0;
+    std::string path;
+    if (!dir.empty()) {
+        // second hash: FNV-1a with another offset basis over the same text
+        unsigned long long check = fnv1a(source, fnv1a(key_text, 0x84222325cbf29ce4ull));
+        char name[96];
+        std::snprintf(name, sizeof(name), "/%s-%016llx-%016llx.hsaco", ctx->arch.c_str(), key,
+                      check);
+        path = dir + name;
+        std::vector<char> code;
+        if (read_file(path, &code) && hipModuleLoadData(&mod, code.data()) == hipSuccess) {
+            cached = 2;
+        } else {
+            mod = nullptr;
+        }
+    }
+    if (!mod) {
+        Compiled c = hiprtc_compile(source, ctx->arch, options, num_options);
+        if (c.rc != GQ_OK) return fail(c.rc, c.log);
+        e = hipModuleLoadData(&mod, c.code.data());
+        if (e != hipSuccess) return hip_fail(e, "hipModuleLoadData");
+        if (!path.empty()) write_file_atomic(path, c.code);
+    }
     gq_module* m = new gq_module();
     m->ctx = ctx;
     m->module = mod;
@@ -187,7 +254,7 @@ int gq_module_compile(gq_ctx* ctx, const char* source, const char* const* option
         ctx->modules[key] = m;
     }
     *out = m;
-    if (cached_out) *cached_out = 0;
+    if (cached_out) *cached_out = cached;
     if (build_seconds) {
         *build_seconds =
             std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

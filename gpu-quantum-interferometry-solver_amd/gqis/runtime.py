@@ -75,6 +75,7 @@ def load_library(path=LIB_PATH):
             "gq_module_compile": (c_int, [c_void_p, c_char_p, ctypes.POINTER(c_char_p), c_int,
                                           ctypes.POINTER(c_void_p), ctypes.POINTER(c_int),
                                           ctypes.POINTER(ctypes.c_double)]),
+            "gq_ctx_set_cache_dir": (c_int, [c_void_p, c_char_p]),
             "gq_compile_check": (c_int, [c_char_p, c_char_p, ctypes.POINTER(c_char_p), c_int,
                                          ctypes.POINTER(ctypes.c_longlong)]),
             "gq_launch_sweep": (c_int, [c_void_p, c_char_p, ctypes.c_uint, ctypes.c_uint,
@@ -105,7 +106,8 @@ def load_library(path=LIB_PATH):
 
 def exported_symbols():
     """Names the header declares (checked by the CPU tests against the built library)."""
-    return ("gq_ctx_create", "gq_ctx_destroy", "gq_ctx_info", "gq_module_compile",
+    return ("gq_ctx_create", "gq_ctx_destroy", "gq_ctx_info", "gq_ctx_set_cache_dir",
+            "gq_module_compile",
             "gq_compile_check", "gq_launch_sweep", "gq_resident_groups", "gq_module_release",
             "gq_event_create", "gq_event_record", "gq_event_elapsed_ms", "gq_event_destroy",
             "gq_stream_synchronize", "gq_status_read", "gq_sweep_args_size", "gq_last_error")
@@ -160,6 +162,19 @@ class Context:
         self.arch = buf.value.decode()
         self.compute_units = cus.value
         self.modules = {}
+        self.cache_dir = None
+        if os.environ.get("GQIS_KERNEL_CACHE_DIR"):
+            self.set_cache_dir(os.environ["GQIS_KERNEL_CACHE_DIR"])
+
+    def set_cache_dir(self, path):
+        """Keep compiled code objects in ``path`` (created if missing) so that a new process
+        loads them instead of compiling; ``None`` turns the on-disk cache off."""
+        if path:
+            path = os.path.abspath(os.path.expanduser(str(path)))
+            os.makedirs(path, exist_ok=True)
+        _check(self.lib.gq_ctx_set_cache_dir(self.handle, path.encode() if path else None),
+               "gq_ctx_set_cache_dir")
+        self.cache_dir = path or None
 
     @classmethod
     def get(cls, device=None):
@@ -173,7 +188,8 @@ class Context:
             return ctx
 
     def compile(self, source, options):
-        """hiprtc compile + load (cached by the library); returns (module, cached, seconds)."""
+        """hiprtc compile + load; returns (module, cached, seconds) with cached 0 (compiled),
+        1 (module cache of this process) or 2 (code object from the on-disk cache)."""
         arr, n = _c_options(options)
         mod = ctypes.c_void_p()
         cached = ctypes.c_int(0)
@@ -181,7 +197,7 @@ class Context:
         _check(self.lib.gq_module_compile(self.handle, source.encode(), arr, n, ctypes.byref(mod),
                                           ctypes.byref(cached), ctypes.byref(secs)),
                "hiprtc compile")
-        return mod, bool(cached.value), secs.value
+        return mod, int(cached.value), secs.value
 
     def launch(self, module, name, grid, block, args, stream=None):
         if stream is None:

@@ -258,6 +258,14 @@ def mesolve_2D(H, Drive=None, Col_Ops=None, mean_operator=None, tlist=None,
     return _solve_symbolic_system(**args)
 
 
+def set_kernel_cache_dir(path, device=None):
+    """Keep compiled sweep kernels in ``path`` across processes (``None`` turns it off).  The
+    environment variable ``GQIS_KERNEL_CACHE_DIR`` sets the same at the first sweep.  The
+    reference's kernel cache is per process only (``gqis/solver.py:2054-2060``)."""
+    from .runtime import Context
+    Context.get(device).set_cache_dir(path)
+
+
 def _check_tlist(tlist):
     """(num_steps, dt) of a uniform grid starting at zero (``gqis/solver.py:1951-1979``)."""
     if tlist is None:
@@ -491,7 +499,7 @@ def _finish(outputs, stage_s, rhs_status, spec, solver_frequency, unroll, collec
                 "ode_prepare_s": stage_s["ode_prepare_s"],
                 "ode_optimize_s": stage_s["ode_optimize_s"], "code_emit_s": stage_s["emit_s"],
                 "kernel_build_s": outputs["build_s"], "cached_rhs": rhs_status,
-                "cached_kernel": "hit" if outputs["cached_kernel"] else "miss",
+                "cached_kernel": {0: "miss", 1: "hit", 2: "disk"}[int(outputs["cached_kernel"])],
                 "solver": spec.name, "solver_frequency": solver_frequency, "unroll": unroll}
         if outputs.get("adaptive_steps") is not None:
             counts = outputs["adaptive_steps"]

@@ -88,9 +88,14 @@ void gq_ctx_destroy(gq_ctx* ctx);
 int gq_ctx_info(gq_ctx* ctx, char* arch, int arch_len, int* compute_units);
 
 /* Compile HIP source with hiprtc for the context's architecture and load it.  Modules are
- * cached in the context by a hash of source, options and architecture; a cache hit returns
- * the loaded module (cached_out = 1).  Replaces cp.RawKernel(...).compile()
- * (gqis/solver.py:2237-2241). */
+ * cached in the context by a hash of compiler version, source, options and architecture; a
+ * cache hit returns the loaded module (cached_out = 1); with an on-disk cache directory set, a
+ * code object found there is loaded without compiling (cached_out = 2) and new ones are stored
+ * there.  Replaces cp.RawKernel(...).compile() (gqis/solver.py:2237-2241). */
+/* Directory of the on-disk code-object cache (NULL or "" disables it; it must exist).  No
+ * counterpart in the reference, whose kernel cache lives only in the process
+ * (gqis/solver.py:2054-2060): it removes the compile from a new process's first sweep. */
+int gq_ctx_set_cache_dir(gq_ctx* ctx, const char* dir);
 int gq_module_compile(gq_ctx* ctx, const char* source, const char* const* options, int num_options,
                       gq_module** out, int* cached_out, double* build_seconds);
 /* Compile only (no device needed): returns the code-object size, or the log on failure.

@@ -1,0 +1,71 @@
+"""HIP kernels against the committed FP64 golden fixtures (tests/golden/, written by the oracle
+on exact subsets of the benchmark grids): FP64 within 1e-10 of each map's scale."""
+import os
+
+import numpy as np
+import pytest
+
+from gqis import mesolve_2D, odesolve_2D, workloads
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _load(name):
+    return np.load(os.path.join(HERE, name))
+
+
+def _rel(got, ref):
+    return float(np.max(np.abs(got - ref)) / max(1e-300, float(np.max(np.abs(ref)))))
+
+
+def _sweep(w, eps, amp, **kw):
+    e, a = list(w["var_arrays"])
+    return mesolve_2D(w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"],
+                      var_arrays={e: eps, a: amp}, **kw)
+
+
+def test_lzs_64x64_fixture(gpu):
+    g = _load("c1_lzs_64x64_rk4_mean_fp64.npz")
+    got = _sweep(workloads.lzs_two_level(), g["eps"], g["amp"], fp64=True)
+    assert got.shape == (64, 64)
+    assert _rel(got, g["result"]) <= 1e-10
+
+
+def test_four_level_benchmark_subset_fixture(gpu):
+    g = _load("c2_four_level_16x16_subset_rk4_mean_fp64.npz")
+    w = workloads.four_level(n_eps=16, n_amp=16)
+    for tab in (True, False):       # tabulated time terms and the per-point form
+        got = _sweep(w, g["eps"], g["amp"], fp64=True, time_table=tab)
+        assert _rel(got, g["result"]) <= 1e-10
+
+
+@pytest.mark.parametrize("n,liouville", [(4, "lanes"), (8, "lanes"), (8, "mfma")])
+def test_random_n_level_fixture_fp64(gpu, n, liouville):
+    g = _load(f"c3_random_{n}_level_8x8_subset_rk4_mean_fp64.npz")
+    w = workloads.random_n_level(n, seed=int(g["seed"]), n_eps=8, n_amp=8)
+    got = _sweep(w, g["eps"], g["amp"], fp64=True, liouville=liouville)
+    assert _rel(got, g["result"]) <= 1e-10
+
+
+def test_random_8_level_fixture_fp32_mfma(gpu):
+    # FP32 matrix-core products and FP32 state over 2000 steps: within 2e-5 of the map's scale
+    g = _load("c3_random_8_level_8x8_subset_rk4_mean_fp64.npz")
+    w = workloads.random_n_level(8, seed=int(g["seed"]), n_eps=8, n_amp=8)
+    got = _sweep(w, g["eps"], g["amp"], fp64=False, liouville="mfma")
+    assert _rel(got, g["result"]) <= 2e-5
+
+
+def test_duffing_tsit5a_fixture(gpu):
+    g = _load("c4_duffing_8x8_subset_t20_tsit5a_fp64.npz")
+    w = workloads.duffing(t_end=float(g["t_end"]), n_out=int(g["n_t"]))
+    gs, xs = list(w["var_arrays"])
+    res, info = odesolve_2D(w["rhs"], w["state_symbols"], w["y0"], w["tlist"],
+                            var_arrays={gs: g["gamma"], xs: g["x0"]}, solver="tsit5a",
+                            fp64=True, return_timing_info=True,
+                            adaptive_options={"step_rtol": float(g["rtol"]),
+                                              "step_atol": float(g["atol"]),
+                                              "max_substeps": 1000, "count_steps": True})
+    np.testing.assert_array_equal(info["adaptive_steps"]["accepted"], g["accepted"])
+    np.testing.assert_array_equal(info["adaptive_steps"]["rejected"], g["rejected"])
+    assert _rel(res, g["result"]) <= 1e-10
