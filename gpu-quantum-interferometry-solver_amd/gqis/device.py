@@ -8,7 +8,7 @@ import time
 
 import numpy as np
 
-from . import codegen, runtime
+from . import autotune, codegen, runtime
 
 
 def _axis(values, dtype, device):
@@ -52,7 +52,11 @@ def run_sweep(*, source, variant, spec, compiled, mode, var_arrays, rho0_values,
     npts = nx * ny
     size = compiled.state_size
 
-    module, cached, build_s = ctx.compile(source, codegen.HIPRTC_BASE_OPTIONS + tuple(options))
+    options = codegen.HIPRTC_BASE_OPTIONS + tuple(options)
+    tune = block_size == "auto"
+    if tune:
+        block_size = 256       # the emitted source's size; measured below
+    module, cached, build_s = ctx.compile(source, options)
     rho0 = (_initial_states(rho0_values, nx, ny, size, rdt, dev)
             if rho0_values is not None else None)
     consts = torch.as_tensor(np.asarray(const_vals if const_vals else [0.0], np.float64),
@@ -99,16 +103,42 @@ def run_sweep(*, source, variant, spec, compiled, mode, var_arrays, rho0_values,
     a.work_cursor = cursor.data_ptr()
 
     stream = torch.cuda.current_stream(dev).cuda_stream
+    kname = codegen.kernel_name(spec.kernel, fp64)
+    persistent = spec.adaptive and spec.options.get("persistent", True)
+    tune_s, tune_times = 0.0, {}
+    if tune and npts > 0 and num_steps > 0:
+        t0 = time.time()
+        if use_table and rows > 0:
+            ctx.launch(module, codegen.kernel_name(table_kernel(spec), fp64), (rows + 255) // 256,
+                       256, a, stream)
+        trial_steps = min(int(num_steps), 64)
+
+        def trial(mod, block):
+            t = runtime.SweepArgs.from_buffer_copy(a)
+            t.num_steps, t.warmup_steps, t.num_trace, t.trace = trial_steps, 0, 0, None
+            grid = (npts + block - 1) // block
+            if persistent:
+                grid = min(ctx.resident_groups(mod, kname, block), grid)
+            return _timed_launches(ctx, mod, kname, max(grid, 1), block, t, stream, cursor)
+
+        if npts * trial_steps >= autotune.MIN_TRIAL_POINT_STEPS:
+            block_size, tune_times = autotune.choose(ctx, source, options, trial)
+        else:
+            key = autotune.source_key(source, options, ctx.arch)
+            block_size = autotune.lookup(ctx, key) or 256
+        module, _, _ = ctx.compile(autotune.with_block(source, block_size), options)
+        ctx.read_status(status, stream, clear=True)
+        cursor.zero_()
+        tune_s = time.time() - t0
     timer = runtime.EventTimer(ctx, stream) if collect else None
     setup_s = time.time() - setup_start
     if npts > 0 and num_steps >= 0:
         if timer:
             timer.record_start()
-        kname = codegen.kernel_name(spec.kernel, fp64)
         if use_table and rows > 0:
             ctx.launch(module, codegen.kernel_name(table_kernel(spec), fp64), (rows + 255) // 256,
                        256, a, stream)
-        if spec.adaptive and spec.options.get("persistent", True):
+        if persistent:
             grid = min(ctx.resident_groups(module, kname, block_size),
                        (npts + block_size - 1) // block_size)
         else:
@@ -133,7 +163,26 @@ def run_sweep(*, source, variant, spec, compiled, mode, var_arrays, rho0_values,
             tr = trace.t().reshape(nx, ny, num_trace).contiguous()
     steps = steps_out.reshape(nx, ny, 2).cpu().numpy() if steps_out is not None else None
     return {"result": out, "trace": tr, "status": flags, "gpu_s": gpu_s, "build_s": build_s,
-            "cached_kernel": cached, "setup_s": setup_s, "adaptive_steps": steps}
+            "cached_kernel": cached, "setup_s": setup_s, "adaptive_steps": steps,
+            "block_size": int(block_size), "autotune_s": tune_s, "autotune": tune_times}
+
+
+def _timed_launches(ctx, module, kname, grid, block, args, stream, cursor, repeats=2):
+    """Shortest of ``repeats`` timed launches after one untimed (first use of a module)."""
+    timer = runtime.EventTimer(ctx, stream)
+    best = float("inf")
+    try:
+        for rep in range(repeats + 1):
+            cursor.zero_()
+            timer.record_start()
+            ctx.launch(module, kname, grid, block, args, stream)
+            timer.record_stop()
+            ms = timer.elapsed_ms()
+            if rep:
+                best = min(best, ms)
+    finally:
+        timer.close()
+    return best / 1000.0
 
 
 def table_kernel(spec):

@@ -171,8 +171,23 @@ def constants(m, runtime_values):
     return np.concatenate(buf).astype(np.float64), offs
 
 
+LDS_BYTES_PER_CU = 160 * 1024      # gfx950
+
+
+def lds_bytes(n, fp64, block, jmax):
+    """Static LDS of the kernel (the ``__shared__`` arrays of ``kernels/hilbert_rk4.hip``)."""
+    r = 8 if fp64 else 4
+    waves, pts, tiles = block // 64, 32 // n, (2 * n + 31) // 32
+    total = waves * 2 * pts * n * (n + 1) * r + waves * pts * n * r + n * n * r
+    total += n * jmax * (r + 4) + waves * 64 * 8
+    total += (4 * n * n * 8 + waves * pts * 2 * n * n * 8) if fp64 else tiles * n * 64 * 4
+    return total
+
+
 def block_for(m, fp64, block_size=256):
     """Work-group size: the FP64 N = 32 kernel keeps its LDS under 160 KB with two waves."""
+    if block_size == "auto":
+        return block_size
     return 128 if (fp64 and m.n == 32) else block_size
 
 
@@ -235,13 +250,17 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
     consts, offs = constants(m, runtime_values)
     uses_rho0 = rho0_values is not None
     block_size = block_for(m, fp64, block_size)
+    tune = block_size == "auto"
+    if tune:
+        block_size = block_for(m, fp64, 256)
     skey = (m.key, mode, block_size, uses_rho0, tuple(options), bool(fp64))
     src = _SOURCES.get(skey)
     if src is None:
         src = emit(m, offs, mode, block_size, fp64).replace(
             "GQ_RHO0_FROM_ARGS", "1" if uses_rho0 else "0")
         _SOURCES[skey] = src
-    module, cached, build_s = ctx.compile(src, codegen.HIPRTC_BASE_OPTIONS + tuple(options))
+    options = codegen.HIPRTC_BASE_OPTIONS + tuple(options)
+    module, cached, build_s = ctx.compile(src, options)
     rdt = torch.float64 if fp64 else torch.float32
     axes = list(var_arrays.values())
     px = _axis(axes[0], rdt, dev)
@@ -271,6 +290,36 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
     a.time_table = table.data_ptr()
     a.result, a.status = result.data_ptr(), status.data_ptr()
     stream = torch.cuda.current_stream(dev).cuda_stream
+    kname = codegen.kernel_name("gq_hilbert_rk4", fp64)
+    tune_s, tune_times = 0.0, {}
+    if tune and npts > 0 and num_steps > 0:
+        from . import autotune
+        from .device import _timed_launches
+        t0 = time.time()
+        if n_atoms > 0:
+            ctx.launch(module, codegen.kernel_name("gq_rk4_time_table", fp64), (rows + 255) // 256,
+                       256, a, stream)
+        trial_steps = min(int(num_steps), 64)
+        jmax = jump_width(m)
+        fits = tuple(b for b in autotune.CANDIDATES
+                     if lds_bytes(m.n, fp64, b, jmax) <= LDS_BYTES_PER_CU)
+        spare = torch.zeros(1, dtype=torch.int32, device=dev)
+
+        def trial(mod, block):
+            t = runtime.SweepArgs.from_buffer_copy(a)
+            t.num_steps, t.warmup_steps = trial_steps, 0
+            per = (block // 64) * (32 // m.n)
+            return _timed_launches(ctx, mod, kname, (npts + per - 1) // per, block, t, stream,
+                                   spare)
+
+        if npts * trial_steps >= autotune.MIN_TRIAL_POINT_STEPS:
+            block_size, tune_times = autotune.choose(ctx, src, options, trial, fits)
+        else:
+            block_size = autotune.lookup(ctx, autotune.source_key(src, options, ctx.arch)) \
+                or block_size
+        module, _, _ = ctx.compile(autotune.with_block(src, block_size), options)
+        ctx.read_status(status, stream, clear=True)
+        tune_s = time.time() - t0
     timer = runtime.EventTimer(ctx, stream) if collect else None
     setup_s = time.time() - setup_start
     per_block = (block_size // 64) * (32 // m.n)
@@ -280,8 +329,7 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
         if n_atoms > 0:
             ctx.launch(module, codegen.kernel_name("gq_rk4_time_table", fp64), (rows + 255) // 256,
                        256, a, stream)
-        ctx.launch(module, codegen.kernel_name("gq_hilbert_rk4", fp64),
-                   (npts + per_block - 1) // per_block, block_size, a, stream)
+        ctx.launch(module, kname, (npts + per_block - 1) // per_block, block_size, a, stream)
         if timer:
             timer.record_stop()
     flags = ctx.read_status(status, stream)
@@ -293,4 +341,5 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
     else:
         out = result[:npts].reshape(nx, ny)
     return {"result": out, "trace": None, "status": flags, "gpu_s": gpu_s, "build_s": build_s,
-            "cached_kernel": cached, "setup_s": setup_s, "adaptive_steps": None}
+            "cached_kernel": cached, "setup_s": setup_s, "adaptive_steps": None,
+            "block_size": int(block_size), "autotune_s": tune_s, "autotune": tune_times}

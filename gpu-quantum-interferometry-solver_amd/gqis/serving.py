@@ -19,7 +19,7 @@ import time
 
 import numpy as np
 
-from . import codegen, runtime
+from . import autotune, codegen, runtime
 from .solver import _emit_source, _runtime_const_values, _check_tlist
 from .solvers import get_solver_spec
 from .tableaux import rows_per_step
@@ -63,7 +63,7 @@ class _Launch:
 
 class SweepServer:
     def __init__(self, device=0, streams=4, hbm_budget_bytes=16 << 30,
-                 max_batch_points=1 << 22, block_size=256):
+                 max_batch_points=1 << 22, block_size="auto"):
         import torch
         self.torch = torch
         self.ctx = runtime.Context.get(device)
@@ -73,7 +73,7 @@ class SweepServer:
         self.budget = int(hbm_budget_bytes)
         self.in_use = 0
         self.max_batch_points = int(max_batch_points)
-        self.block = int(block_size)
+        self.block = block_size if block_size == "auto" else int(block_size)
         self.pending = {}        # batch key -> deque of requests
         self.running = []
         self.finished = []
@@ -111,22 +111,26 @@ class SweepServer:
         if hm is not None:
             from . import hilbert
             cvals, offs = hilbert.constants(hm, list(key[-1]))
-            block = hilbert.block_for(hm, r.fp64, self.block)
+            block = hilbert.block_for(hm, r.fp64, 256 if self.block == "auto" else self.block)
             src = hilbert.emit(hm, offs, r.output_mode, block, r.fp64).replace(
                 "GQ_RHO0_FROM_ARGS", "0")
             kname = codegen.kernel_name("gq_hilbert_rk4", r.fp64)
             natoms = int(src.split("#define GQ_NATOMS ")[1].split()[0])
-            per_block = (block // 64) * (32 // hm.n)
+            fits = tuple(b for b in autotune.CANDIDATES
+                         if hilbert.lds_bytes(hm.n, r.fp64, b, hilbert.jump_width(hm))
+                         <= hilbert.LDS_BYTES_PER_CU)
+            points_per_block = lambda b: (b // 64) * (32 // hm.n)     # noqa: E731
             consts_vals = cvals
             table_kernel = codegen.kernel_name("gq_rk4_time_table", r.fp64)
             rows = 2 * num_steps + 1
         else:
-            block = self.block
+            block = 256 if self.block == "auto" else self.block
             src = _emit_source(r.compiled, spec, r.output_mode, False, False, False, r.fp64, True,
                                block, False)
             kname = codegen.kernel_name(spec.kernel, r.fp64)
             natoms = len(r.compiled.plan.time_atoms)
-            per_block = block
+            fits = autotune.CANDIDATES
+            points_per_block = lambda b: b                               # noqa: E731
             table_kernel = codegen.kernel_name(
                 "gq_rk4_time_table" if spec.template == "fixed_rk4.hip" else "gq_fixed_time_table",
                 r.fp64)
@@ -143,10 +147,54 @@ class SweepServer:
                             torch.cuda.current_stream(self.dev).cuda_stream)
             torch.cuda.synchronize(self.dev)
         m = {"module": mod, "spec": spec, "num_steps": num_steps, "dt": dt, "kernel": kname,
-             "block": block, "per_block": per_block, "consts": consts, "table": table,
-             "hilbert": hm is not None}
+             "block": block, "per_block": points_per_block(block), "consts": consts,
+             "table": table, "hilbert": hm is not None, "autotune": {}}
+        if self.block == "auto":
+            block, m["autotune"] = autotune.choose(
+                self.ctx, src, codegen.HIPRTC_BASE_OPTIONS,
+                lambda module, b: self._trial(m, r, key, module, b, points_per_block(b)), fits)
+            m["module"], _, _ = self.ctx.compile(autotune.with_block(src, block),
+                                                 codegen.HIPRTC_BASE_OPTIONS)
+            m["block"], m["per_block"] = block, points_per_block(block)
         self.modules[key] = m
         return m
+
+    def _trial(self, m, r, key, module, block, per_block, side=256, steps=64):
+        """One timed launch of a candidate work-group size: side x side points spread over the
+        request's parameter ranges, the first ``steps`` steps of its time grid."""
+        torch = self.torch
+        rdt = torch.float64 if r.fp64 else torch.float32
+        x = np.linspace(float(np.min(r.axes[0])), float(np.max(r.axes[0])), side)
+        y = (np.linspace(float(np.min(r.axes[1])), float(np.max(r.axes[1])), side)
+             if len(r.axes) > 1 else np.zeros(1))
+        npts = x.size * y.size
+        pxd = torch.as_tensor(np.repeat(x, y.size), dtype=rdt, device=self.dev)
+        pyd = torch.as_tensor(np.tile(y, x.size), dtype=rdt, device=self.dev)
+        size = r.compiled.state_size
+        if r.output_mode in ("final_rho", "mean_state"):
+            result = torch.empty((size, npts), dtype=rdt, device=self.dev)
+        else:
+            result = torch.empty(npts, dtype=torch.complex128 if r.fp64 else torch.complex64,
+                                 device=self.dev)
+        flags = torch.zeros(2, dtype=torch.int32, device=self.dev)
+        spec = m["spec"]
+        a = runtime.SweepArgs()
+        a.dt, a.t_in = float(m["dt"]), 0.0
+        a.num_x, a.num_y, a.point_begin, a.point_count = npts, 1, 0, npts
+        a.num_steps, a.warmup_steps = min(steps, m["num_steps"]), 0
+        a.trace_stride, a.trace_point_stride, a.param_mode = 1, npts, 1
+        a.par_x, a.par_y, a.consts = pxd.data_ptr(), pyd.data_ptr(), m["consts"].data_ptr()
+        a.time_table = m["table"].data_ptr() if m["table"] is not None else None
+        for i, v in enumerate(spec.parameters[:runtime.MAX_SOLVER_PARAMS]):
+            a.solver_param[i] = float(v)
+        a.result, a.status, a.work_cursor = result.data_ptr(), flags.data_ptr(), flags.data_ptr() + 4
+        grid = (npts + per_block - 1) // per_block
+        if spec.adaptive and spec.options.get("persistent", True):
+            grid = min(grid, self.ctx.resident_groups(module, m["kernel"], block))
+        from .device import _timed_launches
+        stream = torch.cuda.current_stream(self.dev).cuda_stream
+        return _timed_launches(self.ctx, module, m["kernel"], max(1, grid), block, a, stream,
+                               flags[1:])
 
     def _hilbert_model(self, r):
         """Matrix-core model of a Lindblad request with N >= 16 whose operators fit the

@@ -237,7 +237,8 @@ def mesolve_2D(H, Drive=None, Col_Ops=None, mean_operator=None, tlist=None,
     (``gqis/solver.py:1712-1736``, ``GQIS_API.md:505-528``), plus three code-generation options:
     ``hiprtc_options`` (extra compiler flags), ``time_table`` (FP64 table of the time-only drive
     terms for fixed-step solvers; False evaluates them per point as the reference does) and
-    ``block_size`` (threads per work-group) and ``liouville`` ("auto", "lanes" or "mfma": the
+    ``block_size`` (threads per work-group, or ``"auto"``: measured once per kernel and device)
+    and ``liouville`` ("auto", "lanes" or "mfma": the
     Hilbert-space path for N = 8, 16, 32 -- matrix cores in FP32, VALU dot products in FP64 --
     used automatically from N = 16 in RK4 sweeps whose operators fit it).  ``return_device=True`` returns torch tensors on
     the GPU.  ``H`` may be a ``CompiledRHS`` from ``compile_lindblad_rhs``.
@@ -319,9 +320,10 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
     t_in = float(t_in)
     if not np.isfinite(t_in):
         raise ValueError("t_in must be finite.")
-    block_size = int(block_size)
-    if block_size not in (64, 128, 256, 512, 1024):
-        raise ValueError("block_size must be 64, 128, 256, 512 or 1024.")
+    if block_size != "auto":
+        block_size = int(block_size)
+        if block_size not in (64, 128, 256, 512, 1024):
+            raise ValueError("block_size must be 64, 128, 256, 512, 1024 or 'auto'.")
 
     var_arrays = {} if var_arrays is None else dict(var_arrays)
     for sym, arr in ({} if rho0_var_arrays is None else dict(rho0_var_arrays)).items():
@@ -500,7 +502,10 @@ def _finish(outputs, stage_s, rhs_status, spec, solver_frequency, unroll, collec
                 "ode_optimize_s": stage_s["ode_optimize_s"], "code_emit_s": stage_s["emit_s"],
                 "kernel_build_s": outputs["build_s"], "cached_rhs": rhs_status,
                 "cached_kernel": {0: "miss", 1: "hit", 2: "disk"}[int(outputs["cached_kernel"])],
-                "solver": spec.name, "solver_frequency": solver_frequency, "unroll": unroll}
+                "solver": spec.name, "solver_frequency": solver_frequency, "unroll": unroll,
+                "block_size": outputs.get("block_size"), "autotune_s": outputs.get("autotune_s", 0.0)}
+        if outputs.get("autotune"):
+            info["autotune"] = outputs["autotune"]
         if outputs.get("adaptive_steps") is not None:
             counts = outputs["adaptive_steps"]
             info["adaptive_steps"] = {"accepted": counts[..., 0], "rejected": counts[..., 1]}
@@ -575,7 +580,8 @@ def _kernel_variant(compiled, spec, mode, uses_rho0_values, trace, state_trace, 
     adaptive_key = tuple(sorted((k, v) for k, v in spec.options.items()
                                 if k in ("count_steps", "state_fp64", "persistent")))
     return (compiled.cache_key, spec.name, adaptive_key, mode, bool(uses_rho0_values),
-            bool(trace), bool(state_trace), bool(fp64), bool(time_table), int(block_size),
+            bool(trace), bool(state_trace), bool(fp64), bool(time_table),
+            block_size if block_size == "auto" else int(block_size),
             bool(unroll), options)
 
 
@@ -604,7 +610,8 @@ def _emit_source(compiled, spec, mode, uses_rho0_values, trace, state_trace, fp6
     elif spec.template == "fixed_tableau.hip":
         extra = tableaux.emit_fixed(spec.name)
     return codegen.assemble(compiled.plan, templates, fp64=fp64, acc64=True,
-                            block_size=block_size, defines=defines,
+                            block_size=256 if block_size == "auto" else block_size,
+                            defines=defines,
                             rho0_exprs=compiled.rho0_exprs, extra=extra)
 
 

@@ -1,0 +1,38 @@
+"""block_size="auto" on the GPU: the measured choice gives the same results as a fixed size,
+and the choice is made once per kernel."""
+import numpy as np
+import pytest
+
+from gqis import autotune, mesolve_2D, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("solver", ["rk4", "tsit5a"])
+def test_auto_block_matches_fixed_block(gpu, solver):
+    autotune.clear_memory()
+    w = workloads.chain(8, n_eps=256, n_amp=64, n_t=401, t_end=20.0)
+    args = (w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"])
+    fixed = mesolve_2D(*args, var_arrays=w["var_arrays"], fp64=True, solver=solver,
+                       liouville="lanes", block_size=256)
+    got, info = mesolve_2D(*args, var_arrays=w["var_arrays"], fp64=True, solver=solver,
+                           liouville="lanes", block_size="auto", return_timing_info=True)
+    print(solver, "chose", info["block_size"], info.get("autotune"), flush=True)
+    assert info["block_size"] in autotune.CANDIDATES
+    assert set(info["autotune"]) <= set(autotune.CANDIDATES) and len(info["autotune"]) >= 2
+    np.testing.assert_array_equal(got, fixed)
+    _, again = mesolve_2D(*args, var_arrays=w["var_arrays"], fp64=True, solver=solver,
+                          liouville="lanes", block_size="auto", return_timing_info=True)
+    assert again["block_size"] == info["block_size"] and "autotune" not in again
+
+
+def test_auto_block_on_the_matrix_core_path(gpu):
+    autotune.clear_memory()
+    w = workloads.random_n_level(16, n_eps=128, n_amp=128, n_t=201, t_end=2.0)
+    args = (w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"])
+    fixed = mesolve_2D(*args, var_arrays=w["var_arrays"], liouville="mfma", block_size=256)
+    got, info = mesolve_2D(*args, var_arrays=w["var_arrays"], liouville="mfma",
+                           block_size="auto", return_timing_info=True)
+    print("mfma chose", info["block_size"], info.get("autotune"), flush=True)
+    assert info["block_size"] in autotune.CANDIDATES
+    np.testing.assert_array_equal(got, fixed)
