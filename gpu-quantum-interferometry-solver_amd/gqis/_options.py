@@ -17,7 +17,7 @@ OPTION_GROUPS = {
                 "factor_terms", "cse_batch_size", "cse_simplify", "hoist_rho_independent",
                 "symbolic_workers", "max_symbolic_workers", "nvrtc_options",
                 # new in this implementation:
-                "hiprtc_options", "time_table", "block_size", "liouville"),
+                "hiprtc_options", "time_table", "block_size", "liouville", "lanes_per_point"),
     "diagnostics": ("timings", "return_timing_info", "Actual_Kernel_Save", "beep_on_error",
                     "ignore_non_finite_output"),
 }

@@ -74,39 +74,50 @@ def _disk_store(ctx, key, record):
             os.remove(tmp)
 
 
-def lookup(ctx, key):
+def _as_choice(value):
+    return tuple(value) if isinstance(value, list) else value
+
+
+def lookup(ctx, key, candidates=CANDIDATES):
+    """The recorded choice for ``key`` if it is one of ``candidates``, else None."""
     with _lock:
-        if key in _memory:
+        if key in _memory and _memory[key] in candidates:
             return _memory[key]
     rec = _disk_load(ctx).get(key)
-    if isinstance(rec, dict) and rec.get("block") in CANDIDATES:
+    if isinstance(rec, dict) and _as_choice(rec.get("choice")) in candidates:
+        choice = _as_choice(rec["choice"])
         with _lock:
-            _memory[key] = int(rec["block"])
-        return int(rec["block"])
+            _memory[key] = choice
+        return choice
     return None
 
 
-def choose(ctx, source, options, trial, candidates=CANDIDATES):
-    """Fastest work-group size for ``source``: ``trial(module, block)`` runs one measured
-    launch and returns its duration in seconds.  Returns ``(block, {block: seconds})``; the
-    timings are empty when the choice came from memory or disk."""
+def choose(ctx, source, options, trial, candidates=CANDIDATES, source_of=None):
+    """Fastest candidate for the kernel of ``source``.  A candidate is a work-group size (the
+    source's GQ_BLOCK is replaced) or, with ``source_of(candidate)`` giving its source, any
+    variant such as (lanes per point, work-group size).  ``trial(module, candidate)`` runs one
+    measured launch and returns its duration in seconds.  Returns ``(choice, {candidate:
+    seconds})``; the timings are empty when the choice came from memory or disk."""
+    candidates = tuple(candidates)
     key = source_key(source, options, ctx.arch)
-    hit = lookup(ctx, key)
+    hit = lookup(ctx, key, candidates)
     if hit is not None:
         return hit, {}
     times = {}
-    for block in candidates:
+    for cand in candidates:
+        src = source_of(cand) if source_of is not None else with_block(source, cand)
         try:
-            module, _, _ = ctx.compile(with_block(source, block), options)
+            module, _, _ = ctx.compile(src, options)
         except Exception:       # e.g. the LDS of a larger work-group does not fit: skip it
-            if block == candidates[-1] and not times:
+            if cand == candidates[-1] and not times:
                 raise
             continue
-        times[block] = float(trial(module, block))
+        times[cand] = float(trial(module, cand))
     best = min(times, key=times.get)
     with _lock:
         _memory[key] = best
-    _disk_store(ctx, key, {"block": best, "seconds": {str(b): t for b, t in times.items()}})
+    _disk_store(ctx, key, {"choice": list(best) if isinstance(best, tuple) else best,
+                           "seconds": {str(c): t for c, t in times.items()}})
     return best, times
 
 

@@ -230,6 +230,141 @@ def emit_model(plan, rho0_exprs=None):
     return "\n".join(lines)
 
 
+def _closure(ordered, seeds):
+    """The entries of an ordered [(symbol, expr)] list that ``seeds`` reach (in list order);
+    an entry's expression may use earlier entries."""
+    need = set(seeds)
+    keep = []
+    for sym, expr in reversed(ordered):
+        if sym in need:
+            keep.append((sym, expr))
+            need |= expr.free_symbols
+    keep.reverse()
+    return keep, need
+
+
+def group_rows(d, groups):
+    """Rows of each wave of a point group: contiguous blocks of ceil(d / groups)."""
+    rw = -(-d // groups)
+    return rw, [(g * rw, min(d, (g + 1) * rw)) for g in range(groups)]
+
+
+def emit_grouped(plan, groups, rho0_exprs=None):
+    """Device code of the grouped kernels: ``groups`` wavefronts share one set of 64
+    parameter points, wavefront g owning a contiguous block of state rows.  For every g a
+    specialisation ``gq_grp<g>`` holds: its first row and row count, the statics and stage
+    values its rows (and, for g = 0, the observable) use, renumbered into small per-wave arrays;
+    its RHS rows, reading the stage state from LDS (point-minor, ``Y[j * 64 + lane]``); its rows
+    of the initial state.  ``GQ_GROUP_DISPATCH(fn, args)`` calls ``fn<g> args`` for the
+    wave-uniform g, so each wave runs code specialised to its rows."""
+    d = plan.state_size
+    rw, blocks = group_rows(d, groups)
+    if groups < 2 or rw < 1 or blocks[-1][0] >= d:
+        raise ValueError(f"{groups} groups do not split {d} state rows")
+    state_names = {s.name: j for j, s in enumerate(plan.states)}
+    parts = ["template <int GI> struct gq_grp;"]
+    for g, (lo, hi) in enumerate(blocks):
+        exprs = [sp.sympify(e) for e in plan.rhs_out[lo:hi]]
+        seeds = set().union(*[e.free_symbols for e in exprs]) if exprs else set()
+        obs, obs_temps = [], []
+        if g == 0:
+            obs = [sp.sympify(e) for e in plan.obs_out]
+            obs_temps, reach = _closure(plan.obs_temps, set().union(*[e.free_symbols for e in obs]))
+            seeds |= reach
+        temps, seeds = _closure(plan.rhs_temps, seeds)
+        stage, seeds = _closure(plan.stage, seeds)
+        statics, seeds = _closure(plan.statics, seeds)
+        names = {"ParX": "px", "ParY": "py", "t": "t_now"}
+        for i, c in enumerate(plan.const_syms):
+            names[c.name] = f"C[{i}]"
+        for i, (sym, _) in enumerate(plan.time_atoms):
+            names[sym.name] = f"TT[{i}]"
+        for k, (sym, _) in enumerate(statics):
+            names[sym.name] = f"Sg[{k}]"
+        for k, (sym, _) in enumerate(stage):
+            names[sym.name] = f"Gg[{k}]"
+        for sym, _ in list(temps) + list(obs_temps):
+            names[sym.name] = sym.name
+        for s_name, j in state_names.items():
+            names[s_name] = f"y{j}"
+        pr = HipPrinter(names)
+        dev = "__device__ static __forceinline__ void"
+        parts.append(f"template <> struct gq_grp<{g}> {{")
+        parts.append(f"    static constexpr int R0 = {lo}, NR = {hi - lo}, "
+                     f"NS = {max(1, len(statics))}, NG = {max(1, len(stage))};")
+        parts.append(f"    {dev} statics(const real px, const real py, "
+                     "const real* __restrict__ C, real* __restrict__ Sg) {")
+        parts.append("        (void)px; (void)py; (void)C; (void)Sg;")
+        parts += [f"        Sg[{k}] = {pr.doprint(e)};" for k, (_, e) in enumerate(statics)]
+        parts.append("    }")
+        parts.append(f"    {dev} stage(const real px, const real py, const real* __restrict__ C, "
+                     "const real* __restrict__ Sg, const real* __restrict__ TT, "
+                     "real* __restrict__ Gg) {")
+        parts.append("        (void)px; (void)py; (void)C; (void)Sg; (void)TT; (void)Gg;")
+        parts += [f"        Gg[{k}] = {pr.doprint(e)};" for k, (_, e) in enumerate(stage)]
+        parts.append("    }")
+        cols = sorted({state_names[s.name] for e in exprs + [t for _, t in temps]
+                       for s in e.free_symbols if s.name in state_names})
+        parts.append(f"    {dev} rhs(const real* __restrict__ Y, const int lane, const real px, "
+                     "const real py, const real* __restrict__ C, const real* __restrict__ Sg, "
+                     "const real* __restrict__ Gg, real* __restrict__ dy) {")
+        parts.append("        (void)Y; (void)lane; (void)px; (void)py; (void)C; (void)Sg; (void)Gg;")
+        parts += [f"        const real y{j} = Y[{j} * 64 + lane];" for j in cols]
+        parts += [f"        const real {sym.name} = {pr.doprint(e)};" for sym, e in temps]
+        parts += [f"        dy[{i - lo}] = {pr.doprint(e)};" for i, e in zip(range(lo, hi), exprs)]
+        parts.append("    }")
+        if g == 0:
+            obs_cols = sorted({state_names[s.name] for e in obs + [t for _, t in obs_temps]
+                               for s in e.free_symbols if s.name in state_names})
+            parts.append(f"    {dev} obs(const real* __restrict__ Y, const int lane, "
+                         "const real px, const real py, const real* __restrict__ C, "
+                         "const real* __restrict__ Sg, const real* __restrict__ Gg, "
+                         "real& o_re, real& o_im) {")
+            parts.append("        (void)Y; (void)lane; (void)px; (void)py; (void)C; (void)Sg; "
+                         "(void)Gg;")
+            parts += [f"        const real y{j} = Y[{j} * 64 + lane];" for j in obs_cols]
+            parts += [f"        const real {sym.name} = {pr.doprint(e)};" for sym, e in obs_temps]
+            parts.append(f"        o_re = {pr.doprint(obs[0])};")
+            parts.append(f"        o_im = {pr.doprint(obs[1])};")
+            parts.append("    }")
+        parts.append(f"    {dev} init(real* __restrict__ y, const real px, const real py, "
+                     "const real* __restrict__ C) {")
+        parts.append("        (void)px; (void)py; (void)C;")
+        if rho0_exprs is None:
+            parts += [f"        y[{i - lo}] = R({1.0 if i == 0 else 0.0});" for i in range(lo, hi)]
+        else:
+            ipr = HipPrinter(names)
+            parts += [f"        y[{i - lo}] = {ipr.doprint(sp.sympify(rho0_exprs[i]))};"
+                      for i in range(lo, hi)]
+        parts.append("    }")
+        parts.append("};")
+    head = [f"#define GQ_D {d}", f"#define GQ_G {groups}", f"#define GQ_RW {rw}",
+            f"#define GQ_NATOMS {len(plan.time_atoms)}",
+            f"#define GQ_NATOMS_ALLOC {max(1, len(plan.time_atoms))}"]
+    cases = " ".join(f"case {g}: fn<{g}> args; break;" for g in range(len(blocks)))
+    head.append(f"#define GQ_GROUP_DISPATCH(g, fn, args) switch (g) {{ {cases} default: break; }}")
+    # time atoms, shared by every wave
+    anames = {"t": "t_now"}
+    for i, c in enumerate(plan.const_syms):
+        anames[c.name] = f"(double)C[{i}]"
+    apr = HipPrinter(anames, literal="DBL")
+    head.append("__device__ __forceinline__ void gq_atoms_f64(const double t_now, "
+                "const real* __restrict__ C, double* __restrict__ TT) {")
+    head.append("    (void)t_now; (void)C; (void)TT;")
+    head += [f"    TT[{i}] = {apr.doprint(e)};" for i, (_, e) in enumerate(plan.time_atoms)]
+    head.append("}")
+    rnames = {"t": "t_now"}
+    for i, c in enumerate(plan.const_syms):
+        rnames[c.name] = f"C[{i}]"
+    rpr = HipPrinter(rnames)
+    head.append("__device__ __forceinline__ void gq_atoms_real(const real t_now, "
+                "const real* __restrict__ C, real* __restrict__ TT) {")
+    head.append("    (void)t_now; (void)C; (void)TT;")
+    head += [f"    TT[{i}] = {rpr.doprint(e)};" for i, (_, e) in enumerate(plan.time_atoms)]
+    head.append("}")
+    return "\n".join(head + parts)
+
+
 def kernel_suffix(fp64):
     """Kernel names carry the precision (gq_rk4_f64, gq_adaptive_f32, ...)."""
     return "f64" if fp64 else "f32"
@@ -261,12 +396,16 @@ def prelude(fp64, acc64, block_size, defines):
 
 
 def assemble(plan, templates, *, fp64, acc64=True, block_size=256, defines=None,
-             rho0_exprs=None, extra=""):
-    """Full hiprtc source: ABI header + prelude + model + solver constants + templates."""
+             rho0_exprs=None, extra="", groups=1):
+    """Full hiprtc source: ABI header + prelude + model + solver constants + templates
+    (``groups`` > 1: the grouped model functions, ``emit_grouped``)."""
     with open(_header_path()) as fh:
         header = fh.read()
-    parts = [header, prelude(fp64, acc64, block_size, defines or {}), emit_model(plan, rho0_exprs),
-             extra]
+    model = (emit_model(plan, rho0_exprs) if groups == 1
+             else emit_grouped(plan, groups, rho0_exprs))
+    defines = dict(defines or {})
+    defines.setdefault("GQ_GROUPED", int(groups > 1))
+    parts = [header, prelude(fp64, acc64, block_size, defines), model, extra]
     parts += [read_kernel_template(t) for t in (["common.hip"] + list(templates))]
     return "\n".join(parts)
 

@@ -36,9 +36,27 @@ def _initial_states(rho0_values, nx, ny, size, dtype, device):
     return r.reshape(nx * ny, size).contiguous()
 
 
+GROUP_CHOICES = (2, 4, 8)
+MIN_GROUPED_STATES = 24     # below this one lane holds a point's state without spills
+
+
+def _kernel_for(spec, groups, fp64):
+    return codegen.kernel_name("gq_rk4_grouped" if groups > 1 else spec.kernel, fp64)
+
+
+def _grid(npts, groups, block):
+    """Work-groups of a launch: one point per lane, or 64 points per group of waves."""
+    per = 64 if groups > 1 else block
+    return (npts + per - 1) // per
+
+
 def run_sweep(*, source, variant, spec, compiled, mode, var_arrays, rho0_values, const_vals,
               fp64, dt, t_in, num_steps, warmup_steps, num_trace, trace_stride, trace_offset,
-              state_trace, time_table, block_size, options, reuse, collect):
+              state_trace, time_table, block_size, options, reuse, collect, emit=None,
+              lanes_per_point=1):
+    """Buffers, time table, launch and output layouts of one sweep.  ``source`` is the kernel
+    for ``lanes_per_point`` lanes per point; ``emit(groups)`` returns the source of another
+    grouping (used when ``block_size`` is "auto")."""
     import torch
     setup_start = time.time()
     ctx = runtime.Context.get()
@@ -53,9 +71,12 @@ def run_sweep(*, source, variant, spec, compiled, mode, var_arrays, rho0_values,
     size = compiled.state_size
 
     options = codegen.HIPRTC_BASE_OPTIONS + tuple(options)
+    groups = int(lanes_per_point)
     tune = block_size == "auto"
     if tune:
         block_size = 256       # the emitted source's size; measured below
+    if groups > 1:
+        block_size = 64 * groups
     module, cached, build_s = ctx.compile(source, options)
     rho0 = (_initial_states(rho0_values, nx, ny, size, rdt, dev)
             if rho0_values is not None else None)
@@ -103,33 +124,43 @@ def run_sweep(*, source, variant, spec, compiled, mode, var_arrays, rho0_values,
     a.work_cursor = cursor.data_ptr()
 
     stream = torch.cuda.current_stream(dev).cuda_stream
-    kname = codegen.kernel_name(spec.kernel, fp64)
     persistent = spec.adaptive and spec.options.get("persistent", True)
     tune_s, tune_times = 0.0, {}
-    if tune and npts > 0 and num_steps > 0:
+    if tune and groups == 1 and npts > 0 and num_steps > 0:
         t0 = time.time()
         if use_table and rows > 0:
             ctx.launch(module, codegen.kernel_name(table_kernel(spec), fp64), (rows + 255) // 256,
                        256, a, stream)
         trial_steps = min(int(num_steps), 64)
+        from .solver import grouped_applies
+        cands = [(1, b) for b in autotune.CANDIDATES]
+        if emit is not None and size >= MIN_GROUPED_STATES:
+            cands += [(g, 64 * g) for g in GROUP_CHOICES if grouped_applies(spec, size, fp64, g)]
 
-        def trial(mod, block):
+        def source_of(cand):
+            return autotune.with_block(source if cand[0] == 1 else emit(cand[0]), cand[1])
+
+        def trial(mod, cand):
+            g, block = cand
             t = runtime.SweepArgs.from_buffer_copy(a)
             t.num_steps, t.warmup_steps, t.num_trace, t.trace = trial_steps, 0, 0, None
-            grid = (npts + block - 1) // block
-            if persistent:
-                grid = min(ctx.resident_groups(mod, kname, block), grid)
-            return _timed_launches(ctx, mod, kname, max(grid, 1), block, t, stream, cursor)
+            kn = _kernel_for(spec, g, fp64)
+            grid = _grid(npts, g, block)
+            if persistent and g == 1:
+                grid = min(ctx.resident_groups(mod, kn, block), grid)
+            return _timed_launches(ctx, mod, kn, max(grid, 1), block, t, stream, cursor)
 
         if npts * trial_steps >= autotune.MIN_TRIAL_POINT_STEPS:
-            block_size, tune_times = autotune.choose(ctx, source, options, trial)
+            choice, tune_times = autotune.choose(ctx, source, options, trial, cands, source_of)
         else:
             key = autotune.source_key(source, options, ctx.arch)
-            block_size = autotune.lookup(ctx, key) or 256
-        module, _, _ = ctx.compile(autotune.with_block(source, block_size), options)
+            choice = autotune.lookup(ctx, key, cands) or (1, 256)
+        groups, block_size = choice
+        module, _, _ = ctx.compile(source_of(choice), options)
         ctx.read_status(status, stream, clear=True)
         cursor.zero_()
         tune_s = time.time() - t0
+    kname = _kernel_for(spec, groups, fp64)
     timer = runtime.EventTimer(ctx, stream) if collect else None
     setup_s = time.time() - setup_start
     if npts > 0 and num_steps >= 0:
@@ -138,11 +169,11 @@ def run_sweep(*, source, variant, spec, compiled, mode, var_arrays, rho0_values,
         if use_table and rows > 0:
             ctx.launch(module, codegen.kernel_name(table_kernel(spec), fp64), (rows + 255) // 256,
                        256, a, stream)
-        if persistent:
+        if persistent and groups == 1:
             grid = min(ctx.resident_groups(module, kname, block_size),
                        (npts + block_size - 1) // block_size)
         else:
-            grid = (npts + block_size - 1) // block_size
+            grid = _grid(npts, groups, block_size)
         ctx.launch(module, kname, max(grid, 1), block_size, a, stream)
         if timer:
             timer.record_stop()
@@ -164,7 +195,8 @@ def run_sweep(*, source, variant, spec, compiled, mode, var_arrays, rho0_values,
     steps = steps_out.reshape(nx, ny, 2).cpu().numpy() if steps_out is not None else None
     return {"result": out, "trace": tr, "status": flags, "gpu_s": gpu_s, "build_s": build_s,
             "cached_kernel": cached, "setup_s": setup_s, "adaptive_steps": steps,
-            "block_size": int(block_size), "autotune_s": tune_s, "autotune": tune_times}
+            "block_size": int(block_size), "lanes_per_point": int(groups),
+            "autotune_s": tune_s, "autotune": tune_times}
 
 
 def _timed_launches(ctx, module, kname, grid, block, args, stream, cursor, repeats=2):

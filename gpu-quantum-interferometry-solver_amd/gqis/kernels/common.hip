@@ -19,7 +19,7 @@ __device__ __forceinline__ gq_point gq_locate(const gq_sweep_args& a, long long 
     return pt;
 }
 
-#if !GQ_HILBERT
+#if !GQ_HILBERT && !GQ_GROUPED
 // Initial state: explicit per-point values, else the model's expressions (|0><0| by default).
 __device__ __forceinline__ void gq_init_state(real* __restrict__ y, const gq_sweep_args& a,
                                               const gq_point& pt, real px, real py,
@@ -54,7 +54,7 @@ __device__ __forceinline__ void gq_store_complex(void* base, long long idx, real
     ((real2*)base)[idx] = v;
 }
 
-#if !GQ_HILBERT
+#if !GQ_HILBERT && !GQ_GROUPED
 // Per-point output state of the fixed-step kernels (mean, state mean, trace) and its epilogue
 // (gqis/solver.py:2126-2193 output modes).
 struct gq_outputs {
@@ -155,4 +155,4 @@ __device__ __forceinline__ void gq_out_final(gq_outputs& o, const gq_sweep_args&
     gq_flag(a.status, o.trace_bad, GQ_STATUS_NONFINITE_TRACE);
 #endif
 }
-#endif  // !GQ_HILBERT
+#endif  // !GQ_HILBERT && !GQ_GROUPED

@@ -230,7 +230,8 @@ def mesolve_2D(H, Drive=None, Col_Ops=None, mean_operator=None, tlist=None,
                Actual_Kernel_Save=False, beep_on_error=False, ignore_non_finite_output=False,
                t_in=0.0, return_device=False, calibrate_timesteps=None, consts=None, initial=None,
                output=None, solver_options=None, codegen=None, diagnostics=None,
-               hiprtc_options=(), time_table=True, block_size=256, liouville="auto"):
+               hiprtc_options=(), time_table=True, block_size=256, liouville="auto",
+               lanes_per_point=1):
     """Lindblad master-equation sweep over one or two parameter axes on the GPU.
 
     Same signature, defaults, return shapes and errors as the reference ``mesolve_2D``
@@ -238,7 +239,9 @@ def mesolve_2D(H, Drive=None, Col_Ops=None, mean_operator=None, tlist=None,
     ``hiprtc_options`` (extra compiler flags), ``time_table`` (FP64 table of the time-only drive
     terms for fixed-step solvers; False evaluates them per point as the reference does) and
     ``block_size`` (threads per work-group, or ``"auto"``: measured once per kernel and device)
-    and ``liouville`` ("auto", "lanes" or "mfma": the
+    ``lanes_per_point`` (1, 2, 4 or 8 lanes, one per wavefront, share each point's state rows;
+    ``block_size="auto"`` also measures 2, 4, 8 for RK4 models of 24 or more states) and
+    ``liouville`` ("auto", "lanes" or "mfma": the
     Hilbert-space path for N = 8, 16, 32 -- matrix cores in FP32, VALU dot products in FP64 --
     used automatically from N = 16 in RK4 sweeps whose operators fit it).  ``return_device=True`` returns torch tensors on
     the GPU.  ``H`` may be a ``CompiledRHS`` from ``compile_lindblad_rhs``.
@@ -302,7 +305,8 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
                            beep_on_error=False, ignore_non_finite_output=False, t_in=0.0,
                            return_device=False, calibrate_timesteps=None, compiled_rhs=None,
                            hiprtc_options=(), time_table=True, block_size=256,
-                           liouville="auto", _ode_size=None, _state_trace=False,
+                           liouville="auto", lanes_per_point=1, _ode_size=None,
+                           _state_trace=False,
                            _time_trace_offset=0):
     if calibrate_timesteps is not None and calibrate_timesteps is not False:
         call = dict(locals())
@@ -324,6 +328,9 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
         block_size = int(block_size)
         if block_size not in (64, 128, 256, 512, 1024):
             raise ValueError("block_size must be 64, 128, 256, 512, 1024 or 'auto'.")
+    lanes_per_point = int(lanes_per_point)
+    if lanes_per_point not in (1, 2, 4, 8):
+        raise ValueError("lanes_per_point must be 1, 2, 4 or 8.")
 
     var_arrays = {} if var_arrays is None else dict(var_arrays)
     for sym, arr in ({} if rho0_var_arrays is None else dict(rho0_var_arrays)).items():
@@ -402,7 +409,8 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
     if liouville not in ("auto", "lanes", "mfma"):
         raise ValueError("liouville must be 'auto', 'lanes' or 'mfma'.")
     hilbert_out = None
-    if compiled_rhs is None and _ode_size is None and liouville != "lanes":
+    if (compiled_rhs is None and _ode_size is None and liouville != "lanes"
+            and not (lanes_per_point > 1 and liouville == "auto")):
         hilbert_out = _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values,
                                    runtime_consts, keep_symbolic_consts, auto_runtime_consts,
                                    rho0, rho0_values, spec, mode, fp64, return_time_trace,
@@ -437,14 +445,24 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
                               _state_trace, fp64, time_table, block_size, unroll,
                               tuple(hiprtc_options) + _as_tuple(nvrtc_options))
     emit_start = time.time()
-    source = _KERNEL_CACHE.get(("source",) + variant) if RHSreuse else None
-    emit_s = 0.0
-    if source is None:
-        source = _emit_source(compiled, spec, mode, uses_rho0_values, return_time_trace,
-                              _state_trace, fp64, time_table, block_size, unroll)
-        emit_s = time.time() - emit_start
-        if RHSreuse:
-            _KERNEL_CACHE[("source",) + variant] = source
+    def emit_for(groups):
+        key = ("source", groups) + variant
+        src = _KERNEL_CACHE.get(key) if RHSreuse else None
+        if src is None:
+            src = _emit_source(compiled, spec, mode, uses_rho0_values, return_time_trace,
+                               _state_trace, fp64, time_table, block_size, unroll, groups)
+            if RHSreuse:
+                _KERNEL_CACHE[key] = src
+        return src
+
+    if lanes_per_point > 1 and not grouped_applies(spec, compiled.state_size, fp64,
+                                                   lanes_per_point):
+        raise ValueError(f"lanes_per_point={lanes_per_point} needs the rk4 solver and a state "
+                         f"that splits into {lanes_per_point} row blocks whose LDS copy fits "
+                         f"(state size {compiled.state_size}).")
+    cached_source = RHSreuse and ("source", lanes_per_point) + variant in _KERNEL_CACHE
+    source = emit_for(lanes_per_point)
+    emit_s = 0.0 if cached_source else time.time() - emit_start
     if rhs_status != "miss":
         emit_s = 0.0
     stage_s["emit_s"] = emit_s
@@ -459,7 +477,8 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
     # 3) device work
     from . import device
     outputs = device.run_sweep(
-        source=source, variant=variant, spec=spec, compiled=compiled, mode=mode,
+        source=source, emit=emit_for, lanes_per_point=lanes_per_point,
+        variant=variant, spec=spec, compiled=compiled, mode=mode,
         var_arrays=var_arrays, rho0_values=rho0_values, const_vals=const_vals, fp64=fp64,
         dt=dt, t_in=t_in, num_steps=num_steps, warmup_steps=warmup_steps,
         num_trace=num_trace, trace_stride=trace_stride, trace_offset=trace_offset,
@@ -503,7 +522,9 @@ def _finish(outputs, stage_s, rhs_status, spec, solver_frequency, unroll, collec
                 "kernel_build_s": outputs["build_s"], "cached_rhs": rhs_status,
                 "cached_kernel": {0: "miss", 1: "hit", 2: "disk"}[int(outputs["cached_kernel"])],
                 "solver": spec.name, "solver_frequency": solver_frequency, "unroll": unroll,
-                "block_size": outputs.get("block_size"), "autotune_s": outputs.get("autotune_s", 0.0)}
+                "block_size": outputs.get("block_size"),
+                "lanes_per_point": outputs.get("lanes_per_point", 1),
+                "autotune_s": outputs.get("autotune_s", 0.0)}
         if outputs.get("autotune"):
             info["autotune"] = outputs["autotune"]
         if outputs.get("adaptive_steps") is not None:
@@ -585,8 +606,20 @@ def _kernel_variant(compiled, spec, mode, uses_rho0_values, trace, state_trace, 
             bool(unroll), options)
 
 
+LDS_BYTES_PER_CU = 160 * 1024      # gfx950
+
+
+def grouped_applies(spec, d, fp64, groups):
+    """The grouped RK4 kernel exists for this solver and its double-buffered LDS copy of the
+    state (2 x d x 64 reals) fits one compute unit."""
+    if spec.name != "rk4" or groups < 2:
+        return False
+    rw = -(-d // groups)
+    return (groups - 1) * rw < d and 2 * d * 64 * (8 if fp64 else 4) <= LDS_BYTES_PER_CU
+
+
 def _emit_source(compiled, spec, mode, uses_rho0_values, trace, state_trace, fp64, time_table,
-                 block_size, unroll):
+                 block_size, unroll, groups=1):
     defines = {
         "GQ_OUT_MEAN": int(mode == "mean"),
         "GQ_OUT_FINAL": int(mode == "final"),
@@ -597,7 +630,7 @@ def _emit_source(compiled, spec, mode, uses_rho0_values, trace, state_trace, fp6
         "GQ_TIME_TABLE": int(bool(time_table) and not spec.adaptive),
     }
     templates = ([spec.template] if spec.template != "fixed_rk4.hip"
-                 else ["rk4_table.hip", "fixed_rk4.hip"])
+                 else ["rk4_table.hip", "fixed_rk4.hip" if groups == 1 else "grouped_rk4.hip"])
     from . import tableaux
     extra = ""
     if spec.adaptive:
@@ -609,10 +642,12 @@ def _emit_source(compiled, spec, mode, uses_rho0_values, trace, state_trace, fp6
         defines["GQ_PERSISTENT"] = int(bool(spec.options.get("persistent", True)))
     elif spec.template == "fixed_tableau.hip":
         extra = tableaux.emit_fixed(spec.name)
+    if groups > 1:
+        block_size = 64 * groups
     return codegen.assemble(compiled.plan, templates, fp64=fp64, acc64=True,
                             block_size=256 if block_size == "auto" else block_size,
                             defines=defines,
-                            rho0_exprs=compiled.rho0_exprs, extra=extra)
+                            rho0_exprs=compiled.rho0_exprs, extra=extra, groups=groups)
 
 
 def _print_timings(api_name, info, mode):

@@ -53,11 +53,24 @@ def test_choose_measures_once_and_remembers(tmp_path):
     best2, measured2 = autotune.choose(ctx, SRC, ("-O3",), trial)
     assert best2 == 128 and measured2 == {} and len(calls) == 4
     rec = json.load(open(tmp_path / "autotune.json"))
-    assert list(rec.values())[0]["block"] == 128
+    assert list(rec.values())[0]["choice"] == 128
     autotune.clear_memory()                       # a new process reads the disk record
     assert autotune.lookup(FakeCtx(cache_dir=str(tmp_path)),
                            autotune.source_key(SRC, ("-O3",), "gfx950")) == 128
     assert autotune.lookup(FakeCtx(), autotune.source_key(SRC, ("-O3",), "gfx950")) == 128
+
+
+def test_variant_candidates_with_their_own_sources(tmp_path):
+    ctx = FakeCtx(cache_dir=str(tmp_path))
+    cands = [(1, 64), (1, 256), (4, 256)]
+    t = {(1, 64): 2.0, (1, 256): 3.0, (4, 256): 1.0}
+    best, measured = autotune.choose(ctx, SRC, (), lambda m, c: t[c], cands,
+                                     source_of=lambda c: autotune.with_block(SRC, c[1]))
+    assert best == (4, 256) and measured == t
+    autotune.clear_memory()
+    key = autotune.source_key(SRC, (), "gfx950")
+    assert autotune.lookup(ctx, key, cands) == (4, 256)      # tuples survive the JSON record
+    assert autotune.lookup(ctx, key, [(1, 64)]) is None        # not a candidate any more
 
 
 def test_candidates_that_do_not_compile_are_skipped():

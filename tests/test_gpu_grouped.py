@@ -1,0 +1,89 @@
+"""Lanes-per-point (grouped) RK4 kernels on the GPU: the same results as the one-lane-per-point
+kernel for every output mode, FP64 parity with the oracle's golden fixture, continuation from
+a saved state, ragged point counts, and the autotuner's choice among both forms."""
+import os
+
+import numpy as np
+import pytest
+
+from gqis import autotune, mesolve_2D, workloads
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _chain8(n_eps=12, n_amp=10, n_t=401, t_end=20.0):
+    w = workloads.chain(8, n_eps=n_eps, n_amp=n_amp, n_t=n_t, t_end=t_end)
+    return (w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"]), w["var_arrays"]
+
+
+def _rel(got, ref):
+    return float(np.max(np.abs(got - ref)) / max(1e-300, float(np.max(np.abs(ref)))))
+
+
+@pytest.mark.parametrize("mode", ["mean", "final", "final_rho"])
+@pytest.mark.parametrize("groups", [2, 4, 8])
+def test_grouped_matches_per_lane_fp64(gpu, mode, groups):
+    args, var = _chain8()
+    ref = mesolve_2D(*args, var_arrays=var, fp64=True, output_mode=mode, liouville="lanes")
+    got = mesolve_2D(*args, var_arrays=var, fp64=True, output_mode=mode, liouville="lanes",
+                     lanes_per_point=groups)
+    print(mode, groups, "bitwise" if np.array_equal(got, ref) else _rel(got, ref), flush=True)
+    assert got.shape == ref.shape and got.dtype == ref.dtype
+    assert _rel(got, ref) <= 1e-13
+
+
+def test_grouped_traces_and_state_mean(gpu):
+    args, var = _chain8(n_eps=9, n_amp=7)
+    for kw in ({"output_mode": "mean", "return_time_trace": True, "time_trace_every": 7},
+               {"output_mode": "final_rho", "return_time_trace": True, "time_trace_every": 50}):
+        ref = mesolve_2D(*args, var_arrays=var, fp64=True, liouville="lanes", **kw)
+        got = mesolve_2D(*args, var_arrays=var, fp64=True, liouville="lanes",
+                         lanes_per_point=4, **kw)
+        for a, b in zip(got[:2], ref[:2]):
+            assert a.shape == b.shape and _rel(a, b) <= 1e-13
+        np.testing.assert_array_equal(got[2], ref[2])
+
+
+def test_grouped_fp32_matches_per_lane(gpu):
+    args, var = _chain8()
+    ref = mesolve_2D(*args, var_arrays=var, fp64=False, liouville="lanes")
+    got = mesolve_2D(*args, var_arrays=var, fp64=False, liouville="lanes", lanes_per_point=4)
+    assert _rel(got, ref) <= 1e-5
+
+
+@pytest.mark.parametrize("groups", [2, 4, 8])
+def test_grouped_random_8_level_golden_fixture(gpu, groups):
+    g = np.load(os.path.join(GOLDEN, "c3_random_8_level_8x8_subset_rk4_mean_fp64.npz"))
+    w = workloads.random_n_level(8, seed=int(g["seed"]), n_eps=8, n_amp=8)
+    e, a = list(w["var_arrays"])
+    got = mesolve_2D(w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"],
+                     var_arrays={e: g["eps"], a: g["amp"]}, fp64=True, liouville="lanes",
+                     lanes_per_point=groups)
+    assert _rel(got, g["result"]) <= 1e-10
+
+
+def test_grouped_continuation_and_ragged_grid(gpu):
+    args, var = _chain8(n_eps=7, n_amp=9, n_t=201, t_end=10.0)    # 63 points: a ragged group
+    e, a = list(var)
+    whole = mesolve_2D(*args, var_arrays=var, fp64=True, output_mode="final_rho",
+                       liouville="lanes", lanes_per_point=4)
+    half = (args[0], args[1], args[2], args[3], np.linspace(0.0, 5.0, 101))
+    first = mesolve_2D(*half, var_arrays=var, fp64=True, output_mode="final_rho",
+                       liouville="lanes", lanes_per_point=4)
+    second = mesolve_2D(*half, var_arrays=var, fp64=True, output_mode="final_rho",
+                        liouville="lanes", lanes_per_point=4, rho0_values=first, t_in=5.0)
+    assert whole.shape == (7, 9, 63)
+    assert _rel(second, whole) <= 1e-12
+
+
+def test_autotuner_considers_grouped_kernels(gpu):
+    autotune.clear_memory()
+    args, var = _chain8(n_eps=128, n_amp=128, n_t=201, t_end=10.0)
+    ref = mesolve_2D(*args, var_arrays=var, fp64=True, liouville="lanes")
+    got, info = mesolve_2D(*args, var_arrays=var, fp64=True, liouville="lanes",
+                           block_size="auto", return_timing_info=True)
+    print("autotune", info["lanes_per_point"], info["block_size"], info.get("autotune"),
+          flush=True)
+    assert {c[0] for c in info["autotune"]} == {1, 2, 4, 8}
+    assert _rel(got, ref) <= 1e-13
