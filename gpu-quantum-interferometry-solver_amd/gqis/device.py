@@ -41,7 +41,10 @@ MIN_GROUPED_STATES = 24     # below this one lane holds a point's state without 
 
 
 def _kernel_for(spec, groups, fp64):
-    return codegen.kernel_name("gq_rk4_grouped" if groups > 1 else spec.kernel, fp64)
+    if groups > 1:
+        return codegen.kernel_name("gq_adaptive_grouped" if spec.adaptive else "gq_rk4_grouped",
+                                   fp64)
+    return codegen.kernel_name(spec.kernel, fp64)
 
 
 def _grid(npts, groups, block):

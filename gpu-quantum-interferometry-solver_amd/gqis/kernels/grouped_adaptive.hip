@@ -1,0 +1,353 @@
+// Adaptive embedded Runge-Kutta (Tsit5 / DOP853) between the output points, with the state
+// of each parameter point split over GQ_G lanes (one per wavefront of the work-group, rows
+// [R0, R0 + NR) of gq_grp<g>), as in grouped_rk4.hip.  Semantics are those of the per-lane
+// adaptive kernel (adaptive.hip, after gqis/cuda_solvers.py:731-908): interval snapping,
+// FSAL derivative kept on rejection, FP64 time / error norm / controller, NaN on
+// max_substeps, trapezoid mean.
+//
+// Every wave of a group keeps an identical copy of each lane's controller state (time, step
+// size, counters).  Stage inputs go through LDS as in the grouped RK4 kernel; the error norm
+// is a sum over rows, so each wave writes its partial sum and every wave adds the GQ_G
+// partials in the same order: the norm, and with it every accept / reject / step-size
+// decision, is bitwise the same in all waves, so all waves take the same branches and reach
+// the same barriers.  Points are assigned one per lane for the whole integration (no work
+// stealing across groups).
+
+#if GQ_STATE_FP64
+typedef double st_t;
+#else
+typedef real st_t;
+#endif
+
+#define GQ_NS GQ_NSTAGES
+#define GQ_GW (GQ_G * 64)
+#if GQ_METHOD_DOP853
+#define GQ_FAC_MIN (1.0 / 3.0)
+#define GQ_FAC_MAX 6.0
+#define GQ_ERR_EXPONENT (-1.0 / 8.0)
+#else
+#define GQ_FAC_MIN 0.2
+#define GQ_FAC_MAX 5.0
+#define GQ_ERR_EXPONENT (-0.2)
+#endif
+
+template <int GI>
+__device__ __forceinline__ void gq_stage_at(double t_abs, real px, real py,
+                                            const real* __restrict__ C,
+                                            const real* __restrict__ Sg, real* __restrict__ Gg) {
+    double TTd[GQ_NATOMS_ALLOC];
+    gq_atoms_f64(t_abs, C, TTd);
+    real TT[GQ_NATOMS_ALLOC];
+#pragma unroll
+    for (int i = 0; i < GQ_NATOMS_ALLOC; ++i) TT[i] = (real)TTd[i];
+    gq_grp<GI>::stage(px, py, C, Sg, TT, Gg);
+}
+
+template <int GI>
+__device__ __forceinline__ void gq_share(real* __restrict__ Y, const st_t* __restrict__ v,
+                                         int lane) {
+    using P = gq_grp<GI>;
+#pragma unroll
+    for (int r = 0; r < P::NR; ++r) Y[(P::R0 + r) * 64 + lane] = (real)v[r];
+}
+
+// Sum of the waves' partial sums for this lane, in wave order (identical in every wave).
+__device__ __forceinline__ double gq_group_sum(double* __restrict__ red, double part, int g,
+                                               int lane) {
+    red[g * 64 + lane] = part;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < GQ_G; ++w) s += red[w * 64 + lane];
+    __syncthreads();   // the partials are rewritten by the next reduction
+    return s;
+}
+
+template <int GI>
+__device__ __forceinline__ void gq_adaptive_body(const gq_sweep_args& a,
+                                                 real (*s_y)[GQ_D * 64], double* s_red,
+                                                 const int lane, const long long q,
+                                                 const bool valid, const real px, const real py,
+                                                 const gq_point& pt) {
+    using P = gq_grp<GI>;
+    const double dt_out = a.dt;
+    const double rtol = a.solver_param[0], atol = a.solver_param[1];
+    const int max_sub = (int)a.solver_param[2];
+    const int num_steps = a.num_steps;
+    const real* __restrict__ C = (const real*)a.consts;
+    const double inv_avg_time = 1.0 / ((double)(num_steps - a.warmup_steps) * dt_out);
+
+    real Sg[P::NS];
+    P::statics(px, py, C, Sg);
+    real Gg[P::NG];
+    st_t y[P::NR];
+    {
+        real y0[P::NR];
+#if GQ_RHO0_VALUES
+        const real* __restrict__ r0v = (const real*)a.rho0;
+#pragma unroll
+        for (int r = 0; r < P::NR; ++r) y0[r] = r0v[pt.q * GQ_D + P::R0 + r];
+#else
+        (void)pt;
+        P::init(y0, px, py, C);
+#endif
+#pragma unroll
+        for (int r = 0; r < P::NR; ++r) y[r] = (st_t)y0[r];
+    }
+    double tau = 0.0, h = dt_out;
+    int step = 0, sub = 0, n_acc = 0, n_rej = 0;
+    bool failed = false, bad_trace = false;
+    bool live = valid && num_steps > 0;
+    int buf = 0;
+
+    gq_stage_at<GI>(a.t_in, px, py, C, Sg, Gg);
+    gq_share<GI>(s_y[buf], y, lane);
+    __syncthreads();
+    real k0[P::NR];
+    P::rhs(s_y[buf], lane, px, py, C, Sg, Gg, k0);
+#if GQ_OUT_MEAN
+    acc_t avg_re = 0, avg_im = 0;
+    real op_re = R(0.0), op_im = R(0.0);
+    if constexpr (GI == 0) P::obs(s_y[buf], lane, px, py, C, Sg, Gg, op_re, op_im);
+#endif
+    buf ^= 1;
+#if GQ_OUT_MEAN_STATE
+    acc_t savg[P::NR];
+#pragma unroll
+    for (int r = 0; r < P::NR; ++r) savg[r] = 0;
+#endif
+
+    // lanes with nothing to do (past the end, or zero steps) take part in the barriers only
+    while (__any(live)) {
+        const double remaining = dt_out - tau;
+        const bool last = h * 1.01 >= remaining;
+        const double hs = last ? remaining : h;
+        const double t0 = (double)step * dt_out + tau;
+        real k[GQ_NS][P::NR];
+#pragma unroll
+        for (int r = 0; r < P::NR; ++r) k[0][r] = k0[r];
+#pragma unroll
+        for (int s = 1; s < GQ_NS; ++s) {
+            st_t ys[P::NR];
+#pragma unroll
+            for (int r = 0; r < P::NR; ++r) {
+                st_t acc = 0;
+#pragma unroll
+                for (int j = 0; j < s; ++j)
+                    if (GQ_A[s][j] != 0.0) acc = fma((st_t)GQ_A[s][j], (st_t)k[j][r], acc);
+                ys[r] = fma((st_t)hs, acc, y[r]);
+            }
+            gq_stage_at<GI>(a.t_in + (t0 + GQ_C[s] * hs), px, py, C, Sg, Gg);
+            gq_share<GI>(s_y[buf], ys, lane);
+            __syncthreads();
+            P::rhs(s_y[buf], lane, px, py, C, Sg, Gg, k[s]);
+            buf ^= 1;
+        }
+        st_t yn[P::NR];
+#pragma unroll
+        for (int r = 0; r < P::NR; ++r) {
+            st_t acc = 0;
+#pragma unroll
+            for (int j = 0; j < GQ_NS; ++j)
+                if (GQ_B[j] != 0.0) acc = fma((st_t)GQ_B[j], (st_t)k[j][r], acc);
+            yn[r] = fma((st_t)hs, acc, y[r]);
+        }
+        // Gg now holds the stage values at t0 + hs (the last stage has c = 1)
+        const real* __restrict__ Yn = s_y[buf];
+        gq_share<GI>(s_y[buf], yn, lane);
+        __syncthreads();
+        buf ^= 1;
+        real kf[P::NR];
+        double err_norm;
+#if GQ_METHOD_DOP853
+        {
+            double s5 = 0.0, s3 = 0.0;
+#pragma unroll
+            for (int r = 0; r < P::NR; ++r) {
+                double e5 = 0.0, e3 = 0.0;
+#pragma unroll
+                for (int j = 0; j < GQ_NS; ++j) {
+                    if (GQ_E5[j] != 0.0) e5 = fma(GQ_E5[j], (double)k[j][r], e5);
+                    if (GQ_E3[j] != 0.0) e3 = fma(GQ_E3[j], (double)k[j][r], e3);
+                }
+                const double sc = fma(rtol, fmax(fabs((double)y[r]), fabs((double)yn[r])), atol);
+                s5 = fma(e5 / sc, e5 / sc, s5);
+                s3 = fma(e3 / sc, e3 / sc, s3);
+            }
+            s5 = gq_group_sum(s_red, s5, GI, lane);
+            s3 = gq_group_sum(s_red, s3, GI, lane);
+            double deno = s5 + 0.01 * s3;
+            if (!(deno > 0.0)) deno = 1.0;
+            err_norm = fabs(hs) * s5 / sqrt(deno * (double)GQ_D);
+        }
+#else
+        P::rhs(Yn, lane, px, py, C, Sg, Gg, kf);
+        {
+            double ssum = 0.0;
+#pragma unroll
+            for (int r = 0; r < P::NR; ++r) {
+                double e = GQ_E[GQ_NS] * (double)kf[r];
+#pragma unroll
+                for (int j = 0; j < GQ_NS; ++j) e = fma(GQ_E[j], (double)k[j][r], e);
+                e *= hs;
+                const double sc = fma(rtol, fmax(fabs((double)y[r]), fabs((double)yn[r])), atol);
+                ssum = fma(e / sc, e / sc, ssum);
+            }
+            ssum = gq_group_sum(s_red, ssum, GI, lane);
+            err_norm = sqrt(ssum / (double)GQ_D);
+        }
+#endif
+        double factor = 0.9 * pow(err_norm, GQ_ERR_EXPONENT);
+        factor = (factor != factor) ? GQ_FAC_MIN : fmin(GQ_FAC_MAX, fmax(GQ_FAC_MIN, factor));
+        const bool accept = live && err_norm <= 1.0;
+#if GQ_METHOD_DOP853
+        P::rhs(Yn, lane, px, py, C, Sg, Gg, kf);
+#endif
+#if GQ_OUT_MEAN
+        real o_re = R(0.0), o_im = R(0.0);
+        if constexpr (GI == 0) P::obs(Yn, lane, px, py, C, Sg, Gg, o_re, o_im);
+#endif
+        if (accept) {
+            ++n_acc;
+#if GQ_OUT_MEAN
+            if (step >= a.warmup_steps) {
+                const acc_t wgt = (acc_t)(0.5 * hs * inv_avg_time);
+                avg_re = fma(wgt, (acc_t)o_re + (acc_t)op_re, avg_re);
+                avg_im = fma(wgt, (acc_t)o_im + (acc_t)op_im, avg_im);
+            }
+            op_re = o_re;
+            op_im = o_im;
+#endif
+#if GQ_OUT_MEAN_STATE
+            if (step >= a.warmup_steps) {
+                const acc_t wgt = (acc_t)(0.5 * hs * inv_avg_time);
+#pragma unroll
+                for (int r = 0; r < P::NR; ++r)
+                    savg[r] = fma(wgt, (acc_t)yn[r] + (acc_t)y[r], savg[r]);
+            }
+#endif
+#pragma unroll
+            for (int r = 0; r < P::NR; ++r) {
+                y[r] = yn[r];
+                k0[r] = kf[r];
+            }
+            tau = last ? dt_out : tau + hs;
+            h = last ? fmax(h, hs * factor) : hs * factor;
+        } else if (live) {
+            ++n_rej;
+            h = hs * factor;
+        }
+        bool interval_end = false;
+        if (live) {
+            if (++sub > max_sub) {
+                failed = true;
+#pragma unroll
+                for (int r = 0; r < P::NR; ++r) y[r] = (st_t)__builtin_nan("");
+            }
+            interval_end = failed || tau >= dt_out;
+        }
+#if GQ_TRACE
+        // trace sample at the end of an output interval (or at a failure inside it)
+        const bool sample = interval_end && step >= a.trace_offset &&
+                            (step - a.trace_offset) % a.trace_stride == 0 &&
+                            (step - a.trace_offset) / a.trace_stride < a.num_trace;
+        if (__any(sample)) {
+            const long long ld = a.trace_point_stride;
+            const int kk = (step - a.trace_offset) / a.trace_stride;
+#if GQ_TRACE == 1
+            gq_share<GI>(s_y[buf], y, lane);
+            __syncthreads();
+            if constexpr (GI == 0) {
+                if (sample) {
+                    real t_re, t_im;
+                    P::obs(s_y[buf], lane, px, py, C, Sg, Gg, t_re, t_im);
+                    bad_trace |= !(gq_finite(t_re) && gq_finite(t_im));
+                    gq_store_complex(a.trace, (long long)kk * ld + q, t_re, t_im);
+                }
+            }
+            buf ^= 1;
+#else
+            if (sample) {
+#pragma unroll
+                for (int r = 0; r < P::NR; ++r) {
+                    bad_trace |= !gq_finite((real)y[r]);
+                    ((real*)a.trace)[((long long)kk * GQ_D + P::R0 + r) * ld + q] = (real)y[r];
+                }
+            }
+#endif
+        }
+#endif
+        if (interval_end) {
+            ++step;
+            tau = 0.0;
+            sub = 0;
+        }
+        const bool done = live && (failed || step >= num_steps);
+        if (__any(done)) {
+            // ---- points finished: outputs -----------------------------------------------
+#if GQ_OUT_FINAL
+            gq_share<GI>(s_y[buf], y, lane);
+            __syncthreads();
+#endif
+            bool bad = false;
+            if (done) {
+#if GQ_OUT_MEAN
+                if constexpr (GI == 0) {
+                    const real r_re = failed ? (real)__builtin_nan("") : (real)avg_re;
+                    const real r_im = failed ? (real)__builtin_nan("") : (real)avg_im;
+                    bad = !(gq_finite(r_re) && gq_finite(r_im));
+                    gq_store_complex(a.result, q, r_re, r_im);
+                }
+#elif GQ_OUT_FINAL
+                if constexpr (GI == 0) {
+                    real r_re, r_im;
+                    P::obs(s_y[buf], lane, px, py, C, Sg, Gg, r_re, r_im);
+                    bad = !(gq_finite(r_re) && gq_finite(r_im));
+                    gq_store_complex(a.result, q, r_re, r_im);
+                }
+#elif GQ_OUT_FINAL_RHO
+#pragma unroll
+                for (int r = 0; r < P::NR; ++r) {
+                    const real v = (real)y[r];
+                    bad |= !gq_finite(v);
+                    ((real*)a.result)[(long long)(P::R0 + r) * a.point_count + q] = v;
+                }
+#elif GQ_OUT_MEAN_STATE
+#pragma unroll
+                for (int r = 0; r < P::NR; ++r) {
+                    const real v = failed ? (real)__builtin_nan("") : (real)savg[r];
+                    bad |= !gq_finite(v);
+                    ((real*)a.result)[(long long)(P::R0 + r) * a.point_count + q] = v;
+                }
+#endif
+#if GQ_COUNT_STEPS
+                if constexpr (GI == 0) {
+                    a.adaptive_steps[2 * q] = n_acc;
+                    a.adaptive_steps[2 * q + 1] = n_rej;
+                }
+#endif
+                live = false;
+            }
+#if GQ_OUT_FINAL
+            buf ^= 1;
+#endif
+            gq_flag(a.status, bad, GQ_STATUS_NONFINITE_RESULT);
+            if constexpr (GI == 0) gq_flag(a.status, done && failed, GQ_STATUS_SUBSTEP_OVERFLOW);
+        }
+    }
+    gq_flag(a.status, bad_trace, GQ_STATUS_NONFINITE_TRACE);
+}
+
+extern "C" __global__ __launch_bounds__(GQ_GW) void GQ_KERNEL(gq_adaptive_grouped)(const gq_sweep_args a) {
+    __shared__ real s_y[2][GQ_D * 64];
+    __shared__ double s_red[GQ_G * 64];
+    const int g = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int lane = (int)(threadIdx.x & 63u);
+    const long long q_raw = (long long)blockIdx.x * 64 + lane;
+    const bool valid = q_raw < a.point_count;
+    const long long q = valid ? q_raw : a.point_count - 1;
+    const gq_point pt = gq_locate(a, q);
+    const real px = ((const real*)a.par_x)[pt.ix];
+    const real py = ((const real*)a.par_y)[pt.iy];
+    GQ_GROUP_DISPATCH(g, gq_adaptive_body, (a, s_y, s_red, lane, q, valid, px, py, pt))
+}

@@ -457,7 +457,8 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
 
     if lanes_per_point > 1 and not grouped_applies(spec, compiled.state_size, fp64,
                                                    lanes_per_point):
-        raise ValueError(f"lanes_per_point={lanes_per_point} needs the rk4 solver and a state "
+        raise ValueError(f"lanes_per_point={lanes_per_point} needs the rk4, tsit5a or dop853a "
+                         f"solver and a state "
                          f"that splits into {lanes_per_point} row blocks whose LDS copy fits "
                          f"(state size {compiled.state_size}).")
     cached_source = RHSreuse and ("source", lanes_per_point) + variant in _KERNEL_CACHE
@@ -609,13 +610,19 @@ def _kernel_variant(compiled, spec, mode, uses_rho0_values, trace, state_trace, 
 LDS_BYTES_PER_CU = 160 * 1024      # gfx950
 
 
+GROUPED_TEMPLATES = {"rk4": "grouped_rk4.hip", "tsit5a": "grouped_adaptive.hip",
+                     "dop853a": "grouped_adaptive.hip"}
+
+
 def grouped_applies(spec, d, fp64, groups):
-    """The grouped RK4 kernel exists for this solver and its double-buffered LDS copy of the
-    state (2 x d x 64 reals) fits one compute unit."""
-    if spec.name != "rk4" or groups < 2:
+    """A grouped kernel exists for this solver, the state splits into ``groups`` row blocks and
+    the double-buffered LDS copy of the state (2 x d x 64 reals, plus the adaptive kernels'
+    error-norm partials) fits one compute unit."""
+    if spec.name not in GROUPED_TEMPLATES or groups < 2:
         return False
     rw = -(-d // groups)
-    return (groups - 1) * rw < d and 2 * d * 64 * (8 if fp64 else 4) <= LDS_BYTES_PER_CU
+    lds = 2 * d * 64 * (8 if fp64 else 4) + (groups * 64 * 8 if spec.adaptive else 0)
+    return (groups - 1) * rw < d and lds <= LDS_BYTES_PER_CU
 
 
 def _emit_source(compiled, spec, mode, uses_rho0_values, trace, state_trace, fp64, time_table,
@@ -630,7 +637,10 @@ def _emit_source(compiled, spec, mode, uses_rho0_values, trace, state_trace, fp6
         "GQ_TIME_TABLE": int(bool(time_table) and not spec.adaptive),
     }
     templates = ([spec.template] if spec.template != "fixed_rk4.hip"
-                 else ["rk4_table.hip", "fixed_rk4.hip" if groups == 1 else "grouped_rk4.hip"])
+                 else ["rk4_table.hip", "fixed_rk4.hip"])
+    if groups > 1:
+        templates = [t if t not in ("fixed_rk4.hip", "adaptive.hip")
+                     else GROUPED_TEMPLATES[spec.name] for t in templates]
     from . import tableaux
     extra = ""
     if spec.adaptive:

@@ -87,3 +87,71 @@ def test_autotuner_considers_grouped_kernels(gpu):
           flush=True)
     assert {c[0] for c in info["autotune"]} == {1, 2, 4, 8}
     assert _rel(got, ref) <= 1e-13
+
+
+ADAPT = {"step_rtol": 1e-8, "step_atol": 1e-10, "max_substeps": 2000, "count_steps": True}
+
+
+@pytest.mark.parametrize("solver", ["tsit5a", "dop853a"])
+@pytest.mark.parametrize("groups", [2, 4, 8])
+def test_grouped_adaptive_matches_per_lane(gpu, solver, groups):
+    # The error norm is summed per wave and then over the waves, so a borderline
+    # accept/reject decision can differ from the one-lane kernel's sequential sum; results
+    # then differ at the level of the step tolerance.
+    args, var = _chain8(n_eps=8, n_amp=8, n_t=101, t_end=5.0)
+    kw = dict(var_arrays=var, fp64=True, solver=solver, liouville="lanes",
+              adaptive_options=ADAPT, return_timing_info=True)
+    for mode in ("mean", "final_rho"):
+        ref, ri = mesolve_2D(*args, output_mode=mode, **kw)
+        got, gi = mesolve_2D(*args, output_mode=mode, lanes_per_point=groups, **kw)
+        same = np.mean(ri["adaptive_steps"]["accepted"] == gi["adaptive_steps"]["accepted"])
+        print(solver, groups, mode, _rel(got, ref), "same counts", same, flush=True)
+        assert _rel(got, ref) <= 1e-7 and same >= 0.95
+
+
+def test_grouped_adaptive_golden_duffing(gpu):
+    g = np.load(os.path.join(GOLDEN, "c4_duffing_8x8_subset_t20_tsit5a_fp64.npz"))
+    w = workloads.duffing(t_end=float(g["t_end"]), n_out=int(g["n_t"]))
+    gs, xs = list(w["var_arrays"])
+    from gqis import odesolve_2D
+    res, info = odesolve_2D(w["rhs"], w["state_symbols"], w["y0"], w["tlist"],
+                            var_arrays={gs: g["gamma"], xs: g["x0"]}, solver="tsit5a",
+                            fp64=True, return_timing_info=True, lanes_per_point=2,
+                            adaptive_options={"step_rtol": float(g["rtol"]),
+                                              "step_atol": float(g["atol"]),
+                                              "max_substeps": 1000, "count_steps": True})
+    same = np.mean(info["adaptive_steps"]["accepted"] == g["accepted"])
+    print("duffing grouped", _rel(res, g["result"]), same, flush=True)
+    assert same >= 0.9 and _rel(res, g["result"]) <= 1e-7
+
+
+def test_grouped_adaptive_traces_and_overflow(gpu):
+    args, var = _chain8(n_eps=5, n_amp=6, n_t=41, t_end=2.0)
+    kw = dict(var_arrays=var, fp64=True, solver="tsit5a", liouville="lanes",
+              adaptive_options=ADAPT)
+    for mode, every in (("mean", 3), ("final_rho", 10)):
+        ref = mesolve_2D(*args, output_mode=mode, return_time_trace=True,
+                         time_trace_every=every, **kw)
+        got = mesolve_2D(*args, output_mode=mode, return_time_trace=True,
+                         time_trace_every=every, lanes_per_point=4, **kw)
+        for a, b in zip(got[:2], ref[:2]):
+            assert a.shape == b.shape and _rel(a, b) <= 1e-7
+    tight = dict(kw, adaptive_options={"step_rtol": 1e-14, "step_atol": 1e-14,
+                                       "max_substeps": 2})
+    with pytest.raises(RuntimeError, match="Non-finite"):
+        mesolve_2D(*args, lanes_per_point=4, **tight)
+    res = mesolve_2D(*args, lanes_per_point=4, ignore_non_finite_output=True, **tight)
+    assert np.all(np.isnan(res))
+
+
+def test_autotuner_considers_grouped_adaptive_kernels(gpu):
+    autotune.clear_memory()
+    args, var = _chain8(n_eps=128, n_amp=128, n_t=101, t_end=5.0)
+    kw = dict(var_arrays=var, fp64=True, solver="tsit5a", liouville="lanes",
+              adaptive_options={"step_rtol": 1e-8, "step_atol": 1e-10, "max_substeps": 2000})
+    ref = mesolve_2D(*args, **kw)
+    got, info = mesolve_2D(*args, block_size="auto", return_timing_info=True, **kw)
+    print("autotune adaptive", info["lanes_per_point"], info["block_size"],
+          info.get("autotune"), flush=True)
+    assert {c[0] for c in info["autotune"]} == {1, 2, 4, 8}
+    assert _rel(got, ref) <= 1e-7
