@@ -19,8 +19,8 @@ import time
 
 import numpy as np
 
-from . import autotune, codegen, runtime
-from .solver import _emit_source, _runtime_const_values, _check_tlist
+from . import autotune, codegen, device, runtime
+from .solver import _emit_source, _runtime_const_values, _check_tlist, grouped_applies
 from .solvers import get_solver_spec
 from .tableaux import rows_per_step
 
@@ -116,10 +116,12 @@ class SweepServer:
                 "GQ_RHO0_FROM_ARGS", "0")
             kname = codegen.kernel_name("gq_hilbert_rk4", r.fp64)
             natoms = int(src.split("#define GQ_NATOMS ")[1].split()[0])
-            fits = tuple(b for b in autotune.CANDIDATES
-                         if hilbert.lds_bytes(hm.n, r.fp64, b, hilbert.jump_width(hm))
-                         <= hilbert.LDS_BYTES_PER_CU)
-            points_per_block = lambda b: (b // 64) * (32 // hm.n)     # noqa: E731
+            cands = tuple((1, b) for b in autotune.CANDIDATES
+                          if hilbert.lds_bytes(hm.n, r.fp64, b, hilbert.jump_width(hm))
+                          <= hilbert.LDS_BYTES_PER_CU)
+
+            def variant(c):
+                return (autotune.with_block(src, c[1]), kname, (c[1] // 64) * (32 // hm.n))
             consts_vals = cvals
             table_kernel = codegen.kernel_name("gq_rk4_time_table", r.fp64)
             rows = 2 * num_steps + 1
@@ -129,8 +131,18 @@ class SweepServer:
                                block, False)
             kname = codegen.kernel_name(spec.kernel, r.fp64)
             natoms = len(r.compiled.plan.time_atoms)
-            fits = autotune.CANDIDATES
-            points_per_block = lambda b: b                               # noqa: E731
+            size = r.compiled.state_size
+            cands = tuple((1, b) for b in autotune.CANDIDATES)
+            if size >= device.MIN_GROUPED_STATES:
+                cands += tuple((g, 64 * g) for g in device.GROUP_CHOICES
+                               if grouped_applies(spec, size, r.fp64, g))
+
+            def variant(c):
+                g, b = c
+                text = src if g == 1 else _emit_source(r.compiled, spec, r.output_mode, False,
+                                                       False, False, r.fp64, True, b, False, g)
+                return (autotune.with_block(text, b), device._kernel_for(spec, g, r.fp64),
+                        64 if g > 1 else b)
             table_kernel = codegen.kernel_name(
                 "gq_rk4_time_table" if spec.template == "fixed_rk4.hip" else "gq_fixed_time_table",
                 r.fp64)
@@ -147,21 +159,23 @@ class SweepServer:
                             torch.cuda.current_stream(self.dev).cuda_stream)
             torch.cuda.synchronize(self.dev)
         m = {"module": mod, "spec": spec, "num_steps": num_steps, "dt": dt, "kernel": kname,
-             "block": block, "per_block": points_per_block(block), "consts": consts,
-             "table": table, "hilbert": hm is not None, "autotune": {}}
+             "block": block, "per_block": variant((1, block))[2], "groups": 1,
+             "consts": consts, "table": table, "hilbert": hm is not None, "autotune": {}}
         if self.block == "auto":
-            block, m["autotune"] = autotune.choose(
+            choice, m["autotune"] = autotune.choose(
                 self.ctx, src, codegen.HIPRTC_BASE_OPTIONS,
-                lambda module, b: self._trial(m, r, key, module, b, points_per_block(b)), fits)
-            m["module"], _, _ = self.ctx.compile(autotune.with_block(src, block),
-                                                 codegen.HIPRTC_BASE_OPTIONS)
-            m["block"], m["per_block"] = block, points_per_block(block)
+                lambda module, c: self._trial(m, r, module, c, *variant(c)[1:]), cands,
+                source_of=lambda c: variant(c)[0])
+            text, m["kernel"], m["per_block"] = variant(choice)
+            m["module"], _, _ = self.ctx.compile(text, codegen.HIPRTC_BASE_OPTIONS)
+            m["groups"], m["block"] = choice
         self.modules[key] = m
         return m
 
-    def _trial(self, m, r, key, module, block, per_block, side=256, steps=64):
-        """One timed launch of a candidate work-group size: side x side points spread over the
-        request's parameter ranges, the first ``steps`` steps of its time grid."""
+    def _trial(self, m, r, module, cand, kernel, per_block, side=256, steps=64):
+        """One timed launch of a candidate (lanes per point, work-group size): side x side
+        points spread over the request's parameter ranges, the first ``steps`` steps of its
+        time grid."""
         torch = self.torch
         rdt = torch.float64 if r.fp64 else torch.float32
         x = np.linspace(float(np.min(r.axes[0])), float(np.max(r.axes[0])), side)
@@ -188,13 +202,13 @@ class SweepServer:
         for i, v in enumerate(spec.parameters[:runtime.MAX_SOLVER_PARAMS]):
             a.solver_param[i] = float(v)
         a.result, a.status, a.work_cursor = result.data_ptr(), flags.data_ptr(), flags.data_ptr() + 4
+        groups, block = cand
         grid = (npts + per_block - 1) // per_block
-        if spec.adaptive and spec.options.get("persistent", True):
-            grid = min(grid, self.ctx.resident_groups(module, m["kernel"], block))
-        from .device import _timed_launches
+        if spec.adaptive and spec.options.get("persistent", True) and groups == 1:
+            grid = min(grid, self.ctx.resident_groups(module, kernel, block))
         stream = torch.cuda.current_stream(self.dev).cuda_stream
-        return _timed_launches(self.ctx, module, m["kernel"], max(1, grid), block, a, stream,
-                               flags[1:])
+        return device._timed_launches(self.ctx, module, kernel, max(1, grid), block, a, stream,
+                                      flags[1:])
 
     def _hilbert_model(self, r):
         """Matrix-core model of a Lindblad request with N >= 16 whose operators fit the
@@ -320,7 +334,7 @@ class SweepServer:
             sid = stream.cuda_stream
             start_ev.record(stream)
             grid = (npts + ent["per_block"] - 1) // ent["per_block"]
-            if spec.adaptive and spec.options.get("persistent", True):
+            if spec.adaptive and spec.options.get("persistent", True) and ent["groups"] == 1:
                 if "resident" not in ent:
                     ent["resident"] = self.ctx.resident_groups(ent["module"], ent["kernel"],
                                                                ent["block"])

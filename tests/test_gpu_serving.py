@@ -74,3 +74,37 @@ def test_non_finite_request_does_not_contaminate_the_batch(gpu):
     assert np.all(np.isfinite(good.result))
     ref = mesolve_2D(c, tlist=w["tlist"], var_arrays=dict(zip(c.sweep_symbols, good.axes)))
     np.testing.assert_array_equal(good.result, ref)
+
+
+def test_large_models_alone_and_in_the_mix(gpu):
+    # 63 states: the server's measured choice may split each point over several lanes; the
+    # choice is made once per kernel, so alone and mixed launches run the same code
+    w = workloads.chain(8, n_t=201, t_end=10.0)
+    c = compile_lindblad_rhs(w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"],
+                             list(w["var_arrays"]))
+    rng = np.random.default_rng(11)
+
+    def reqs():
+        out = []
+        for i in range(6):
+            side = [64, 128, 20][i % 3]
+            axes = (np.linspace(-4, 4, side), np.linspace(0, 3.0 + i, side))
+            out.append(SweepRequest(c, w["tlist"], axes, solver=["rk4", "tsit5a"][i % 2],
+                                    fp64=i < 4, arrival=0.001 * i))
+        return out
+    alone = [SweepServer(streams=1, max_batch_points=1).serve([r], realtime=False)[0].result
+             for r in reqs()]
+    mixed = reqs()
+    srv = SweepServer(streams=3)
+    srv.serve(mixed, realtime=False)
+    for a, r in zip(alone, mixed):
+        assert r.status == 0
+        np.testing.assert_array_equal(a, r.result)
+    for r in mixed:
+        api = mesolve_2D(c, tlist=w["tlist"], var_arrays=dict(zip(c.sweep_symbols, r.axes)),
+                         solver=r.solver, fp64=r.fp64)
+        scale = float(np.max(np.abs(api)))
+        tol = 1e-10 if r.fp64 else 1e-5
+        assert float(np.max(np.abs(r.result - api))) <= tol * scale
+    print("chosen", {k[1:3]: (m["groups"], m["block"]) for k, m in srv.modules.items()},
+          flush=True)
