@@ -8,7 +8,8 @@
 // Time atoms (the time-only parts of the drives) come from a table built in FP64 once per
 // launch when GQ_TIME_TABLE is 1: index 0 is t = 0, 2s+1 the midpoint and 2s+2 the end of step
 // s.  With GQ_TIME_TABLE 0 each lane evaluates them in its own precision from a time computed
-// as the reference does.
+// as the reference does.  The table rows of a step are loaded one step ahead, so the latency of
+// those (scalar) loads is hidden behind the current step's arithmetic.
 
 __device__ __forceinline__ void gq_time_atoms(const gq_sweep_args& a, const real* __restrict__ C,
                                               int stage_index, real t_local,
@@ -51,10 +52,37 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_rk4)(const g
 
     gq_outputs out;
     gq_out_init(out);
+#if GQ_TIME_TABLE
+    const real* __restrict__ tab = (const real*)a.time_table;
+    real TT_mid[GQ_NATOMS_ALLOC], TT_end[GQ_NATOMS_ALLOC];
+    {
+        const int first = (num_steps > 0) ? 1 : 0;   // the table has 2 * num_steps + 1 rows
+#pragma unroll
+        for (int k = 0; k < GQ_NATOMS; ++k) {
+            TT_mid[k] = tab[first * GQ_NATOMS + k];
+            TT_end[k] = tab[2 * first * GQ_NATOMS + k];
+        }
+    }
+#endif
 
     for (int step = 0; step < num_steps; ++step) {
         real t_mid = ((real)step + R(0.5)) * dt;
+#if GQ_TIME_TABLE
+        real TT_end_now[GQ_NATOMS_ALLOC];
+        {
+            // rows 2s+1 (midpoint) and 2s+2 (end) of this step; next step's rows requested now
+            const int next = (step + 1 < num_steps) ? 2 * step + 3 : 2 * step + 1;
+#pragma unroll
+            for (int k = 0; k < GQ_NATOMS; ++k) {
+                TT[k] = TT_mid[k];
+                TT_end_now[k] = TT_end[k];
+                TT_mid[k] = tab[next * GQ_NATOMS + k];
+                TT_end[k] = tab[(next + 1) * GQ_NATOMS + k];
+            }
+        }
+#else
         gq_time_atoms(a, C, 2 * step + 1, t_mid, TT);
+#endif
         gq_stage(px, py, C, S, TT, G);
         real yt[GQ_D], acc[GQ_D];
 #pragma unroll
@@ -75,7 +103,12 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_rk4)(const g
             acc[i] = fma(R(2.0), k[i], acc[i]);
         }
         t_mid += dt2;
+#if GQ_TIME_TABLE
+#pragma unroll
+        for (int k = 0; k < GQ_NATOMS; ++k) TT[k] = TT_end_now[k];
+#else
         gq_time_atoms(a, C, 2 * step + 2, t_mid, TT);
+#endif
         gq_stage(px, py, C, S, TT, G);
         gq_rhs(yt, px, py, C, S, G, k);
 #pragma unroll
