@@ -121,6 +121,11 @@ struct gq_module {
     int refs = 1;
 };
 
+struct gq_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
 struct gq_ctx {
     int device = 0;
     std::string arch;
@@ -290,6 +295,61 @@ int gq_launch_sweep(gq_module* module, const char* name, unsigned int grid, unsi
                                          params, nullptr);
     if (e != hipSuccess) return hip_fail(e, "hipModuleLaunchKernel");
     return GQ_OK;
+}
+
+int gq_graph_create(gq_module* module, const gq_launch_desc* launches, int count,
+                    const gq_sweep_args* args, gq_graph** out) {
+    if (!module || !launches || count <= 0 || !args || !out)
+        return fail(GQ_E_INVALID, "gq_graph_create: null argument or empty launch list");
+    std::vector<hipFunction_t> fns((size_t)count);
+    for (int i = 0; i < count; ++i) {
+        if (!launches[i].name || launches[i].grid == 0 || launches[i].block == 0 ||
+            launches[i].block > 1024)
+            return fail(GQ_E_INVALID, "gq_graph_create: bad launch description");
+        int rc = get_function(module, launches[i].name, &fns[(size_t)i]);
+        if (rc != GQ_OK) return rc;
+    }
+    hipError_t e = hipSetDevice(module->ctx->device);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    // capture on a private stream, so that no other work of the process is recorded
+    hipStream_t s = nullptr;
+    e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamCreateWithFlags");
+    gq_graph* g = new gq_graph();
+    e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        gq_sweep_args local = *args;
+        void* params[] = {&local};
+        for (int i = 0; i < count && e == hipSuccess; ++i) {
+            e = hipModuleLaunchKernel(fns[(size_t)i], launches[i].grid, 1, 1, launches[i].block,
+                                      1, 1, 0, s, params, nullptr);
+        }
+        hipGraph_t graph = nullptr;
+        hipError_t e_end = hipStreamEndCapture(s, &graph);
+        if (e == hipSuccess) e = e_end;
+        g->graph = graph;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    (void)hipStreamDestroy(s);
+    if (e != hipSuccess) {
+        gq_graph_destroy(g);
+        return hip_fail(e, "gq_graph_create");
+    }
+    *out = g;
+    return GQ_OK;
+}
+
+int gq_graph_launch(gq_graph* graph, void* stream) {
+    if (!graph || !graph->exec) return fail(GQ_E_INVALID, "gq_graph_launch: null graph");
+    hipError_t e = hipGraphLaunch(graph->exec, (hipStream_t)stream);
+    return e == hipSuccess ? GQ_OK : hip_fail(e, "hipGraphLaunch");
+}
+
+void gq_graph_destroy(gq_graph* graph) {
+    if (!graph) return;
+    if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+    if (graph->graph) (void)hipGraphDestroy(graph->graph);
+    delete graph;
 }
 
 int gq_resident_groups(gq_module* module, const char* name, unsigned int block, int* groups) {

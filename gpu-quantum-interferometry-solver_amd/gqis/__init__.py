@@ -6,7 +6,8 @@ load lazily, so ``import gqis`` needs neither torch nor a GPU.
 from importlib import import_module
 
 __all__ = ["build_independent_rho", "build_reduced_lindblad_rhs", "mesolve_2D", "odesolve_2D",
-           "compile_lindblad_rhs", "compile_ode_rhs", "CompiledRHS", "set_kernel_cache_dir"]
+           "compile_lindblad_rhs", "compile_ode_rhs", "CompiledRHS", "set_kernel_cache_dir",
+           "prepare_sweep"]
 _ODE_API_NAMES = {"odesolve_2D", "compile_ode_rhs"}
 
 __version__ = "0.3.0+mi355x"

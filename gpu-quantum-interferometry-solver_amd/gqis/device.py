@@ -56,10 +56,12 @@ def _grid(npts, groups, block):
 def run_sweep(*, source, variant, spec, compiled, mode, var_arrays, rho0_values, const_vals,
               fp64, dt, t_in, num_steps, warmup_steps, num_trace, trace_stride, trace_offset,
               state_trace, time_table, block_size, options, reuse, collect, emit=None,
-              lanes_per_point=1):
+              lanes_per_point=1, prepare=False):
     """Buffers, time table, launch and output layouts of one sweep.  ``source`` is the kernel
     for ``lanes_per_point`` lanes per point; ``emit(groups)`` returns the source of another
-    grouping (used when ``block_size`` is "auto")."""
+    grouping (used when ``block_size`` is "auto").  With ``prepare`` the buffers, module and
+    launch chain are returned without launching (``collect_outputs`` turns them into results
+    after a launch)."""
     import torch
     setup_start = time.time()
     ctx = runtime.Context.get()
@@ -164,42 +166,63 @@ def run_sweep(*, source, variant, spec, compiled, mode, var_arrays, rho0_values,
         cursor.zero_()
         tune_s = time.time() - t0
     kname = _kernel_for(spec, groups, fp64)
-    timer = runtime.EventTimer(ctx, stream) if collect else None
-    setup_s = time.time() - setup_start
+    launches = []
     if npts > 0 and num_steps >= 0:
-        if timer:
-            timer.record_start()
         if use_table and rows > 0:
-            ctx.launch(module, codegen.kernel_name(table_kernel(spec), fp64), (rows + 255) // 256,
-                       256, a, stream)
+            launches.append((codegen.kernel_name(table_kernel(spec), fp64), (rows + 255) // 256,
+                             256))
         if persistent and groups == 1:
             grid = min(ctx.resident_groups(module, kname, block_size),
                        (npts + block_size - 1) // block_size)
         else:
             grid = _grid(npts, groups, block_size)
-        ctx.launch(module, kname, max(grid, 1), block_size, a, stream)
-        if timer:
-            timer.record_stop()
+        launches.append((kname, max(grid, 1), block_size))
+    plan = {"ctx": ctx, "module": module, "args": a, "launches": launches, "fp64": fp64,
+            "mode": mode, "nx": nx, "ny": ny, "size": size, "num_trace": num_trace,
+            "state_trace": state_trace, "adaptive": spec.adaptive,
+            "buffers": {"px": px, "py": py, "consts": consts, "rho0": rho0, "result": result,
+                        "trace": trace, "status": status, "cursor": cursor, "table": table,
+                        "steps": steps_out},
+            "info": {"build_s": build_s, "cached_kernel": cached, "block_size": int(block_size),
+                     "lanes_per_point": int(groups), "autotune_s": tune_s,
+                     "autotune": tune_times}}
+    if prepare:
+        return plan
+    timer = runtime.EventTimer(ctx, stream) if collect else None
+    setup_s = time.time() - setup_start
+    if timer:
+        timer.record_start()
+    for name, grid, block in launches:
+        ctx.launch(module, name, grid, block, a, stream)
+    if timer:
+        timer.record_stop()
     flags = ctx.read_status(status, stream)
     gpu_s = timer.elapsed_ms() / 1000.0 if timer else 0.0
     if timer:
         timer.close()
+    out = collect_outputs(plan, flags)
+    out.update(gpu_s=gpu_s, setup_s=setup_s)
+    return out
 
-    if mode in ("final_rho", "mean_state"):
-        out = result[:, :npts].t().reshape(nx, ny, size).contiguous()
+
+def collect_outputs(plan, flags):
+    """Results of a launched plan in the reference's shapes (device tensors) and its info."""
+    b = plan["buffers"]
+    nx, ny, size, num_trace = plan["nx"], plan["ny"], plan["size"], plan["num_trace"]
+    npts = nx * ny
+    if plan["mode"] in ("final_rho", "mean_state"):
+        out = b["result"][:, :npts].t().reshape(nx, ny, size).contiguous()
     else:
-        out = result[:npts].reshape(nx, ny)
+        out = b["result"][:npts].reshape(nx, ny).clone()
     tr = None
-    if trace is not None:
-        if state_trace:
-            tr = trace.permute(2, 0, 1).reshape(nx, ny, num_trace, size).contiguous()
+    if b["trace"] is not None:
+        if plan["state_trace"]:
+            tr = b["trace"].permute(2, 0, 1).reshape(nx, ny, num_trace, size).contiguous()
         else:
-            tr = trace.t().reshape(nx, ny, num_trace).contiguous()
-    steps = steps_out.reshape(nx, ny, 2).cpu().numpy() if steps_out is not None else None
-    return {"result": out, "trace": tr, "status": flags, "gpu_s": gpu_s, "build_s": build_s,
-            "cached_kernel": cached, "setup_s": setup_s, "adaptive_steps": steps,
-            "block_size": int(block_size), "lanes_per_point": int(groups),
-            "autotune_s": tune_s, "autotune": tune_times}
+            tr = b["trace"].t().reshape(nx, ny, num_trace).contiguous()
+    steps = b["steps"].reshape(nx, ny, 2).cpu().numpy() if b["steps"] is not None else None
+    return {"result": out, "trace": tr, "status": flags, "adaptive_steps": steps,
+            "gpu_s": 0.0, "setup_s": 0.0, **plan["info"]}
 
 
 def _timed_launches(ctx, module, kname, grid, block, args, stream, cursor, repeats=2):

@@ -46,6 +46,15 @@ __device__ __forceinline__ void gq_flag(unsigned int* status, bool bad, unsigned
     }
 }
 
+// Status word and work cursor back to zero: the first node of a replayed launch chain (a
+// kernel rather than a memset, which a replayed graph must not contain).
+extern "C" __global__ void GQ_KERNEL(gq_clear_flags)(const gq_sweep_args a) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (a.status) *a.status = 0u;
+        if (a.work_cursor) *a.work_cursor = 0u;
+    }
+}
+
 // Outputs at local index q (layouts in gqis_hip.h).
 __device__ __forceinline__ void gq_store_complex(void* base, long long idx, real re, real im) {
     real2 v;

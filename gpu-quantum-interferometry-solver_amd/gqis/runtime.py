@@ -48,6 +48,11 @@ class SweepArgs(ctypes.Structure):
     ]
 
 
+class LaunchDesc(ctypes.Structure):
+    """Host mirror of ``gq_launch_desc`` (one launch of a recorded graph)."""
+    _fields_ = [("name", ctypes.c_char_p), ("grid", ctypes.c_uint), ("block", ctypes.c_uint)]
+
+
 _lib = None
 _lib_lock = threading.Lock()
 
@@ -83,6 +88,10 @@ def load_library(path=LIB_PATH):
             "gq_resident_groups": (c_int, [c_void_p, c_char_p, ctypes.c_uint,
                                            ctypes.POINTER(c_int)]),
             "gq_module_release": (None, [c_void_p]),
+            "gq_graph_create": (c_int, [c_void_p, ctypes.POINTER(LaunchDesc), c_int,
+                                        ctypes.POINTER(SweepArgs), ctypes.POINTER(c_void_p)]),
+            "gq_graph_launch": (c_int, [c_void_p, c_void_p]),
+            "gq_graph_destroy": (None, [c_void_p]),
             "gq_event_create": (c_int, [ctypes.POINTER(c_void_p)]),
             "gq_event_record": (c_int, [c_void_p, c_void_p]),
             "gq_event_elapsed_ms": (c_int, [c_void_p, c_void_p, ctypes.POINTER(ctypes.c_float)]),
@@ -109,6 +118,7 @@ def exported_symbols():
     return ("gq_ctx_create", "gq_ctx_destroy", "gq_ctx_info", "gq_ctx_set_cache_dir",
             "gq_module_compile",
             "gq_compile_check", "gq_launch_sweep", "gq_resident_groups", "gq_module_release",
+            "gq_graph_create", "gq_graph_launch", "gq_graph_destroy",
             "gq_event_create", "gq_event_record", "gq_event_elapsed_ms", "gq_event_destroy",
             "gq_stream_synchronize", "gq_status_read", "gq_sweep_args_size", "gq_last_error")
 
@@ -205,6 +215,26 @@ class Context:
         _check(self.lib.gq_launch_sweep(module, name.encode(), int(grid), int(block),
                                         ctypes.byref(args), ctypes.c_void_p(stream)),
                f"launch {name}")
+
+    def graph_create(self, module, launches, args):
+        """Record ``launches`` ([(kernel name, grid, block)], in order) with ``args`` as one
+        HIP graph; returns its handle (``graph_launch`` replays it, ``graph_destroy`` frees)."""
+        names = [n.encode() for n, _, _ in launches]
+        descs = (LaunchDesc * len(launches))(*[LaunchDesc(nm, int(g), int(b))
+                                                for nm, (_, g, b) in zip(names, launches)])
+        h = ctypes.c_void_p()
+        _check(self.lib.gq_graph_create(module, descs, len(launches), ctypes.byref(args),
+                                        ctypes.byref(h)), "gq_graph_create")
+        return h
+
+    def graph_launch(self, graph, stream=None):
+        if stream is None:
+            stream = self.torch.cuda.current_stream(self.device).cuda_stream
+        _check(self.lib.gq_graph_launch(graph, ctypes.c_void_p(stream)), "gq_graph_launch")
+
+    def graph_destroy(self, graph):
+        if graph:
+            self.lib.gq_graph_destroy(graph)
 
     def resident_groups(self, module, name, block):
         g = ctypes.c_int(0)

@@ -270,6 +270,31 @@ def set_kernel_cache_dir(path, device=None):
     Context.get(device).set_cache_dir(path)
 
 
+def prepare_sweep(*args, **kwargs):
+    """A sweep prepared once and replayed many times (``PreparedSweep.run``): same arguments
+    as ``mesolve_2D``; the per-point kernels (any fixed-step or adaptive solver, lanes per
+    point) with their buffers resident and the launch chain recorded as a HIP graph.  For
+    fitting loops that re-run one sweep with new sweep values, runtime constants or initial
+    states.  No counterpart in the reference, which redoes the whole call each time."""
+    import inspect
+    bound = inspect.signature(mesolve_2D).bind(*args, **kwargs)
+    bound.apply_defaults()
+    call = dict(bound.arguments)
+    groups = {name: call.pop(name) for name in OPTION_GROUPS}
+    call, _ = apply_option_groups(mesolve_2D, call, groups)
+    if call["output_mode"] not in {"mean", "final", "final_rho"}:
+        raise ValueError("output_mode must be 'mean', 'final', or 'final_rho'.")
+    for name in ("return_timing_info", "timings"):
+        if call[name]:
+            raise ValueError(f"{name} is not available for a prepared sweep (use run(timed=True) "
+                             "and last_gpu_s).")
+    if isinstance(call["H"], CompiledRHS):
+        if call["H"].kind != "lindblad":
+            raise ValueError("This CompiledRHS comes from compile_ode_rhs; pass it to odesolve_2D.")
+        call.update(H=None, compiled_rhs=call["H"])
+    return _solve_symbolic_system(**call, _prepare=True)
+
+
 def _check_tlist(tlist):
     """(num_steps, dt) of a uniform grid starting at zero (``gqis/solver.py:1951-1979``)."""
     if tlist is None:
@@ -306,9 +331,12 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
                            return_device=False, calibrate_timesteps=None, compiled_rhs=None,
                            hiprtc_options=(), time_table=True, block_size=256,
                            liouville="auto", lanes_per_point=1, _ode_size=None,
-                           _state_trace=False,
+                           _state_trace=False, _prepare=False,
                            _time_trace_offset=0):
     if calibrate_timesteps is not None and calibrate_timesteps is not False:
+        if _prepare:
+            raise ValueError("calibrate_timesteps cannot be combined with prepare_sweep: "
+                             "calibrate once with mesolve_2D, then prepare on the chosen tlist.")
         call = dict(locals())
         from .calibration import solve_with_timestep_calibration
         return solve_with_timestep_calibration(call)
@@ -409,7 +437,10 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
     if liouville not in ("auto", "lanes", "mfma"):
         raise ValueError("liouville must be 'auto', 'lanes' or 'mfma'.")
     hilbert_out = None
-    if (compiled_rhs is None and _ode_size is None and liouville != "lanes"
+    if _prepare and liouville == "mfma":
+        raise ValueError("prepare_sweep runs the per-point kernels; use liouville='auto' or "
+                         "'lanes'.")
+    if (compiled_rhs is None and _ode_size is None and liouville != "lanes" and not _prepare
             and not (lanes_per_point > 1 and liouville == "auto")):
         hilbert_out = _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values,
                                    runtime_consts, keep_symbolic_consts, auto_runtime_consts,
@@ -485,7 +516,20 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
         num_trace=num_trace, trace_stride=trace_stride, trace_offset=trace_offset,
         state_trace=_state_trace, time_table=time_table, block_size=block_size,
         options=tuple(hiprtc_options) + _as_tuple(nvrtc_options), reuse=RHSreuse,
-        collect=collect)
+        collect=collect, prepare=_prepare)
+    if _prepare:
+        from .prepared import PreparedSweep
+
+        def finish(outs, *, return_device, ignore_non_finite_output, total_start):
+            return _finish(outs, stage_s, rhs_status, spec, solver_frequency, unroll, False,
+                           False, return_device, return_time_trace, False,
+                           ignore_non_finite_output, _ode_size, num_trace, trace_stride,
+                           trace_offset, fp64, t_in, dt, total_start)
+        return PreparedSweep(outputs, compiled=compiled, spec=spec, const_values=const_values,
+                             runtime_consts=runtime_consts,
+                             use_defaults=compiled_rhs is not None,
+                             sweep_symbols=tuple(var_arrays),
+                             uses_rho0_values=uses_rho0_values, finish=finish)
     return _finish(outputs, stage_s, rhs_status, spec, solver_frequency, unroll, collect, timings,
                    return_device, return_time_trace, return_timing_info, ignore_non_finite_output,
                    _ode_size, num_trace, trace_stride, trace_offset, fp64, t_in, dt, total_start)

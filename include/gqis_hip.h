@@ -112,6 +112,22 @@ int gq_launch_sweep(gq_module* module, const char* name, unsigned int grid, unsi
 int gq_resident_groups(gq_module* module, const char* name, unsigned int block, int* groups);
 void gq_module_release(gq_module* module);
 
+/* A fixed chain of launches of one module with one argument struct, recorded once as a HIP
+ * graph and replayed with a single call (the inputs live in device buffers the caller updates
+ * between replays).  No counterpart in the reference, which launches once per call
+ * (gqis/solver.py:2322-2340); it removes per-launch host work from repeated sweeps.  The chain
+ * should start with gq_clear_flags_<f32|f64> (status word and work cursor back to zero: a
+ * kernel, not a memset, inside the graph). */
+typedef struct gq_graph gq_graph;
+typedef struct gq_launch_desc {
+    const char* name;
+    unsigned int grid, block;
+} gq_launch_desc;
+int gq_graph_create(gq_module* module, const gq_launch_desc* launches, int count,
+                    const gq_sweep_args* args, gq_graph** out);
+int gq_graph_launch(gq_graph* graph, void* stream);
+void gq_graph_destroy(gq_graph* graph);
+
 /* Event pair timing on a stream (cp.cuda.Event, gqis/solver.py:2316-2345). */
 int gq_event_create(void** event);
 int gq_event_record(void* event, void* stream);

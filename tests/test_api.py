@@ -197,3 +197,17 @@ def test_package_import_is_lazy():
     env["PYTHONPATH"] = os.path.dirname(os.path.dirname(gqis.__file__))
     proc = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env)
     assert proc.returncode == 0, proc.stderr
+
+
+def test_prepare_sweep_argument_checks():
+    from gqis import prepare_sweep
+    w = workloads.lzs_two_level(n_eps=4, n_amp=4, n_t=11, t_end=1.0)
+    args = (w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"])
+    with pytest.raises(ValueError, match="calibrate_timesteps cannot be combined"):
+        prepare_sweep(*args, var_arrays=w["var_arrays"], calibrate_timesteps=True)
+    with pytest.raises(ValueError, match="not available for a prepared sweep"):
+        prepare_sweep(*args, var_arrays=w["var_arrays"], return_timing_info=True)
+    with pytest.raises(ValueError, match="per-point kernels"):
+        prepare_sweep(*args, var_arrays=w["var_arrays"], liouville="mfma")
+    with pytest.raises(ValueError, match="output_mode"):
+        prepare_sweep(*args, var_arrays=w["var_arrays"], output_mode="mean_state")
