@@ -200,7 +200,7 @@ def test_package_import_is_lazy():
 
 
 def test_prepare_sweep_argument_checks():
-    from gqis import prepare_sweep
+    from gqis import prepare_sweep, workloads
     w = workloads.lzs_two_level(n_eps=4, n_amp=4, n_t=11, t_end=1.0)
     args = (w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"])
     with pytest.raises(ValueError, match="calibrate_timesteps cannot be combined"):
