@@ -306,7 +306,8 @@ def bench_duffing(args):
 def bench_scaling(args):
     """Hilbert-space scaling: seeded random N-level models (N = 4, 8, 16, 32), 256 x 256 grid,
     RK4 2000 steps (dt = 0.01).  N = 4 runs the per-point register kernel; N = 8, 16, 32 the
-    matrix-core path (FP32 MFMA), N = 8 also the register kernel.  Errors are against the FP64
+    matrix-core path (FP32 MFMA, and FP16 / BF16 hi/lo split products), N = 8 also the
+    register kernel.  Errors are against the FP64
     GPU paths (per-point FP64 for N = 4 and 8, FP64 Hilbert-space path for N = 16, 32) on a
     16 x 16 subset of the grid."""
     import numpy as np
@@ -317,8 +318,9 @@ def bench_scaling(args):
            "unit": "param-point RK steps/s", "n_gpus": 1, "data": "synthetic",
            "config": {"workload": f"random N-level, {side}x{side}, RK4 2000 steps dt 0.01",
                       "points": side * side}, "results": {}}
-    for n, forms in ((4, ["lanes"]), (8, ["lanes", "mfma", "lanes64", "mfma64"]), (16, ["mfma"]),
-                     (32, ["mfma"])):
+    for n, forms in ((4, ["lanes"]), (8, ["lanes", "mfma", "mfma_f16", "lanes64", "mfma64"]),
+                     (16, ["mfma", "mfma_f16", "mfma_bf16"]),
+                     (32, ["mfma", "mfma_f16", "mfma_bf16"])):
         w = workloads.random_n_level(n, n_eps=side, n_amp=side)
         sub = {k: v[:: side // 16] for k, v in w["var_arrays"].items()}
         args_m = (w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"])
@@ -337,13 +339,17 @@ def bench_scaling(args):
             k = info["gpu_kernel_s"]
             ps = side * side * 2000
             d2 = n * n
-            flops = 4 * 8 * n ** 3 + 4 * 12 * d2 + 13 * (d2 - 1) if form == "mfma" else None
+            flops = (4 * 8 * n ** 3 + 4 * 12 * d2 + 13 * (d2 - 1)
+                     if form.startswith("mfma") and not fp64 else None)
             entry = {"kernel_s": k, "value": ps / k,
                      "max_abs_err_vs_fp64": float(np.max(np.abs(sub_res - ref)))}
             if flops:
                 ach = flops * ps / k / 1e12
                 entry.update({"flops_per_point_step": flops, "achieved_tflops": ach,
                               "mfma_fp32_peak_frac": ach / 157.3})
+                if form != "mfma":
+                    # three half-precision products per FP32-equivalent product
+                    entry["half_mfma_issued_frac"] = 3 * ach / 2500.0
             out["results"][f"N={n} {form}"] = entry
     print(json.dumps(out), flush=True)
 

@@ -143,13 +143,52 @@ def pack_a(m):
     return out
 
 
+SPLIT_KINDS = {None: 0, "f16": 1, "bf16": 2}
+
+
+def _round_half(x, kind):
+    """Round FP64 values to FP16 or BF16 (round to nearest even), returned as FP64."""
+    if kind == "f16":
+        return x.astype(np.float16).astype(np.float64)
+    f = x.astype(np.float32).view(np.uint32).astype(np.uint64)
+    rounded = (f + 0x7FFF + ((f >> 16) & 1)) & 0xFFFF0000
+    return rounded.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def pack_a_split(m, kind):
+    """A operand of the split-precision kernel, hi and lo parts: [tile][K block][lane][8],
+    element j of lane (r, h) in block b holding Hbig[32 t + r][kappa(8 b + j, h)] with
+    kappa(s, h) = 8 (s // 4) + 4 h + s % 4, the Rbig row of the state value x[s] that lanes of
+    half h hold (the B operand is the state itself, no reordering)."""
+    n = m.n
+    big = np.zeros((2 * n, 2 * n))
+    big[:n, :n], big[:n, n:] = m.h_eff.real, -m.h_eff.imag
+    big[n:, :n], big[n:, n:] = m.h_eff.imag, m.h_eff.real
+    tiles, blocks = (2 * n + 31) // 32, n // 8
+    full = np.zeros((tiles, blocks, 64, 8))
+    for t in range(tiles):
+        for b in range(blocks):
+            for lane in range(64):
+                r, h = lane & 31, lane >> 5
+                if 32 * t + r >= 2 * n:
+                    continue
+                for j in range(8):
+                    s_idx = 8 * b + j
+                    kappa = 8 * (s_idx // 4) + 4 * h + (s_idx % 4)
+                    full[t, b, lane, j] = big[32 * t + r, kappa]
+    hi = _round_half(full, kind)
+    lo = _round_half(full - hi, kind)
+    return hi, lo
+
+
 def jump_width(m):
     """Largest number of populations feeding one population (at least 1)."""
     return max(1, int(np.max(np.count_nonzero(m.jump, axis=1))))
 
 
-def constants(m, runtime_values):
-    """Flat FP32 constant buffer and the offsets the kernel reads."""
+def constants(m, runtime_values, split=None):
+    """Flat constant buffer and the offsets the kernel reads (FP16 / BF16 operand parts are
+    stored as the FP32 values they represent exactly)."""
     rc = np.asarray(runtime_values if runtime_values else [0.0], np.float64)
     jmax = jump_width(m)
     jw = np.zeros((m.n, jmax))
@@ -163,6 +202,9 @@ def constants(m, runtime_values):
     big[n:, :n], big[n:, n:] = m.h_eff.imag, m.h_eff.real
     parts = {"RC": rc, "APACK": pack_a(m).ravel(), "HBIG": big.ravel(), "DEP": m.dep.ravel(),
              "JW": jw.ravel(), "JB": jb.ravel(), "VCOEF": m.vcoef.ravel(), "OBS": m.obs}
+    if split:
+        hi, lo = pack_a_split(m, split)
+        parts.update(AHI=hi.ravel(), ALO=lo.ravel())
     offs, buf, pos = {}, [], 0
     for k, v in parts.items():
         offs[k] = pos
@@ -174,13 +216,18 @@ def constants(m, runtime_values):
 LDS_BYTES_PER_CU = 160 * 1024      # gfx950
 
 
-def lds_bytes(n, fp64, block, jmax):
+def lds_bytes(n, fp64, block, jmax, split=None):
     """Static LDS of the kernel (the ``__shared__`` arrays of ``kernels/hilbert_rk4.hip``)."""
     r = 8 if fp64 else 4
     waves, pts, tiles = block // 64, 32 // n, (2 * n + 31) // 32
     total = waves * 2 * pts * n * (n + 1) * r + waves * pts * n * r + n * n * r
     total += n * jmax * (r + 4) + waves * 64 * 8
-    total += (4 * n * n * 8 + waves * pts * 2 * n * n * 8) if fp64 else tiles * n * 64 * 4
+    if fp64:
+        total += 4 * n * n * 8 + waves * pts * 2 * n * n * 8
+    elif split:
+        total += 2 * tiles * (n // 8) * 64 * 8 * 2
+    else:
+        total += tiles * n * 64 * 4
     return total
 
 
@@ -191,8 +238,10 @@ def block_for(m, fp64, block_size=256):
     return 128 if (fp64 and m.n == 32) else block_size
 
 
-def emit(m, offs, mode, block_size=256, fp64=False):
-    """hiprtc source of the Hilbert-space RK4 kernel for model m."""
+def emit(m, offs, mode, block_size=256, fp64=False, split=None):
+    """hiprtc source of the Hilbert-space RK4 kernel for model m (``split``: the commutator
+    GEMM as three FP16 or BF16 products of hi/lo parts with FP32 accumulation)."""
+    split_kind = SPLIT_KINDS[split]
     atoms = {}
     names = {"ParX": "px", "ParY": "py"}
     split = [_split_time_atoms(sp.sympify(b), {TIME}, m.runtime_syms, atoms) for b in m.basis]
@@ -229,7 +278,8 @@ def emit(m, offs, mode, block_size=256, fp64=False):
         header = fh.read()
     defines = {"GQ_HILBERT": 1, "GQ_OUT_MEAN": int(mode == "mean"),
                "GQ_OUT_FINAL": int(mode == "final"), "GQ_OUT_FINAL_RHO": int(mode == "final_rho"),
-               "GQ_OUT_MEAN_STATE": 0, "GQ_TRACE": 0, "GQ_RHO0_VALUES": "GQ_RHO0_FROM_ARGS"}
+               "GQ_OUT_MEAN_STATE": 0, "GQ_TRACE": 0, "GQ_RHO0_VALUES": "GQ_RHO0_FROM_ARGS",
+               "GQ_MFMA_SPLIT": split_kind}
     parts = [header, codegen.prelude(fp64, True, block_size, defines), model_src,
              codegen.read_kernel_template("common.hip"), codegen.read_kernel_template("rk4_table.hip"),
              codegen.read_kernel_template("hilbert_rk4.hip")]
@@ -240,23 +290,25 @@ _SOURCES = {}
 
 
 def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps, warmup_steps,
-        block_size=256, collect=False, options=(), fp64=False):
+        block_size=256, collect=False, options=(), fp64=False, split=None):
     """Launch the Hilbert-space kernel; returns the same dict as ``device.run_sweep``."""
     import torch
     from .device import _axis, _initial_states
     setup_start = time.time()
     ctx = runtime.Context.get()
     dev = torch.device("cuda", ctx.device)
-    consts, offs = constants(m, runtime_values)
+    if split and fp64:
+        raise ValueError("split-precision matrix-core products are an FP32-mode option")
+    consts, offs = constants(m, runtime_values, split)
     uses_rho0 = rho0_values is not None
     block_size = block_for(m, fp64, block_size)
     tune = block_size == "auto"
     if tune:
         block_size = block_for(m, fp64, 256)
-    skey = (m.key, mode, block_size, uses_rho0, tuple(options), bool(fp64))
+    skey = (m.key, mode, block_size, uses_rho0, tuple(options), bool(fp64), split)
     src = _SOURCES.get(skey)
     if src is None:
-        src = emit(m, offs, mode, block_size, fp64).replace(
+        src = emit(m, offs, mode, block_size, fp64, split).replace(
             "GQ_RHO0_FROM_ARGS", "1" if uses_rho0 else "0")
         _SOURCES[skey] = src
     options = codegen.HIPRTC_BASE_OPTIONS + tuple(options)
@@ -302,7 +354,7 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
         trial_steps = min(int(num_steps), 64)
         jmax = jump_width(m)
         fits = tuple(b for b in autotune.CANDIDATES
-                     if lds_bytes(m.n, fp64, b, jmax) <= LDS_BYTES_PER_CU)
+                     if lds_bytes(m.n, fp64, b, jmax, split) <= LDS_BYTES_PER_CU)
         spare = torch.zeros(1, dtype=torch.int32, device=dev)
 
         def trial(mod, block):

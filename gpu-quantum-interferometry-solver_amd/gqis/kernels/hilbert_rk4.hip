@@ -21,12 +21,32 @@
 
 typedef float gq_f32x16 __attribute__((ext_vector_type(16)));
 
+// Split precision (GQ_MFMA_SPLIT 1: FP16, 2: BF16): the GEMM as Ahi*xhi + Ahi*xlo + Alo*xhi on
+// v_mfma_f32_32x32x16_{f16,bf16} with FP32 accumulation (the lo*lo term is dropped; FP16 parts
+// carry 11 + 11 significant bits, BF16 8 + 8).  K runs in blocks of 16: element j of lane
+// (c, h) in block b is the state value x[8b + j] that lane holds (Rbig row kappa(8b + j, h)),
+// and the A operand is packed on the host in the same order, so again no data movement.
+#ifndef GQ_MFMA_SPLIT
+#define GQ_MFMA_SPLIT 0
+#endif
+#if GQ_MFMA_SPLIT == 1
+typedef _Float16 gq_half;
+#define GQ_MFMA_HALF(a, b, c) __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0)
+#elif GQ_MFMA_SPLIT == 2
+typedef __bf16 gq_half;
+#define GQ_MFMA_HALF(a, b, c) __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0)
+#endif
+#if GQ_MFMA_SPLIT
+typedef gq_half gq_half8 __attribute__((ext_vector_type(8)));
+#endif
+
 #define HN GQ_HN                 // Hilbert dimension N (8, 16 or 32)
 #define HP (32 / HN)             // points per wavefront
 #define HT ((2 * HN + 31) / 32)  // 32x32 output tiles
 #define HM (HN / 2)              // complex elements per lane
 #define HWAVES (GQ_BLOCK / 64)
 #define HLD (HN + 1)             // padded LDS row
+#define HNB (HN / 8)             // K blocks of the split-precision products
 
 __device__ __forceinline__ int gq_row_of(int m, int h) { return 8 * (m >> 2) + 4 * h + (m & 3); }
 
@@ -34,6 +54,9 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4)
 #if GQ_FP64
     __shared__ real s_hbig[2 * HN * 2 * HN];
     __shared__ real s_x[HWAVES][HP * 2 * HN * HN];
+#elif GQ_MFMA_SPLIT
+    __shared__ gq_half8 s_ahi[HT * HNB * 64];
+    __shared__ gq_half8 s_alo[HT * HNB * 64];
 #else
     __shared__ float s_apack[HT * HN * 64];
 #endif
@@ -47,6 +70,12 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4)
     const real* __restrict__ cst = (const real*)a.consts;   // packed operator constants
 #if GQ_FP64
     for (int e = threadIdx.x; e < 4 * HN * HN; e += GQ_BLOCK) s_hbig[e] = cst[GQ_OFF_HBIG + e];
+#elif GQ_MFMA_SPLIT
+    for (int e = threadIdx.x; e < HT * HNB * 64 * 8; e += GQ_BLOCK) {
+        // the FP32 constants hold FP16 / BF16 values exactly
+        ((gq_half*)s_ahi)[e] = (gq_half)cst[GQ_OFF_AHI + e];
+        ((gq_half*)s_alo)[e] = (gq_half)cst[GQ_OFF_ALO + e];
+    }
 #else
     for (int e = threadIdx.x; e < HT * HN * 64; e += GQ_BLOCK) s_apack[e] = cst[GQ_OFF_APACK + e];
 #endif
@@ -150,6 +179,33 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4)
             X[m] = s;
         }
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+#elif GQ_MFMA_SPLIT
+        gq_f32x16 acc[HT];
+#pragma unroll
+        for (int t = 0; t < HT; ++t)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[t][e] = 0.0f;
+#pragma unroll
+        for (int b = 0; b < HNB; ++b) {
+            gq_half8 xh, xl;
+#pragma unroll
+            for (int jj = 0; jj < 8; ++jj) {
+                const float v = x[8 * b + jj];
+                const gq_half hv = (gq_half)v;
+                xh[jj] = hv;
+                xl[jj] = (gq_half)(v - (float)hv);
+            }
+#pragma unroll
+            for (int t = 0; t < HT; ++t) {
+                const gq_half8 ah = s_ahi[(t * HNB + b) * 64 + lane];
+                const gq_half8 al = s_alo[(t * HNB + b) * 64 + lane];
+                acc[t] = GQ_MFMA_HALF(al, xh, acc[t]);
+                acc[t] = GQ_MFMA_HALF(ah, xl, acc[t]);
+                acc[t] = GQ_MFMA_HALF(ah, xh, acc[t]);
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < HN; ++m) X[m] = acc[m >> 4][m & 15];
 #else
         gq_f32x16 acc[HT];
 #pragma unroll

@@ -110,14 +110,18 @@ class SweepServer:
         hm = self._hilbert_model(r) if spec.name == "rk4" else None
         if hm is not None:
             from . import hilbert
-            cvals, offs = hilbert.constants(hm, list(key[-1]))
+            from .solver import SPLIT_F16_MAX_ENTRY
+            # FP32 requests use the FP16 hi/lo products, as mesolve_2D does
+            split = ("f16" if not r.fp64 and np.max(np.abs(hm.h_eff)) <= SPLIT_F16_MAX_ENTRY
+                     else None)
+            cvals, offs = hilbert.constants(hm, list(key[-1]), split)
             block = hilbert.block_for(hm, r.fp64, 256 if self.block == "auto" else self.block)
-            src = hilbert.emit(hm, offs, r.output_mode, block, r.fp64).replace(
+            src = hilbert.emit(hm, offs, r.output_mode, block, r.fp64, split).replace(
                 "GQ_RHO0_FROM_ARGS", "0")
             kname = codegen.kernel_name("gq_hilbert_rk4", r.fp64)
             natoms = int(src.split("#define GQ_NATOMS ")[1].split()[0])
             cands = tuple((1, b) for b in autotune.CANDIDATES
-                          if hilbert.lds_bytes(hm.n, r.fp64, b, hilbert.jump_width(hm))
+                          if hilbert.lds_bytes(hm.n, r.fp64, b, hilbert.jump_width(hm), split)
                           <= hilbert.LDS_BYTES_PER_CU)
 
             def variant(c):

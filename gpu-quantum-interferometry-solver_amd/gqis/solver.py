@@ -241,7 +241,8 @@ def mesolve_2D(H, Drive=None, Col_Ops=None, mean_operator=None, tlist=None,
     ``block_size`` (threads per work-group, or ``"auto"``: measured once per kernel and device)
     ``lanes_per_point`` (1, 2, 4 or 8 lanes, one per wavefront, share each point's state rows;
     ``block_size="auto"`` also measures 2, 4, 8 for RK4 models of 24 or more states) and
-    ``liouville`` ("auto", "lanes" or "mfma": the
+    ``liouville`` ("auto", "lanes", "mfma", or "mfma_f16" / "mfma_bf16": the commutator product
+    as three FP16 / BF16 matrix-core products of hi/lo parts with FP32 accumulation; the
     Hilbert-space path for N = 8, 16, 32 -- matrix cores in FP32, VALU dot products in FP64 --
     used automatically from N = 16 in RK4 sweeps whose operators fit it).  ``return_device=True`` returns torch tensors on
     the GPU.  ``H`` may be a ``CompiledRHS`` from ``compile_lindblad_rhs``.
@@ -434,10 +435,13 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
     collect = bool(timings or return_timing_info)
     total_start = time.time()
 
-    if liouville not in ("auto", "lanes", "mfma"):
-        raise ValueError("liouville must be 'auto', 'lanes' or 'mfma'.")
+    if liouville not in ("auto", "lanes", "mfma", "mfma_f16", "mfma_bf16"):
+        raise ValueError("liouville must be 'auto', 'lanes', 'mfma', 'mfma_f16' or 'mfma_bf16'.")
+    if liouville in ("mfma_f16", "mfma_bf16") and fp64:
+        raise ValueError(f"liouville='{liouville}' (split-precision matrix-core products) is an "
+                         "FP32-mode option.")
     hilbert_out = None
-    if _prepare and liouville == "mfma":
+    if _prepare and liouville.startswith("mfma"):
         raise ValueError("prepare_sweep runs the per-point kernels; use liouville='auto' or "
                          "'lanes'.")
     if (compiled_rhs is None and _ode_size is None and liouville != "lanes" and not _prepare
@@ -587,13 +591,18 @@ def _finish(outputs, stage_s, rhs_status, spec, solver_frequency, unroll, collec
     return result
 
 
+SPLIT_F16_MAX_ENTRY = 1.0e4     # largest |H_eff| entry routed to FP16 hi/lo products (max 65504)
+
+
 def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, runtime_consts,
                  keep_symbolic_consts, auto_runtime_consts, rho0, rho0_values, spec, mode, fp64,
                  trace, liouville, dt, t_in, num_steps, warmup_steps, block_size, collect, options):
     """Run the matrix-core (Hilbert-space) path when it applies; None otherwise."""
     from . import hilbert
     n = H.shape[0] if hasattr(H, "shape") else 0
-    wanted = liouville == "mfma" or (liouville == "auto" and n >= 16)
+    forced = liouville.startswith("mfma")
+    split = {"mfma_f16": "f16", "mfma_bf16": "bf16"}.get(liouville)
+    wanted = forced or (liouville == "auto" and n >= 16)
     reasons = []
     if n not in hilbert.SUPPORTED_N:
         reasons.append(f"N = {n} (supported: {hilbert.SUPPORTED_N})")
@@ -606,8 +615,8 @@ def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, run
     if not wanted:
         return None
     if reasons:
-        if liouville == "mfma":
-            raise ValueError("liouville='mfma' does not apply: " + ", ".join(reasons))
+        if forced:
+            raise ValueError(f"liouville='{liouville}' does not apply: " + ", ".join(reasons))
         return None
     start = time.time()
     model = _prepare_model(H, Drive, Col_Ops, mean_operator, sweep_symbols=tuple(var_arrays),
@@ -616,19 +625,23 @@ def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, run
                            auto_runtime_consts=auto_runtime_consts, rho0=rho0, ode_size=None)
     hm = hilbert.extract(model, _drive_map(model["Drive"]))
     if hm is None:
-        if liouville == "mfma":
-            raise ValueError("liouville='mfma' needs H = constant + diagonal parameter/time "
+        if forced:
+            raise ValueError(f"liouville='{liouville}' needs H = constant + diagonal parameter/time "
                              "terms, constant diagonal or single-entry collapse operators and a "
                              "diagonal observable.")
         return None
     values = dict(model["defaults"])
     values.update(runtime_consts or {})
     rt = [float(values[s]) for s in model["runtime_const_syms"]]
+    if liouville == "auto" and not fp64 and np.max(np.abs(hm.h_eff)) <= SPLIT_F16_MAX_ENTRY:
+        # FP32 mode: FP16 hi/lo products reach the FP32 product's error at 1.4x its speed;
+        # the constant operator must stay well inside the FP16 range
+        split = "f16"
     prep = time.time() - start
     outputs = hilbert.run(hm, mode=mode, var_arrays=var_arrays, rho0_values=rho0_values,
                           runtime_values=rt, dt=dt, t_in=t_in, num_steps=num_steps,
                           warmup_steps=warmup_steps, block_size=block_size, collect=collect,
-                          options=options, fp64=fp64)
+                          options=options, fp64=fp64, split=split)
     stage_s = {"ode_prepare_s": prep, "ode_optimize_s": 0.0, "emit_s": 0.0}
     return outputs, stage_s, "miss"
 

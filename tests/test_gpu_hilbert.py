@@ -9,15 +9,18 @@ pytestmark = pytest.mark.gpu
 SEED = 2024
 
 
+@pytest.mark.parametrize("liouville", ["mfma", "mfma_f16"])
 @pytest.mark.parametrize("n,steps", [(8, 400), (16, 200), (32, 100)])
-def test_random_n_level_mean_and_state_match_oracle(gpu, n, steps):
+def test_random_n_level_mean_and_state_match_oracle(gpu, n, steps, liouville):
+    # mfma_f16: the commutator product as FP16 hi/lo parts (three products, FP32 accumulation)
     w = workloads.random_n_level(n, seed=SEED, n_eps=3, n_amp=2, n_t=steps + 1,
                                  t_end=0.01 * steps)
     eps_v, amp_v = w["var_arrays"].values()
     got, info = mesolve_2D(w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"],
-                           var_arrays=w["var_arrays"], liouville="mfma", return_timing_info=True)
+                           var_arrays=w["var_arrays"], liouville=liouville,
+                           return_timing_info=True)
     rho = mesolve_2D(w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"],
-                     var_arrays=w["var_arrays"], liouville="mfma", output_mode="final_rho")
+                     var_arrays=w["var_arrays"], liouville=liouville, output_mode="final_rho")
     m = models.random_n_level(n, SEED, eps_v, amp_v)
     ref, _ = integrate.rk4(m.f, m.ground_state(), 0.01, steps, obs=m.obs)
     fin, _ = integrate.rk4(m.f, m.ground_state(), 0.01, steps, mode="final_rho")
@@ -63,3 +66,30 @@ def test_fp64_hilbert_path_parity(gpu, n, steps):
     ref, _ = integrate.rk4(m.f, m.ground_state(), 0.01, steps, obs=m.obs)
     assert got.dtype == np.complex128
     assert np.max(np.abs(got.ravel() - ref)) <= 1e-10 * max(1.0, np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("n", [8, 16, 32])
+def test_split_precision_products_over_2000_steps(gpu, n):
+    # the random n-level scaling model over its full 2000 steps: FP16 hi/lo products stay at
+    # the FP32 path's error level (<= 5e-7 of the map's scale measured), BF16 hi/lo (8 + 8
+    # significant bits) at a few 1e-6; both against the FP64 path on the same 8 x 8 points
+    w = workloads.random_n_level(n, seed=SEED, n_eps=8, n_amp=8)
+    args = (w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"])
+    ref = mesolve_2D(*args, var_arrays=w["var_arrays"], fp64=True,
+                     liouville="lanes" if n == 8 else "mfma")
+    scale = float(np.max(np.abs(ref)))
+    for lv, tol in (("mfma", 2e-6), ("mfma_f16", 2e-6), ("mfma_bf16", 5e-5)):
+        got = mesolve_2D(*args, var_arrays=w["var_arrays"], liouville=lv)
+        err = float(np.max(np.abs(got - ref)))
+        print(n, lv, err / scale, flush=True)
+        assert err <= tol * scale
+
+
+def test_auto_uses_fp16_split_products_from_16_levels(gpu):
+    w = workloads.random_n_level(16, seed=SEED, n_eps=4, n_amp=4, n_t=101, t_end=1.0)
+    args = (w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"])
+    auto = mesolve_2D(*args, var_arrays=w["var_arrays"])
+    f16 = mesolve_2D(*args, var_arrays=w["var_arrays"], liouville="mfma_f16")
+    np.testing.assert_array_equal(auto, f16)
+    with pytest.raises(ValueError, match="FP32-mode option"):
+        mesolve_2D(*args, var_arrays=w["var_arrays"], liouville="mfma_f16", fp64=True)
