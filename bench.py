@@ -306,8 +306,8 @@ def bench_duffing(args):
 def bench_scaling(args):
     """Hilbert-space scaling: seeded random N-level models (N = 4, 8, 16, 32), 256 x 256 grid,
     RK4 2000 steps (dt = 0.01).  N = 4 runs the per-point register kernel; N = 8, 16, 32 the
-    matrix-core path (FP32 MFMA, and FP16 / BF16 hi/lo split products), N = 8 also the
-    register kernel.  Errors are against the FP64
+    matrix-core path (FP32 MFMA, and FP16 / BF16 hi/lo split products), N = 8 and 16 also
+    the per-point kernels, with one lane or 4 / 8 lanes (wavefronts) per point.  Errors are against the FP64
     GPU paths (per-point FP64 for N = 4 and 8, FP64 Hilbert-space path for N = 16, 32) on a
     16 x 16 subset of the grid."""
     import numpy as np
@@ -318,8 +318,10 @@ def bench_scaling(args):
            "unit": "param-point RK steps/s", "n_gpus": 1, "data": "synthetic",
            "config": {"workload": f"random N-level, {side}x{side}, RK4 2000 steps dt 0.01",
                       "points": side * side}, "results": {}}
-    for n, forms in ((4, ["lanes"]), (8, ["lanes", "mfma", "mfma_f16", "lanes64", "mfma64"]),
-                     (16, ["mfma", "mfma_f16", "mfma_bf16"]),
+    for n, forms in ((4, ["lanes"]),
+                     (8, ["lanes", "lanes_g4", "lanes_g8", "mfma", "mfma_f16", "lanes64",
+                          "lanes_g8_64", "mfma64"]),
+                     (16, ["lanes_g8", "mfma", "mfma_f16", "mfma_bf16"]),
                      (32, ["mfma", "mfma_f16", "mfma_bf16"])):
         w = workloads.random_n_level(n, n_eps=side, n_amp=side)
         sub = {k: v[:: side // 16] for k, v in w["var_arrays"].items()}
@@ -329,13 +331,19 @@ def bench_scaling(args):
         for form in forms:
             fp64 = form.endswith("64")
             lv = form[:-2] if fp64 else form
+            lv = lv.rstrip("_")
+            groups = 1
+            if lv.startswith("lanes_g"):
+                lv, groups = "lanes", int(lv[len("lanes_g"):])
             kw = dict(var_arrays=w["var_arrays"], liouville=lv, return_device=True,
-                      return_timing_info=True, ignore_non_finite_output=True, fp64=fp64)
+                      return_timing_info=True, ignore_non_finite_output=True, fp64=fp64,
+                      lanes_per_point=groups)
             log(f"scaling N={n} {form} ...")
             mesolve_2D(*args_m, **kw)
             _, info = mesolve_2D(*args_m, **kw)
             torch.cuda.synchronize()
-            sub_res = mesolve_2D(*args_m, var_arrays=sub, liouville=lv, fp64=fp64)
+            sub_res = mesolve_2D(*args_m, var_arrays=sub, liouville=lv, fp64=fp64,
+                                 lanes_per_point=groups)
             k = info["gpu_kernel_s"]
             ps = side * side * 2000
             d2 = n * n

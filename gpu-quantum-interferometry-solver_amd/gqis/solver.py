@@ -231,7 +231,7 @@ def mesolve_2D(H, Drive=None, Col_Ops=None, mean_operator=None, tlist=None,
                t_in=0.0, return_device=False, calibrate_timesteps=None, consts=None, initial=None,
                output=None, solver_options=None, codegen=None, diagnostics=None,
                hiprtc_options=(), time_table=True, block_size=256, liouville="auto",
-               lanes_per_point=1):
+               lanes_per_point="auto"):
     """Lindblad master-equation sweep over one or two parameter axes on the GPU.
 
     Same signature, defaults, return shapes and errors as the reference ``mesolve_2D``
@@ -240,6 +240,7 @@ def mesolve_2D(H, Drive=None, Col_Ops=None, mean_operator=None, tlist=None,
     terms for fixed-step solvers; False evaluates them per point as the reference does) and
     ``block_size`` (threads per work-group, or ``"auto"``: measured once per kernel and device)
     ``lanes_per_point`` (1, 2, 4 or 8 lanes, one per wavefront, share each point's state rows;
+    "auto": 8 for states of 48 or more reals where the grouped kernel applies, else 1;
     ``block_size="auto"`` also measures 2, 4, 8 for RK4 models of 24 or more states) and
     ``liouville`` ("auto", "lanes", "mfma", or "mfma_f16" / "mfma_bf16": the commutator product
     as three FP16 / BF16 matrix-core products of hi/lo parts with FP32 accumulation; the
@@ -331,7 +332,7 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
                            beep_on_error=False, ignore_non_finite_output=False, t_in=0.0,
                            return_device=False, calibrate_timesteps=None, compiled_rhs=None,
                            hiprtc_options=(), time_table=True, block_size=256,
-                           liouville="auto", lanes_per_point=1, _ode_size=None,
+                           liouville="auto", lanes_per_point="auto", _ode_size=None,
                            _state_trace=False, _prepare=False,
                            _time_trace_offset=0):
     if calibrate_timesteps is not None and calibrate_timesteps is not False:
@@ -357,9 +358,10 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
         block_size = int(block_size)
         if block_size not in (64, 128, 256, 512, 1024):
             raise ValueError("block_size must be 64, 128, 256, 512, 1024 or 'auto'.")
-    lanes_per_point = int(lanes_per_point)
-    if lanes_per_point not in (1, 2, 4, 8):
-        raise ValueError("lanes_per_point must be 1, 2, 4 or 8.")
+    if lanes_per_point != "auto":
+        lanes_per_point = int(lanes_per_point)
+        if lanes_per_point not in (1, 2, 4, 8):
+            raise ValueError("lanes_per_point must be 1, 2, 4, 8 or 'auto'.")
 
     var_arrays = {} if var_arrays is None else dict(var_arrays)
     for sym, arr in ({} if rho0_var_arrays is None else dict(rho0_var_arrays)).items():
@@ -445,7 +447,8 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
         raise ValueError("prepare_sweep runs the per-point kernels; use liouville='auto' or "
                          "'lanes'.")
     if (compiled_rhs is None and _ode_size is None and liouville != "lanes" and not _prepare
-            and not (lanes_per_point > 1 and liouville == "auto")):
+            and not (lanes_per_point != "auto" and lanes_per_point > 1
+                     and liouville == "auto")):
         hilbert_out = _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values,
                                    runtime_consts, keep_symbolic_consts, auto_runtime_consts,
                                    rho0, rho0_values, spec, mode, fp64, return_time_trace,
@@ -490,6 +493,13 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
                 _KERNEL_CACHE[key] = src
         return src
 
+    if lanes_per_point == "auto":
+        # measured on gfx950 at 63 states: 8 lanes per point 1.2x (RK4 FP32), 1.85x (RK4
+        # FP64) and 8.7x (Tsit5A FP64) faster than one lane, which spills; with
+        # block_size="auto" the tuner measures every grouping instead
+        lanes_per_point = (8 if block_size != "auto"
+                           and compiled.state_size >= GROUPED_AUTO_MIN_STATES
+                           and grouped_applies(spec, compiled.state_size, fp64, 8) else 1)
     if lanes_per_point > 1 and not grouped_applies(spec, compiled.state_size, fp64,
                                                    lanes_per_point):
         raise ValueError(f"lanes_per_point={lanes_per_point} needs the rk4, tsit5a or dop853a "
@@ -666,6 +676,8 @@ def _kernel_variant(compiled, spec, mode, uses_rho0_values, trace, state_trace, 
 
 LDS_BYTES_PER_CU = 160 * 1024      # gfx950
 
+
+GROUPED_AUTO_MIN_STATES = 48
 
 GROUPED_TEMPLATES = {"rk4": "grouped_rk4.hip", "tsit5a": "grouped_adaptive.hip",
                      "dop853a": "grouped_adaptive.hip"}

@@ -14,7 +14,7 @@ def test_auto_block_matches_fixed_block(gpu, solver):
     w = workloads.chain(8, n_eps=256, n_amp=64, n_t=401, t_end=20.0)
     args = (w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"])
     fixed = mesolve_2D(*args, var_arrays=w["var_arrays"], fp64=True, solver=solver,
-                       liouville="lanes", block_size=256)
+                       liouville="lanes", block_size=256, lanes_per_point=1)
     got, info = mesolve_2D(*args, var_arrays=w["var_arrays"], fp64=True, solver=solver,
                            liouville="lanes", block_size="auto", return_timing_info=True)
     print(solver, "chose", info["lanes_per_point"], info["block_size"], info.get("autotune"),

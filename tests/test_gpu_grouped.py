@@ -25,7 +25,8 @@ def _rel(got, ref):
 @pytest.mark.parametrize("groups", [2, 4, 8])
 def test_grouped_matches_per_lane_fp64(gpu, mode, groups):
     args, var = _chain8()
-    ref = mesolve_2D(*args, var_arrays=var, fp64=True, output_mode=mode, liouville="lanes")
+    ref = mesolve_2D(*args, var_arrays=var, fp64=True, output_mode=mode, liouville="lanes",
+                     lanes_per_point=1)
     got = mesolve_2D(*args, var_arrays=var, fp64=True, output_mode=mode, liouville="lanes",
                      lanes_per_point=groups)
     print(mode, groups, "bitwise" if np.array_equal(got, ref) else _rel(got, ref), flush=True)
@@ -37,7 +38,8 @@ def test_grouped_traces_and_state_mean(gpu):
     args, var = _chain8(n_eps=9, n_amp=7)
     for kw in ({"output_mode": "mean", "return_time_trace": True, "time_trace_every": 7},
                {"output_mode": "final_rho", "return_time_trace": True, "time_trace_every": 50}):
-        ref = mesolve_2D(*args, var_arrays=var, fp64=True, liouville="lanes", **kw)
+        ref = mesolve_2D(*args, var_arrays=var, fp64=True, liouville="lanes", lanes_per_point=1,
+                         **kw)
         got = mesolve_2D(*args, var_arrays=var, fp64=True, liouville="lanes",
                          lanes_per_point=4, **kw)
         for a, b in zip(got[:2], ref[:2]):
@@ -47,7 +49,7 @@ def test_grouped_traces_and_state_mean(gpu):
 
 def test_grouped_fp32_matches_per_lane(gpu):
     args, var = _chain8()
-    ref = mesolve_2D(*args, var_arrays=var, fp64=False, liouville="lanes")
+    ref = mesolve_2D(*args, var_arrays=var, fp64=False, liouville="lanes", lanes_per_point=1)
     got = mesolve_2D(*args, var_arrays=var, fp64=False, liouville="lanes", lanes_per_point=4)
     assert _rel(got, ref) <= 1e-5
 
@@ -80,7 +82,7 @@ def test_grouped_continuation_and_ragged_grid(gpu):
 def test_autotuner_considers_grouped_kernels(gpu):
     autotune.clear_memory()
     args, var = _chain8(n_eps=128, n_amp=128, n_t=201, t_end=10.0)
-    ref = mesolve_2D(*args, var_arrays=var, fp64=True, liouville="lanes")
+    ref = mesolve_2D(*args, var_arrays=var, fp64=True, liouville="lanes", lanes_per_point=1)
     got, info = mesolve_2D(*args, var_arrays=var, fp64=True, liouville="lanes",
                            block_size="auto", return_timing_info=True)
     print("autotune", info["lanes_per_point"], info["block_size"], info.get("autotune"),
@@ -102,7 +104,7 @@ def test_grouped_adaptive_matches_per_lane(gpu, solver, groups):
     kw = dict(var_arrays=var, fp64=True, solver=solver, liouville="lanes",
               adaptive_options=ADAPT, return_timing_info=True)
     for mode in ("mean", "final_rho"):
-        ref, ri = mesolve_2D(*args, output_mode=mode, **kw)
+        ref, ri = mesolve_2D(*args, output_mode=mode, lanes_per_point=1, **kw)
         got, gi = mesolve_2D(*args, output_mode=mode, lanes_per_point=groups, **kw)
         same = np.mean(ri["adaptive_steps"]["accepted"] == gi["adaptive_steps"]["accepted"])
         print(solver, groups, mode, _rel(got, ref), "same counts", same, flush=True)
@@ -131,7 +133,7 @@ def test_grouped_adaptive_traces_and_overflow(gpu):
               adaptive_options=ADAPT)
     for mode, every in (("mean", 3), ("final_rho", 10)):
         ref = mesolve_2D(*args, output_mode=mode, return_time_trace=True,
-                         time_trace_every=every, **kw)
+                         time_trace_every=every, lanes_per_point=1, **kw)
         got = mesolve_2D(*args, output_mode=mode, return_time_trace=True,
                          time_trace_every=every, lanes_per_point=4, **kw)
         for a, b in zip(got[:2], ref[:2]):
@@ -149,9 +151,20 @@ def test_autotuner_considers_grouped_adaptive_kernels(gpu):
     args, var = _chain8(n_eps=128, n_amp=128, n_t=101, t_end=5.0)
     kw = dict(var_arrays=var, fp64=True, solver="tsit5a", liouville="lanes",
               adaptive_options={"step_rtol": 1e-8, "step_atol": 1e-10, "max_substeps": 2000})
-    ref = mesolve_2D(*args, **kw)
+    ref = mesolve_2D(*args, lanes_per_point=1, **kw)
     got, info = mesolve_2D(*args, block_size="auto", return_timing_info=True, **kw)
     print("autotune adaptive", info["lanes_per_point"], info["block_size"],
           info.get("autotune"), flush=True)
     assert {c[0] for c in info["autotune"]} == {1, 2, 4, 8}
     assert _rel(got, ref) <= 1e-7
+
+
+def test_auto_lanes_per_point_default(gpu):
+    # 63 states: the default takes 8 lanes per point; 15 states stay on one lane
+    args, var = _chain8(n_eps=8, n_amp=8, n_t=101, t_end=5.0)
+    _, info = mesolve_2D(*args, var_arrays=var, fp64=True, return_timing_info=True)
+    assert info["lanes_per_point"] == 8
+    w = workloads.four_level(n_eps=8, n_amp=8, n_t=101, t_end=5.0)
+    _, info = mesolve_2D(w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"],
+                         var_arrays=w["var_arrays"], fp64=True, return_timing_info=True)
+    assert info["lanes_per_point"] == 1
