@@ -155,6 +155,12 @@ def _round_half(x, kind):
     return rounded.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+def kernel_base(split=None):
+    """Kernel name (before the precision suffix) of the FP32 / split-precision forms."""
+    return {None: "gq_hilbert_rk4", "f16": "gq_hilbert_rk4_f16x3",
+            "bf16": "gq_hilbert_rk4_bf16x3"}[split]
+
+
 def pack_a_split(m, kind):
     """A operand of the split-precision kernel, hi and lo parts: [tile][K block][lane][8],
     element j of lane (r, h) in block b holding Hbig[32 t + r][kappa(8 b + j, h)] with
@@ -342,7 +348,7 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
     a.time_table = table.data_ptr()
     a.result, a.status = result.data_ptr(), status.data_ptr()
     stream = torch.cuda.current_stream(dev).cuda_stream
-    kname = codegen.kernel_name("gq_hilbert_rk4", fp64)
+    kname = codegen.kernel_name(kernel_base(split), fp64)
     tune_s, tune_times = 0.0, {}
     if tune and npts > 0 and num_steps > 0:
         from . import autotune

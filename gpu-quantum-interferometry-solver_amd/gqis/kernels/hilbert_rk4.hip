@@ -50,7 +50,15 @@ typedef gq_half gq_half8 __attribute__((ext_vector_type(8)));
 
 __device__ __forceinline__ int gq_row_of(int m, int h) { return 8 * (m >> 2) + 4 * h + (m & 3); }
 
-extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4)(const gq_sweep_args a) {
+// the split-precision forms carry their product type in the name (gq_hilbert_rk4_f16x3_f32)
+#if GQ_MFMA_SPLIT == 1
+#define GQ_HILBERT_NAME gq_hilbert_rk4_f16x3
+#elif GQ_MFMA_SPLIT == 2
+#define GQ_HILBERT_NAME gq_hilbert_rk4_bf16x3
+#else
+#define GQ_HILBERT_NAME gq_hilbert_rk4
+#endif
+extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME)(const gq_sweep_args a) {
 #if GQ_FP64
     __shared__ real s_hbig[2 * HN * 2 * HN];
     __shared__ real s_x[HWAVES][HP * 2 * HN * HN];

@@ -118,7 +118,7 @@ class SweepServer:
             block = hilbert.block_for(hm, r.fp64, 256 if self.block == "auto" else self.block)
             src = hilbert.emit(hm, offs, r.output_mode, block, r.fp64, split).replace(
                 "GQ_RHO0_FROM_ARGS", "0")
-            kname = codegen.kernel_name("gq_hilbert_rk4", r.fp64)
+            kname = codegen.kernel_name(hilbert.kernel_base(split), r.fp64)
             natoms = int(src.split("#define GQ_NATOMS ")[1].split()[0])
             cands = tuple((1, b) for b in autotune.CANDIDATES
                           if hilbert.lds_bytes(hm.n, r.fp64, b, hilbert.jump_width(hm), split)
