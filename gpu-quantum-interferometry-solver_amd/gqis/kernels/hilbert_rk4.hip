@@ -150,6 +150,25 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
 #pragma unroll
     for (int m = 0; m < HM; ++m) odiag[m] = cst[GQ_OFF_OBS + gq_row_of(m, h)];
 
+    // Per-lane constants of the dissipator, kept in registers: the dephasing weights of the
+    // lane's elements and, for the lane holding its column's diagonal element (m = mdiag), that
+    // population's jump list (zero weights elsewhere), so the RHS has no per-element branch.
+    real depw[HM];
+    int mdiag = -1;
+#pragma unroll
+    for (int m = 0; m < HM; ++m) {
+        const int i = gq_row_of(m, h);
+        depw[m] = s_dep[i * HN + j];
+        if (i == j) mdiag = m;
+    }
+    real jwl[GQ_JMAX];
+    int jbl[GQ_JMAX];
+#pragma unroll
+    for (int e = 0; e < GQ_JMAX; ++e) {
+        jwl[e] = (mdiag >= 0) ? s_jw[j * GQ_JMAX + e] : R(0.0);
+        jbl[e] = (mdiag >= 0) ? s_jb[j * GQ_JMAX + e] : 0;
+    }
+
     const real dt = (real)a.dt;
     const real dt2 = dt * R(0.5);
     const real dt6 = dt / R(6.0);
@@ -232,6 +251,7 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
         for (int m = 0; m < HN; ++m) X[m] = acc[m >> 4][m & 15];
 #endif
         real yr[HM], yi[HM];
+        real xd = R(0.0);   // the lane's diagonal element (population), if it holds one
 #pragma unroll
         for (int m = 0; m < HM; ++m) {
             const real xr = X[m];
@@ -242,25 +262,21 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
             const int i = gq_row_of(m, h);
             ybuf_r[i * HLD + j] = yr[m];
             ybuf_i[i * HLD + j] = yi[m];
-            if (i == j) dbuf[i] = x[m];
+            xd = (m == mdiag) ? x[m] : xd;
         }
+        if (mdiag >= 0) dbuf[j] = xd;
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        real jsum = R(0.0);
+#pragma unroll
+        for (int e = 0; e < GQ_JMAX; ++e) jsum = fma(jwl[e], dbuf[jbl[e]], jsum);
 #pragma unroll
         for (int m = 0; m < HM; ++m) {
             const int i = gq_row_of(m, h);
-            const real dep = s_dep[i * HN + j];
             real dr = yr[m] + ybuf_r[j * HLD + i];
             real di = yi[m] - ybuf_i[j * HLD + i];
-            dr = fma(dep, x[m], dr);
-            di = fma(dep, x[m + HM], di);
-            if (i == j) {
-                real sum = R(0.0);
-#pragma unroll
-                for (int e = 0; e < GQ_JMAX; ++e)
-                    sum = fma(s_jw[i * GQ_JMAX + e], dbuf[s_jb[i * GQ_JMAX + e]], sum);
-                dr += sum;
-            }
-            k[m] = dr;
+            dr = fma(depw[m], x[m], dr);
+            di = fma(depw[m], x[m + HM], di);
+            k[m] = (m == mdiag) ? dr + jsum : dr;
             k[m + HM] = di;
         }
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
