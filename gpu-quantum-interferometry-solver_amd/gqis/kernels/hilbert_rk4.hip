@@ -161,6 +161,9 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
         depw[m] = s_dep[i * HN + j];
         if (i == j) mdiag = m;
     }
+    real od = R(0.0);   // observable weight of that diagonal element (0 without one)
+#pragma unroll
+    for (int m = 0; m < HM; ++m) od = (m == mdiag) ? odiag[m] : od;
     real jwl[GQ_JMAX];
     int jbl[GQ_JMAX];
 #pragma unroll
@@ -316,9 +319,11 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
         if (step + 1 < num_steps) rhs(y, k);
 #if GQ_OUT_MEAN
         if (step >= a.warmup_steps) {
+            // the lane's diagonal element (if any) weighted by the observable's diagonal
+            real yd = R(0.0);
 #pragma unroll
-            for (int m = 0; m < HM; ++m)
-                if (gq_row_of(m, h) == j) avg += (double)(odiag[m] * y[m]);
+            for (int m = 0; m < HM; ++m) yd = (m == mdiag) ? y[m] : yd;
+            avg += (double)(od * yd);
         }
 #endif
     }
@@ -326,10 +331,12 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
     bool bad = false;
 #if GQ_OUT_MEAN || GQ_OUT_FINAL
 #if GQ_OUT_FINAL
-    avg = 0.0;
+    {
+        real yd = R(0.0);
 #pragma unroll
-    for (int m = 0; m < HM; ++m)
-        if (gq_row_of(m, h) == j) avg += (double)(odiag[m] * y[m]);
+        for (int m = 0; m < HM; ++m) yd = (m == mdiag) ? y[m] : yd;
+        avg = (double)(od * yd);
+    }
 #endif
     // sum the partial traces of the 2N lanes of each point
     s_red[wave][lane] = avg;
