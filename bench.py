@@ -115,9 +115,11 @@ def main():
                     help="run only `--steps` sweeps of one form (for rocprofv3), no JSON line")
     ap.add_argument("--requests", type=int, default=1000, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="serving: Poisson rate (req/s)")
-    ap.add_argument("--workload", choices=["four_level", "duffing", "serving", "scaling"],
+    ap.add_argument("--workload", choices=["four_level", "lzs", "duffing", "serving", "scaling"],
                     default="four_level",
-                    help="four_level: the headline sweep; duffing: adaptive Tsit5/DOP853 sweep")
+                    help="four_level: the headline sweep; lzs: the 64x64 two-level map with the "
+                         "CPU oracle beside it; duffing: adaptive Tsit5/DOP853 sweep; serving: "
+                         "Poisson request stream; scaling: Hilbert-space N-level models")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="with --profile: also run one device copy of this many bytes "
                          "(the counter calibration kernel)")
@@ -136,6 +138,8 @@ def main():
         return bench_serving(args)
     if args.workload == "scaling":
         return bench_scaling(args)
+    if args.workload == "lzs":
+        return bench_lzs(args)
 
     w = workloads.four_level(n_eps=args.grid, n_amp=args.grid)
     npts = args.grid * args.grid
@@ -359,6 +363,44 @@ def bench_scaling(args):
                     # three half-precision products per FP32-equivalent product
                     entry["half_mfma_issued_frac"] = 3 * ach / 2500.0
             out["results"][f"N={n} {form}"] = entry
+    print(json.dumps(out), flush=True)
+
+
+def bench_lzs(args):
+    """The two-level LZS map (64 x 64, t in [0, 200] with 2001 samples, RK4, FP64, mean of sz)
+    on the GPU and, on the host's cores, the FP64 C oracle over the same full grid; reports both
+    rates and the GPU map's deviation from the oracle's."""
+    import numpy as np
+    import torch
+    from gqis import mesolve_2D, workloads
+    from oracle import cbridge, models
+    w = workloads.lzs_two_level()
+    args_m = (w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"])
+    got = mesolve_2D(*args_m, var_arrays=w["var_arrays"], fp64=True)
+    times = []
+    for _ in range(max(3, args.steps)):
+        _, info = mesolve_2D(*args_m, var_arrays=w["var_arrays"], fp64=True, return_device=True,
+                             return_timing_info=True)
+        times.append(info["gpu_kernel_s"])
+    torch.cuda.synchronize()
+    eps, amp = w["var_arrays"].values()
+    m = models.lzs_two_level(eps, amp)
+    threads = min(len(os.sched_getaffinity(0)), 16)
+    t0 = time.perf_counter()
+    ref = cbridge.rk4_affine(m, 0.1, 2000, threads=threads)
+    cpu_s = time.perf_counter() - t0
+    point_steps = 64 * 64 * 2000
+    k = float(np.median(times))
+    out = {"metric": "param-point RK steps/s (two-level LZS map)", "unit": "param-point RK steps/s",
+           "value": point_steps / k, "n_gpus": 1, "higher_is_better": True, "data": "synthetic",
+           "dtype": "f64", "vs_baseline": None,
+           "config": {"workload": "two-level LZS 64x64, RK4 2000 steps dt 0.1, mean of sz",
+                      "points": 64 * 64},
+           "kernel_s": k, "max_abs_err_vs_cpu_oracle": float(np.max(np.abs(got.ravel() - ref))),
+           "cpu_baseline": {"value": point_steps / cpu_s, "unit": "param-point RK steps/s",
+                            "cores": threads, "kind": "port",
+                            "sample": f"full 64x64 grid, 2000 RK4 steps, FP64 C oracle "
+                                      f"(OpenMP, {threads} threads), {cpu_s:.3f} s"}}
     print(json.dumps(out), flush=True)
 
 
