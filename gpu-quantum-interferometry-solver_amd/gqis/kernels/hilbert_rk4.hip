@@ -200,13 +200,22 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
 #pragma unroll
         for (int m = 0; m < HN; ++m) xcol[gq_row_of(m, h) * HN + j] = x[m];
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        // rows in blocks of 8 with k inside: each state value read from LDS serves 8 rows of
+        // the lane (the Hbig entries are the same for the 32 lanes of a half: broadcast reads)
 #pragma unroll
-        for (int m = 0; m < HN; ++m) {
-            const real* __restrict__ hrow = s_hbig + gq_row_of(m, h) * 2 * HN;
-            real s = R(0.0);
-#pragma unroll 4
-            for (int kk = 0; kk < 2 * HN; ++kk) s = fma(hrow[kk], xcol[kk * HN + j], s);
-            X[m] = s;
+        for (int mb = 0; mb < HN; mb += 8) {
+            real xs[8];
+#pragma unroll
+            for (int m = 0; m < 8; ++m) xs[m] = R(0.0);
+#pragma unroll 2
+            for (int kk = 0; kk < 2 * HN; ++kk) {
+                const real xv = xcol[kk * HN + j];
+#pragma unroll
+                for (int m = 0; m < 8; ++m)
+                    xs[m] = fma(s_hbig[gq_row_of(mb + m, h) * 2 * HN + kk], xv, xs[m]);
+            }
+#pragma unroll
+            for (int m = 0; m < 8; ++m) X[mb + m] = xs[m];
         }
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
 #elif GQ_MFMA_SPLIT
