@@ -75,3 +75,95 @@ def test_tsit5a_calibration_meets_requested_accuracy(gpu):
     err = np.abs(np.asarray(got) - np.asarray(fine))
     assert float(np.sqrt(np.mean(err ** 2))) <= MARGIN * lim["rms_atol"]
     assert float(err.max()) <= MARGIN * lim["max_atol"]
+
+
+# ---- every output mode, traces and ODE outputs (the behaviours the reference checks on its
+# GPU, tests/test_gpu_solver.py:131-266): a coarse start refined until the map meets the limits
+
+EPS_Q, AMP_Q, DRV_Q, DELTA_Q = sp.symbols("eps A Drive Delta", real=True)
+OMEGA_Q = 1.14
+PERIOD_Q = 2.0 * np.pi / OMEGA_Q
+LIMITS = dict(rms_atol=1e-3, max_atol=1e-2, sample_stride=2)
+
+
+def _qubit(steps_per_period, periods=10, **kw):
+    """Driven dissipative qubit on a 16 x 16 (eps, A) grid."""
+    t = sp.Symbol("t", real=True)
+    sz, sx, sm = sp.Matrix([[1, 0], [0, -1]]), sp.Matrix([[0, 1], [1, 0]]), sp.Matrix([[0, 1], [0, 0]])
+    g1 = 0.05 / PERIOD_Q
+    g2 = g1 / 2.0 + 0.04 / PERIOD_Q
+    return mesolve_2D(0.5 * DELTA_Q * sx + 0.5 * DRV_Q * sz,
+                      {DRV_Q: EPS_Q + AMP_Q * sp.sin(OMEGA_Q * t)},
+                      [sp.sqrt(g1) * sm, sp.sqrt(g2) * sz], sp.diag(0, 1),
+                      np.linspace(0.0, periods * PERIOD_Q, periods * steps_per_period + 1),
+                      var_arrays={EPS_Q: np.linspace(-6 * OMEGA_Q, 6 * OMEGA_Q, 16),
+                                  AMP_Q: np.linspace(0.0, 6 * OMEGA_Q, 16)},
+                      const_values={DELTA_Q: 1.0}, **kw)
+
+
+def _within(got, ref, rms, mx):
+    err = np.abs(np.asarray(got) - np.asarray(ref))
+    assert float(np.sqrt(np.mean(err ** 2))) <= rms and float(err.max()) <= mx
+
+
+@pytest.mark.parametrize("solver", ["rk4", "tsit5a"])
+@pytest.mark.parametrize("mode", ["mean", "final", "final_rho"])
+def test_calibration_meets_limits_for_every_output(gpu, solver, mode):
+    core._TIMESTEP_CALIBRATION_CACHE.clear()
+    ref = _qubit(2048, output_mode=mode)
+    got = _qubit(8, output_mode=mode, solver=solver, calibrate_timesteps=LIMITS)
+    _within(got, ref, LIMITS["rms_atol"], LIMITS["max_atol"])
+
+
+@pytest.mark.parametrize("solver", ["rk4", "tsit5a"])
+def test_calibrated_trace_keeps_the_supplied_sample_times(gpu, solver):
+    core._TIMESTEP_CALIBRATION_CACHE.clear()
+    _, _, plain_times = _qubit(8, output_mode="final", solver=solver, return_time_trace=True,
+                               time_trace_every=8, ignore_non_finite_output=True)
+    res, tr, times, info = _qubit(8, output_mode="final", solver=solver, return_time_trace=True,
+                                  time_trace_every=8, calibrate_timesteps=LIMITS,
+                                  return_timing_info=True)
+    np.testing.assert_allclose(times, plain_times, rtol=1e-6)
+    np.testing.assert_allclose(times, (np.arange(len(times)) * 8 + 1) * PERIOD_Q / 8, rtol=1e-6)
+    if solver == "rk4":
+        assert info["timestep_calibration"]["steps"] > 80         # the grid was refined
+    ref, ref_tr, ref_times = _qubit(2048, output_mode="final", return_time_trace=True)
+    picked = np.searchsorted(ref_times, times - 1e-6 * PERIOD_Q)
+    np.testing.assert_allclose(ref_times[picked], times, rtol=1e-5)
+    _within(res, ref, LIMITS["rms_atol"], LIMITS["max_atol"])
+    _within(tr, ref_tr[:, :, picked], LIMITS["rms_atol"], LIMITS["max_atol"])
+
+
+@pytest.mark.parametrize("solver", ["rk4", "tsit5a"])
+@pytest.mark.parametrize("output", ["final", "mean", "observable", "trace"])
+def test_calibration_meets_limits_for_ode_outputs(gpu, solver, output):
+    if solver == "tsit5a" and output == "mean":
+        pytest.skip("the tsit5a mean is a trapezoid rule over its own steps: with a smooth "
+                    "solution and a coarse output grid that quadrature error is not set by the "
+                    "step tolerance")
+    x, v, t, damp, amp = sp.symbols("x v t damping amplitude", real=True)
+    axes = {damp: np.linspace(0.1, 0.5, 16), amp: np.linspace(0.1, 1.0, 16)}
+
+    def run(steps, **kw):
+        return odesolve_2D([v, -damp * v - x - 0.2 * x ** 3 + amp * sp.cos(1.3 * t)], [x, v],
+                           [1.0, 0.0], np.linspace(0.0, 20.0, steps + 1), var_arrays=axes, **kw)
+    lim = dict(rms_atol=1e-4, max_atol=1e-3, sample_stride=2)
+    extra = {"observable": dict(observable=x ** 2 + v ** 2), "mean": dict(output_mode="mean"),
+             "trace": dict(return_time_trace=True, time_trace_every=8), "final": {}}[output]
+    core._TIMESTEP_CALIBRATION_CACHE.clear()
+    got = run(64, solver=solver, calibrate_timesteps=lim, **extra)
+    if output == "trace":
+        _, tr, times = got
+        _, fine_tr, fine_times = run(16384, return_time_trace=True)
+        picked = np.searchsorted(fine_times, times - 1e-6)
+        np.testing.assert_allclose(fine_times[picked], times, rtol=1e-5)
+        _within(tr, fine_tr[:, :, picked], lim["rms_atol"], lim["max_atol"])
+    elif output == "mean":
+        # the exact time average of a fine trace (the initial state is the first sample)
+        _, fine_tr, fine_times = run(16384, return_time_trace=True)
+        states = np.concatenate([np.broadcast_to([1.0, 0.0], fine_tr.shape[:2] + (1, 2)),
+                                 fine_tr], axis=2)
+        ref = np.trapezoid(states, np.concatenate([[0.0], fine_times]), axis=2) / 20.0
+        _within(got, ref, lim["rms_atol"], lim["max_atol"])
+    else:
+        _within(got, run(16384, **extra), lim["rms_atol"], lim["max_atol"])
