@@ -42,8 +42,9 @@ MIN_GROUPED_STATES = 24     # below this one lane holds a point's state without 
 
 def _kernel_for(spec, groups, fp64):
     if groups > 1:
-        return codegen.kernel_name("gq_adaptive_grouped" if spec.adaptive else "gq_rk4_grouped",
-                                   fp64)
+        name = ("gq_adaptive_grouped" if spec.adaptive else
+                "gq_rk4_grouped" if spec.template == "fixed_rk4.hip" else "gq_fixed_grouped")
+        return codegen.kernel_name(name, fp64)
     return codegen.kernel_name(spec.kernel, fp64)
 
 

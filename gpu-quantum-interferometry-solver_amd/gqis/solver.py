@@ -241,7 +241,7 @@ def mesolve_2D(H, Drive=None, Col_Ops=None, mean_operator=None, tlist=None,
     ``block_size`` (threads per work-group, or ``"auto"``: measured once per kernel and device)
     ``lanes_per_point`` (1, 2, 4 or 8 lanes, one per wavefront, share each point's state rows;
     "auto": 8 for states of 48 or more reals where the grouped kernel applies, else 1;
-    ``block_size="auto"`` also measures 2, 4, 8 for RK4 models of 24 or more states) and
+    ``block_size="auto"`` also measures 2, 4, 8 for models of 24 or more states) and
     ``liouville`` ("auto", "lanes", "mfma", or "mfma_f16" / "mfma_bf16": the commutator product
     as three FP16 / BF16 matrix-core products of hi/lo parts with FP32 accumulation; the
     Hilbert-space path for N = 8, 16, 32 -- matrix cores in FP32, VALU dot products in FP64 --
@@ -502,8 +502,8 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
                            and grouped_applies(spec, compiled.state_size, fp64, 8) else 1)
     if lanes_per_point > 1 and not grouped_applies(spec, compiled.state_size, fp64,
                                                    lanes_per_point):
-        raise ValueError(f"lanes_per_point={lanes_per_point} needs the rk4, tsit5a or dop853a "
-                         f"solver and a state "
+        raise ValueError(f"lanes_per_point={lanes_per_point} needs a solver with a grouped "
+                         f"kernel ({', '.join(sorted(GROUPED_TEMPLATES))}) and a state "
                          f"that splits into {lanes_per_point} row blocks whose LDS copy fits "
                          f"(state size {compiled.state_size}).")
     cached_source = RHSreuse and ("source", lanes_per_point) + variant in _KERNEL_CACHE
@@ -681,6 +681,9 @@ GROUPED_AUTO_MIN_STATES = 48
 
 GROUPED_TEMPLATES = {"rk4": "grouped_rk4.hip", "tsit5a": "grouped_adaptive.hip",
                      "dop853a": "grouped_adaptive.hip"}
+GROUPED_TEMPLATES.update({name: "grouped_tableau.hip" for name in
+                          ("dp5", "tsit5", "dop853", "alshina6", "anas5", "lserk4", "lserk4f",
+                           "ab5")})
 
 
 def grouped_applies(spec, d, fp64, groups):
@@ -705,11 +708,16 @@ def _emit_source(compiled, spec, mode, uses_rho0_values, trace, state_trace, fp6
         "GQ_RHO0_VALUES": int(bool(uses_rho0_values)),
         "GQ_TIME_TABLE": int(bool(time_table) and not spec.adaptive),
     }
-    templates = ([spec.template] if spec.template != "fixed_rk4.hip"
-                 else ["rk4_table.hip", "fixed_rk4.hip"])
+    if spec.template == "fixed_rk4.hip":
+        templates = ["rk4_table.hip", "fixed_rk4.hip"]
+    elif spec.template == "fixed_tableau.hip":
+        templates = ["tableau_table.hip", "fixed_tableau.hip"]
+    else:
+        templates = [spec.template]
     if groups > 1:
-        templates = [t if t not in ("fixed_rk4.hip", "adaptive.hip")
-                     else GROUPED_TEMPLATES[spec.name] for t in templates]
+        # the fixed-step grouped kernels share their output and LDS helpers
+        grouped = GROUPED_TEMPLATES[spec.name]
+        templates = templates[:-1] + ([] if spec.adaptive else ["grouped_common.hip"]) + [grouped]
     from . import tableaux
     extra = ""
     if spec.adaptive:

@@ -1,5 +1,5 @@
-"""Lanes-per-point (grouped) RK4 kernels on the GPU: the same results as the one-lane-per-point
-kernel for every output mode, FP64 parity with the oracle's golden fixture, continuation from
+"""Lanes-per-point (grouped) kernels on the GPU (RK4, the other fixed-step methods, Tsit5A and
+DOP853A): the same results as the one-lane-per-point kernels for every output mode, FP64 parity with the oracle's golden fixture, continuation from
 a saved state, ragged point counts, and the autotuner's choice among both forms."""
 import os
 
@@ -168,3 +168,52 @@ def test_auto_lanes_per_point_default(gpu):
     _, info = mesolve_2D(w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"],
                          var_arrays=w["var_arrays"], fp64=True, return_timing_info=True)
     assert info["lanes_per_point"] == 1
+
+
+TABLEAU = ("dp5", "tsit5", "dop853", "alshina6", "anas5", "lserk4", "lserk4f", "ab5")
+
+
+@pytest.mark.parametrize("solver", TABLEAU)
+def test_grouped_tableau_matches_per_lane(gpu, solver):
+    # 63 points (a ragged group of 64) at 63 states; the grouped kernel evaluates the same
+    # expressions in the same order as the one-lane kernel.  AB5 is unstable for the chain's
+    # nearly imaginary eigenvalues at dt = 0.025 (rounding then grows without bound), so it
+    # takes a five times smaller step.
+    args, var = _chain8(n_eps=9, n_amp=7, n_t=2001 if solver == "ab5" else 401, t_end=10.0)
+    for mode, groups in (("mean", 8), ("final_rho", 4), ("final", 2)):
+        ref = mesolve_2D(*args, var_arrays=var, fp64=True, solver=solver, output_mode=mode,
+                         liouville="lanes", lanes_per_point=1)
+        got = mesolve_2D(*args, var_arrays=var, fp64=True, solver=solver, output_mode=mode,
+                         liouville="lanes", lanes_per_point=groups)
+        print(solver, mode, groups, "bitwise" if np.array_equal(got, ref) else _rel(got, ref),
+              flush=True)
+        assert got.shape == ref.shape and np.max(np.abs(ref)) < 10.0     # a stable case
+        assert _rel(got, ref) <= 1e-13
+
+
+@pytest.mark.parametrize("solver", ["dp5", "lserk4", "ab5"])
+def test_grouped_tableau_traces_and_lane_time(gpu, solver):
+    # observable and state traces; stage times evaluated per lane instead of from the table
+    args, var = _chain8(n_eps=5, n_amp=6, n_t=1001 if solver == "ab5" else 201, t_end=5.0)
+    for kw in ({"output_mode": "mean", "return_time_trace": True, "time_trace_every": 7},
+               {"output_mode": "final_rho", "return_time_trace": True, "time_trace_every": 25},
+               {"output_mode": "final_rho", "time_table": False}):
+        ref = mesolve_2D(*args, var_arrays=var, fp64=True, solver=solver, liouville="lanes",
+                         lanes_per_point=1, **kw)
+        got = mesolve_2D(*args, var_arrays=var, fp64=True, solver=solver, liouville="lanes",
+                         lanes_per_point=4, **kw)
+        if not isinstance(ref, tuple):
+            ref, got = (ref,), (got,)
+        for a, b in zip(got[:2], ref[:2]):
+            assert a.shape == b.shape and np.max(np.abs(b)) < 10.0 and _rel(a, b) <= 1e-13
+        if len(ref) > 2:
+            np.testing.assert_array_equal(got[2], ref[2])
+
+
+def test_grouped_tableau_fp32(gpu):
+    args, var = _chain8(n_eps=8, n_amp=8, n_t=201, t_end=5.0)
+    ref = mesolve_2D(*args, var_arrays=var, fp64=False, solver="dp5", liouville="lanes",
+                     lanes_per_point=1)
+    got = mesolve_2D(*args, var_arrays=var, fp64=False, solver="dp5", liouville="lanes",
+                     lanes_per_point=4)
+    assert _rel(got, ref) <= 1e-5

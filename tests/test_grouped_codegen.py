@@ -135,3 +135,16 @@ def test_hiprtc_compiles_grouped_kernels_for_gfx950():
                            fp64, True, 256, False, groups)
         assert "gq_rk4_grouped" in src
         assert runtime.compile_check(src, codegen.HIPRTC_BASE_OPTIONS) > 1000
+
+
+@pytest.mark.parametrize("solver", ["dp5", "lserk4", "ab5", "tsit5a"])
+def test_hiprtc_compiles_grouped_tableau_kernels(solver):
+    # one method of each fixed-step scheme (tableau, 2N-storage, multistep) and an adaptive one
+    compiled, _ = _models()["chain8"]
+    spec = get_solver_spec(solver, step_size=0.05)
+    for fp64, groups, mode, trace in ((True, 8, "mean", True), (False, 4, "final", False)):
+        src = _emit_source(compiled, spec, mode, False, trace, False, fp64, True, 256, False,
+                           groups)
+        name = "gq_adaptive_grouped" if spec.adaptive else "gq_fixed_grouped"
+        assert name in src
+        assert runtime.compile_check(src, codegen.HIPRTC_BASE_OPTIONS) > 1000
