@@ -103,6 +103,29 @@ def cpu_baseline(seconds_budget=15.0):
                       f"(OpenMP, {threads} threads), {dt:.1f} s"}
 
 
+def reference_cpu_path():
+    """The reference's own CPU path restated (per-point Python RK4 on a lambdified RHS, spawn
+    pool over eps columns, workers = cores - 1 as the reference defaults to) on a bounded
+    sample: 16 eps columns x 64 A values spread over the 2048 x 2048 grid, full 2000 steps.
+    The sample's results are compared with the C oracle's."""
+    import numpy as np
+    from oracle import cbridge, models, python_cpu
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    eps = np.linspace(-15.0, 15.0, 2048)[::128]
+    amp = np.linspace(0.0, 30.0, 2048)[::32]
+    res, compute_s, workers = python_cpu.sweep("four_level", eps, amp, 0.05, 2000,
+                                               workers=max(1, cores - 1))
+    ref = np.asarray(cbridge.rk4_affine(models.four_level_chain(eps, amp), 0.05, 2000)).real
+    ok = np.isfinite(ref.reshape(res.shape))
+    dev = float(np.max(np.abs(res - ref.reshape(res.shape))[ok]))
+    return {"value": res.size * 2000 / compute_s, "unit": "param-point RK steps/s",
+            "cores": workers, "kind": "port",
+            "sample": f"{len(eps)} eps x {len(amp)} A values of the 2048 x 2048 grid x 2000 RK4 "
+                      f"steps, per-point Python loop on a lambdified RHS, {workers} spawn "
+                      f"workers, {compute_s:.1f} s after worker preparation",
+            "max_abs_dev_vs_c_oracle": dev}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,6 +259,8 @@ def main():
     if not args.no_cpu:
         log("CPU baseline ...")
         out["cpu_baseline"] = cpu_baseline()
+        log("reference CPU path ...")
+        out["cpu_reference_path"] = reference_cpu_path()
     print(json.dumps(out), flush=True)
 
 

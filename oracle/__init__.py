@@ -15,6 +15,8 @@ Contents:
   forced Duffing).
 * ``rk4_affine.c`` -- the same fixed-step RK4 on an affine model in plain C (OpenMP over points),
   used as the fast oracle for large samples and as the timed CPU baseline.
+* ``python_cpu`` -- the reference's own CPU path restated (per-point Python RK4 on a lambdified
+  RHS in a spawn pool), timed beside the C baseline.
 
 Every function here is pinned by closed forms, the reference's known-answer tests, SciPy's
 ``solve_ivp`` or matrix exponentials (see ``tests/test_oracle.py``).

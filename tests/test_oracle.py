@@ -234,3 +234,17 @@ def test_dop853_fixed_is_eighth_order():
     errs = [np.max(np.abs(integrate.fixed_step("dop853", f, y0[:, None], 4.0 / n, n,
                                                mode="final_rho")[0][0] - exact)) for n in (4, 8)]
     assert np.log2(errs[0] / errs[1]) > 7.0
+
+
+def test_python_cpu_path_matches_c_oracle():
+    # the restated per-point Python path (timed CPU comparison) gives the C oracle's map,
+    # also through its spawn pool
+    from oracle import cbridge, python_cpu
+    eps, amp = np.array([-3.0, 7.5]), np.array([0.0, 4.0, 12.0])
+    ref = np.asarray(cbridge.rk4_affine(models.four_level_chain(eps, amp), 0.05, 200)).real
+    rhs, obs, size = python_cpu.prepare("four_level")
+    one = python_cpu.solve_point(rhs, obs, size, eps[1], amp[2], 0.05, 200)
+    assert abs(one - ref.reshape(2, 3)[1, 2]) < 1e-13
+    out, compute_s, workers = python_cpu.sweep("four_level", eps, amp, 0.05, 200, workers=2)
+    assert workers == 2 and compute_s >= 0.0
+    np.testing.assert_allclose(out, ref.reshape(2, 3), rtol=0, atol=1e-13)
