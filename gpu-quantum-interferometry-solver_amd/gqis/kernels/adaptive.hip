@@ -23,15 +23,6 @@ typedef real st_t;
 #endif
 
 #define GQ_NS GQ_NSTAGES
-#if GQ_METHOD_DOP853
-#define GQ_FAC_MIN (1.0 / 3.0)
-#define GQ_FAC_MAX 6.0
-#define GQ_ERR_EXPONENT (-1.0 / 8.0)
-#else
-#define GQ_FAC_MIN 0.2
-#define GQ_FAC_MAX 5.0
-#define GQ_ERR_EXPONENT (-0.2)
-#endif
 
 __device__ __forceinline__ void gq_stage_at(double t_abs, real px, real py,
                                             const real* __restrict__ C, const gq_static_t& S,
@@ -219,8 +210,7 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_adaptive)(co
                 err_norm = sqrt(ssum / (double)GQ_D);
             }
 #endif
-            double factor = 0.9 * pow(err_norm, GQ_ERR_EXPONENT);
-            factor = (factor != factor) ? GQ_FAC_MIN : fmin(GQ_FAC_MAX, fmax(GQ_FAC_MIN, factor));
+            const double factor = gq_step_factor(err_norm);   // step_control.hip
             if (err_norm <= 1.0) {
                 ++n_acc;
 #if GQ_METHOD_DOP853

@@ -21,15 +21,6 @@ typedef real st_t;
 
 #define GQ_NS GQ_NSTAGES
 #define GQ_GW (GQ_G * 64)
-#if GQ_METHOD_DOP853
-#define GQ_FAC_MIN (1.0 / 3.0)
-#define GQ_FAC_MAX 6.0
-#define GQ_ERR_EXPONENT (-1.0 / 8.0)
-#else
-#define GQ_FAC_MIN 0.2
-#define GQ_FAC_MAX 5.0
-#define GQ_ERR_EXPONENT (-0.2)
-#endif
 
 template <int GI>
 __device__ __forceinline__ void gq_stage_at(double t_abs, real px, real py,
@@ -197,8 +188,7 @@ __device__ __forceinline__ void gq_adaptive_body(const gq_sweep_args& a,
             err_norm = sqrt(ssum / (double)GQ_D);
         }
 #endif
-        double factor = 0.9 * pow(err_norm, GQ_ERR_EXPONENT);
-        factor = (factor != factor) ? GQ_FAC_MIN : fmin(GQ_FAC_MAX, fmax(GQ_FAC_MIN, factor));
+        const double factor = gq_step_factor(err_norm);   // step_control.hip
         const bool accept = live && err_norm <= 1.0;
 #if GQ_METHOD_DOP853
         P::rhs(Yn, lane, px, py, C, Sg, Gg, kf);

@@ -713,7 +713,7 @@ def _emit_source(compiled, spec, mode, uses_rho0_values, trace, state_trace, fp6
     elif spec.template == "fixed_tableau.hip":
         templates = ["tableau_table.hip", "fixed_tableau.hip"]
     else:
-        templates = [spec.template]
+        templates = ["step_control.hip", spec.template] if spec.adaptive else [spec.template]
     if groups > 1:
         # the fixed-step grouped kernels share their output and LDS helpers
         grouped = GROUPED_TEMPLATES[spec.name]
