@@ -143,6 +143,10 @@ def main():
                     help="four_level: the headline sweep; lzs: the 64x64 two-level map with the "
                          "CPU oracle beside it; duffing: adaptive Tsit5/DOP853 sweep; serving: "
                          "Poisson request stream; scaling: Hilbert-space N-level models")
+    ap.add_argument("--levels", type=int, nargs="*", default=None,
+                    help="scaling: only these N (4, 8, 16, 32)")
+    ap.add_argument("--forms", default="",
+                    help="scaling: only these forms, comma-separated (e.g. mfma,mfma_f16)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="with --profile: also run one device copy of this many bytes "
                          "(the counter calibration kernel)")
@@ -347,11 +351,15 @@ def bench_scaling(args):
            "unit": "param-point RK steps/s", "n_gpus": 1, "data": "synthetic",
            "config": {"workload": f"random N-level, {side}x{side}, RK4 2000 steps dt 0.01",
                       "points": side * side}, "results": {}}
-    for n, forms in ((4, ["lanes"]),
-                     (8, ["lanes", "lanes_g4", "lanes_g8", "mfma", "mfma_f16", "lanes64",
-                          "lanes_g8_64", "mfma64"]),
-                     (16, ["lanes_g8", "mfma", "mfma_f16", "mfma_bf16"]),
-                     (32, ["mfma", "mfma_f16", "mfma_bf16"])):
+    plan = ((4, ["lanes"]),
+            (8, ["lanes", "lanes_g4", "lanes_g8", "mfma", "mfma_f16", "lanes64", "lanes_g8_64",
+                 "mfma64"]),
+            (16, ["lanes_g8", "mfma", "mfma_f16", "mfma_bf16"]),
+            (32, ["mfma", "mfma_f16", "mfma_bf16"]))
+    if args.levels:
+        plan = tuple((n, [f for f in forms if not args.forms or f in args.forms.split(",")])
+                     for n, forms in plan if n in args.levels)
+    for n, forms in plan:
         w = workloads.random_n_level(n, n_eps=side, n_amp=side)
         sub = {k: v[:: side // 16] for k, v in w["var_arrays"].items()}
         args_m = (w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"])
