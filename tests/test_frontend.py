@@ -188,3 +188,27 @@ def test_bare_time_in_an_equation_becomes_a_time_atom():
     assert [str(e) for _, e in c.plan.time_atoms] == ["t"]
     got, _ = _eval_plan(c.plan, 0.0, 0.0, 2.25, [0.3])
     assert got[0] == 2.25
+
+
+_A, _DRIVE, _DUMMY = sp.symbols("A Drive dummy", real=True)
+_RABI = (_DRIVE / 2 * sp.Matrix([[0, 1], [1, 0]]), _A, [], sp.diag(0, 1))
+
+
+def test_installation_symbolic_and_code_generation_stages():
+    # the reference's installation self-test (gqis/check_environment.py:284-301): three real
+    # equations, Im(rho_01)' = A/2 at |0><0| for A = 2, three generated RHS lines
+    from gqis import compile_lindblad_rhs
+    from gqis.solver import _emit_source
+    rhs = compile_lindblad_rhs(*_RABI, [_A, _DUMMY])
+    at_ground = dict(zip(rhs.state_symbols, (1, 0, 0)))
+    drive = rhs.drive_map[_DRIVE] if isinstance(rhs.drive_map, dict) else rhs.drive_map
+    values = [float(sp.sympify(e).subs(_DRIVE, drive).subs(at_ground)
+                    .subs({sp.Symbol("ParX", real=True): 2.0, _A: 2.0}))
+              for e in rhs.equations]
+    assert rhs.state_size == 3
+    np.testing.assert_allclose(values, [0.0, 0.0, 1.0], rtol=0, atol=1e-12)
+    src = _emit_source(rhs, get_solver_spec("rk4", step_size=0.01), "final", False, False,
+                       False, False, True, 256, False)
+    body = src[src.index("void gq_rhs("):]
+    body = body[:body.index("\n}")]
+    assert body.count("dy[") == 3
