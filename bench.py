@@ -354,8 +354,8 @@ def bench_scaling(args):
     plan = ((4, ["lanes"]),
             (8, ["lanes", "lanes_g4", "lanes_g8", "mfma", "mfma_f16", "lanes64", "lanes_g8_64",
                  "mfma64"]),
-            (16, ["lanes_g8", "mfma", "mfma_f16", "mfma_bf16"]),
-            (32, ["mfma", "mfma_f16", "mfma_bf16"]))
+            (16, ["lanes_g8", "mfma", "mfma_f16", "mfma_bf16", "mfma64"]),
+            (32, ["mfma", "mfma_f16", "mfma_bf16", "mfma64"]))
     if args.levels:
         plan = tuple((n, [f for f in forms if not args.forms or f in args.forms.split(",")])
                      for n, forms in plan if n in args.levels)
@@ -385,14 +385,15 @@ def bench_scaling(args):
             ps = side * side * 2000
             d2 = n * n
             flops = (4 * 8 * n ** 3 + 4 * 12 * d2 + 13 * (d2 - 1)
-                     if form.startswith("mfma") and not fp64 else None)
+                     if form.startswith("mfma") else None)
             entry = {"kernel_s": k, "value": ps / k,
                      "max_abs_err_vs_fp64": float(np.max(np.abs(sub_res - ref)))}
             if flops:
                 ach = flops * ps / k / 1e12
+                peak = 78.6 if fp64 else 157.3
                 entry.update({"flops_per_point_step": flops, "achieved_tflops": ach,
-                              "mfma_fp32_peak_frac": ach / 157.3})
-                if form != "mfma":
+                              f"mfma_{'fp64' if fp64 else 'fp32'}_peak_frac": ach / peak})
+                if form not in ("mfma", "mfma64"):
                     # three half-precision products per FP32-equivalent product
                     entry["half_mfma_issued_frac"] = 3 * ach / 2500.0
             out["results"][f"N={n} {form}"] = entry

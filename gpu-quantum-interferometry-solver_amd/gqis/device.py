@@ -186,7 +186,7 @@ def run_sweep(*, source, variant, spec, compiled, mode, var_arrays, rho0_values,
                         "steps": steps_out},
             "info": {"build_s": build_s, "cached_kernel": cached, "block_size": int(block_size),
                      "lanes_per_point": int(groups), "autotune_s": tune_s,
-                     "autotune": tune_times}}
+                     "autotune": tune_times, "kernel": kname}}
     if prepare:
         return plan
     timer = runtime.EventTimer(ctx, stream) if collect else None

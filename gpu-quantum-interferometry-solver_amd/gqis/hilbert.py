@@ -143,6 +143,39 @@ def pack_a(m):
     return out
 
 
+def dmfma_order(n):
+    """Logical Rbig row (Re rho_i: i, Im rho_i: n + i) held at each row 16 T + 4 r + g of the
+    FP64 matrix-core kernel's tiles: slot m = 4 T + r of lane group g is Re rho_{g n/4 + m} for
+    m < n/4, else Im rho_{g n/4 + m - n/4} (kernels/hilbert_rk4_dmfma.hip)."""
+    q = n // 4
+    perm = np.zeros(2 * n, int)
+    for t in range(n // 8):
+        for r in range(4):
+            for g in range(4):
+                m = 4 * t + r
+                perm[16 * t + 4 * r + g] = g * q + m if m < q else n + g * q + (m - q)
+    return perm
+
+
+def pack_a_dmfma(m):
+    """A operand of the FP64 matrix-core kernel: [To][T][r][lane], lane (i, g) = (lane % 16,
+    lane // 16) holding Hbig[16 To + i][16 T + 4 r + g] with rows and columns in dmfma_order."""
+    n = m.n
+    big = np.zeros((2 * n, 2 * n))
+    big[:n, :n], big[:n, n:] = m.h_eff.real, -m.h_eff.imag
+    big[n:, :n], big[n:, n:] = m.h_eff.imag, m.h_eff.real
+    perm = dmfma_order(n)
+    bp = big[np.ix_(perm, perm)]
+    tr = n // 8
+    lane = np.arange(64)
+    out = np.zeros((tr, tr, 4, 64))
+    for to in range(tr):
+        for t in range(tr):
+            for r in range(4):
+                out[to, t, r] = bp[16 * to + lane % 16, 16 * t + 4 * r + lane // 16]
+    return out
+
+
 SPLIT_KINDS = {None: 0, "f16": 1, "bf16": 2}
 
 
@@ -155,8 +188,11 @@ def _round_half(x, kind):
     return rounded.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def kernel_base(split=None):
-    """Kernel name (before the precision suffix) of the FP32 / split-precision forms."""
+def kernel_base(split=None, fp64=False, valu=False):
+    """Kernel name (before the precision suffix): FP32 MFMA / split-precision forms, the FP64
+    matrix-core form (gq_hilbert_rk4_dmfma_f64) or, with ``valu``, the FP64 VALU form."""
+    if fp64 and not valu:
+        return "gq_hilbert_rk4_dmfma"
     return {None: "gq_hilbert_rk4", "f16": "gq_hilbert_rk4_f16x3",
             "bf16": "gq_hilbert_rk4_bf16x3"}[split]
 
@@ -206,7 +242,8 @@ def constants(m, runtime_values, split=None):
     big = np.zeros((2 * n, 2 * n))
     big[:n, :n], big[:n, n:] = m.h_eff.real, -m.h_eff.imag
     big[n:, :n], big[n:, n:] = m.h_eff.imag, m.h_eff.real
-    parts = {"RC": rc, "APACK": pack_a(m).ravel(), "HBIG": big.ravel(), "DEP": m.dep.ravel(),
+    parts = {"RC": rc, "APACK": pack_a(m).ravel(), "HBIG": big.ravel(),
+             "DPACK": pack_a_dmfma(m).ravel(), "DEP": m.dep.ravel(),
              "JW": jw.ravel(), "JB": jb.ravel(), "VCOEF": m.vcoef.ravel(), "OBS": m.obs}
     if split:
         hi, lo = pack_a_split(m, split)
@@ -222,8 +259,13 @@ def constants(m, runtime_values, split=None):
 LDS_BYTES_PER_CU = 160 * 1024      # gfx950
 
 
-def lds_bytes(n, fp64, block, jmax, split=None):
-    """Static LDS of the kernel (the ``__shared__`` arrays of ``kernels/hilbert_rk4.hip``)."""
+def lds_bytes(n, fp64, block, jmax, split=None, valu=False):
+    """Static LDS of the kernel (the ``__shared__`` arrays of ``kernels/hilbert_rk4.hip`` and,
+    for FP64 without ``valu``, ``kernels/hilbert_rk4_dmfma.hip``)."""
+    if fp64 and not valu:
+        waves, pts = block // 64, (2 if n == 8 else 1)
+        return (n // 8) ** 2 * 4 * 64 * 8 + waves * (2 * pts * n * (n + 1) + pts * n + 64) * 8 \
+            + n * n * 8 + n * jmax * 12
     r = 8 if fp64 else 4
     waves, pts, tiles = block // 64, 32 // n, (2 * n + 31) // 32
     total = waves * 2 * pts * n * (n + 1) * r + waves * pts * n * r + n * n * r
@@ -237,16 +279,17 @@ def lds_bytes(n, fp64, block, jmax, split=None):
     return total
 
 
-def block_for(m, fp64, block_size=256):
-    """Work-group size: the FP64 N = 32 kernel keeps its LDS under 160 KB with two waves."""
+def block_for(m, fp64, block_size=256, valu=False):
+    """Work-group size: the FP64 VALU N = 32 kernel keeps its LDS under 160 KB with two waves."""
     if block_size == "auto":
         return block_size
-    return 128 if (fp64 and m.n == 32) else block_size
+    return 128 if (fp64 and valu and m.n == 32) else block_size
 
 
-def emit(m, offs, mode, block_size=256, fp64=False, split=None):
+def emit(m, offs, mode, block_size=256, fp64=False, split=None, valu=False):
     """hiprtc source of the Hilbert-space RK4 kernel for model m (``split``: the commutator
-    GEMM as three FP16 or BF16 products of hi/lo parts with FP32 accumulation)."""
+    GEMM as three FP16 or BF16 products of hi/lo parts with FP32 accumulation; FP64 runs on the
+    FP64 matrix cores, or with ``valu`` as VALU dot products)."""
     split_kind = SPLIT_KINDS[split]
     atoms = {}
     names = {"ParX": "px", "ParY": "py"}
@@ -288,16 +331,26 @@ def emit(m, offs, mode, block_size=256, fp64=False, split=None):
                "GQ_MFMA_SPLIT": split_kind}
     parts = [header, codegen.prelude(fp64, True, block_size, defines), model_src,
              codegen.read_kernel_template("common.hip"), codegen.read_kernel_template("rk4_table.hip"),
-             codegen.read_kernel_template("hilbert_rk4.hip")]
+             codegen.read_kernel_template("hilbert_rk4_dmfma.hip" if fp64 and not valu
+                                          else "hilbert_rk4.hip")]
     return "\n".join(parts)
+
+
+def points_per_wave(n, fp64, valu=False):
+    """Points per wavefront: 32 / N columns of one 32-column MFMA tile (FP32 forms, FP64 VALU),
+    16 / N for the FP64 matrix-core form (one point in two column tiles at N = 32)."""
+    if fp64 and not valu:
+        return 2 if n == 8 else 1
+    return 32 // n
 
 
 _SOURCES = {}
 
 
 def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps, warmup_steps,
-        block_size=256, collect=False, options=(), fp64=False, split=None):
-    """Launch the Hilbert-space kernel; returns the same dict as ``device.run_sweep``."""
+        block_size=256, collect=False, options=(), fp64=False, split=None, valu=False):
+    """Launch the Hilbert-space kernel; returns the same dict as ``device.run_sweep``
+    (``valu``: the FP64 VALU form instead of the FP64 matrix-core form)."""
     import torch
     from .device import _axis, _initial_states
     setup_start = time.time()
@@ -307,14 +360,14 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
         raise ValueError("split-precision matrix-core products are an FP32-mode option")
     consts, offs = constants(m, runtime_values, split)
     uses_rho0 = rho0_values is not None
-    block_size = block_for(m, fp64, block_size)
+    block_size = block_for(m, fp64, block_size, valu)
     tune = block_size == "auto"
     if tune:
-        block_size = block_for(m, fp64, 256)
-    skey = (m.key, mode, block_size, uses_rho0, tuple(options), bool(fp64), split)
+        block_size = block_for(m, fp64, 256, valu)
+    skey = (m.key, mode, block_size, uses_rho0, tuple(options), bool(fp64), split, bool(valu))
     src = _SOURCES.get(skey)
     if src is None:
-        src = emit(m, offs, mode, block_size, fp64, split).replace(
+        src = emit(m, offs, mode, block_size, fp64, split, valu).replace(
             "GQ_RHO0_FROM_ARGS", "1" if uses_rho0 else "0")
         _SOURCES[skey] = src
     options = codegen.HIPRTC_BASE_OPTIONS + tuple(options)
@@ -348,7 +401,7 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
     a.time_table = table.data_ptr()
     a.result, a.status = result.data_ptr(), status.data_ptr()
     stream = torch.cuda.current_stream(dev).cuda_stream
-    kname = codegen.kernel_name(kernel_base(split), fp64)
+    kname = codegen.kernel_name(kernel_base(split, fp64, valu), fp64)
     tune_s, tune_times = 0.0, {}
     if tune and npts > 0 and num_steps > 0:
         from . import autotune
@@ -360,13 +413,13 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
         trial_steps = min(int(num_steps), 64)
         jmax = jump_width(m)
         fits = tuple(b for b in autotune.CANDIDATES
-                     if lds_bytes(m.n, fp64, b, jmax, split) <= LDS_BYTES_PER_CU)
+                     if lds_bytes(m.n, fp64, b, jmax, split, valu) <= LDS_BYTES_PER_CU)
         spare = torch.zeros(1, dtype=torch.int32, device=dev)
 
         def trial(mod, block):
             t = runtime.SweepArgs.from_buffer_copy(a)
             t.num_steps, t.warmup_steps = trial_steps, 0
-            per = (block // 64) * (32 // m.n)
+            per = (block // 64) * points_per_wave(m.n, fp64, valu)
             return _timed_launches(ctx, mod, kname, (npts + per - 1) // per, block, t, stream,
                                    spare)
 
@@ -380,7 +433,7 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
         tune_s = time.time() - t0
     timer = runtime.EventTimer(ctx, stream) if collect else None
     setup_s = time.time() - setup_start
-    per_block = (block_size // 64) * (32 // m.n)
+    per_block = (block_size // 64) * points_per_wave(m.n, fp64, valu)
     if npts > 0:
         if timer:
             timer.record_start()
@@ -400,4 +453,5 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
         out = result[:npts].reshape(nx, ny)
     return {"result": out, "trace": None, "status": flags, "gpu_s": gpu_s, "build_s": build_s,
             "cached_kernel": cached, "setup_s": setup_s, "adaptive_steps": None,
-            "block_size": int(block_size), "autotune_s": tune_s, "autotune": tune_times}
+            "block_size": int(block_size), "autotune_s": tune_s, "autotune": tune_times,
+            "kernel": kname}

@@ -118,14 +118,15 @@ class SweepServer:
             block = hilbert.block_for(hm, r.fp64, 256 if self.block == "auto" else self.block)
             src = hilbert.emit(hm, offs, r.output_mode, block, r.fp64, split).replace(
                 "GQ_RHO0_FROM_ARGS", "0")
-            kname = codegen.kernel_name(hilbert.kernel_base(split), r.fp64)
+            kname = codegen.kernel_name(hilbert.kernel_base(split, r.fp64), r.fp64)
             natoms = int(src.split("#define GQ_NATOMS ")[1].split()[0])
             cands = tuple((1, b) for b in autotune.CANDIDATES
                           if hilbert.lds_bytes(hm.n, r.fp64, b, hilbert.jump_width(hm), split)
                           <= hilbert.LDS_BYTES_PER_CU)
 
             def variant(c):
-                return (autotune.with_block(src, c[1]), kname, (c[1] // 64) * (32 // hm.n))
+                return (autotune.with_block(src, c[1]), kname,
+                        (c[1] // 64) * hilbert.points_per_wave(hm.n, r.fp64))
             consts_vals = cvals
             table_kernel = codegen.kernel_name("gq_rk4_time_table", r.fp64)
             rows = 2 * num_steps + 1
@@ -215,12 +216,13 @@ class SweepServer:
                                       flags[1:])
 
     def _hilbert_model(self, r):
-        """Matrix-core model of a Lindblad request with N >= 16 whose operators fit the
-        Hilbert-space kernel, else None (smaller models run faster on the per-point kernels)."""
+        """Matrix-core model of a Lindblad request with N >= 16 (N >= 8 in FP64) whose
+        operators fit the Hilbert-space kernel, else None (smaller models run faster on the
+        per-point kernels); the same choice mesolve_2D makes."""
         if r.compiled.kind != "lindblad" or r.output_mode not in ("mean", "final", "final_rho"):
             return None
         n = int(round(np.sqrt(r.compiled.state_size + 1)))
-        if n < 16:
+        if n < (8 if r.fp64 else 16):
             return None
         src = getattr(r.compiled, "source_model", None)
         if src is None:

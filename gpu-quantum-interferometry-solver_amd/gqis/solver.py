@@ -244,8 +244,8 @@ def mesolve_2D(H, Drive=None, Col_Ops=None, mean_operator=None, tlist=None,
     ``block_size="auto"`` also measures 2, 4, 8 for models of 24 or more states) and
     ``liouville`` ("auto", "lanes", "mfma", or "mfma_f16" / "mfma_bf16": the commutator product
     as three FP16 / BF16 matrix-core products of hi/lo parts with FP32 accumulation; the
-    Hilbert-space path for N = 8, 16, 32 -- matrix cores in FP32, VALU dot products in FP64 --
-    used automatically from N = 16 in RK4 sweeps whose operators fit it).  ``return_device=True`` returns torch tensors on
+    Hilbert-space path for N = 8, 16, 32 on the FP32 or FP64 matrix cores, used automatically
+    in RK4 sweeps whose operators fit it from N = 16, and from N = 8 in FP64).  ``return_device=True`` returns torch tensors on
     the GPU.  ``H`` may be a ``CompiledRHS`` from ``compile_lindblad_rhs``.
 
     Returns ``(num_X, num_Y)`` complex for ``"mean"``/``"final"``, ``(num_X, num_Y, N*N-1)``
@@ -583,7 +583,8 @@ def _finish(outputs, stage_s, rhs_status, spec, solver_frequency, unroll, collec
                 "solver": spec.name, "solver_frequency": solver_frequency, "unroll": unroll,
                 "block_size": outputs.get("block_size"),
                 "lanes_per_point": outputs.get("lanes_per_point", 1),
-                "autotune_s": outputs.get("autotune_s", 0.0)}
+                "autotune_s": outputs.get("autotune_s", 0.0),
+                "kernel": outputs.get("kernel")}
         if outputs.get("autotune"):
             info["autotune"] = outputs["autotune"]
         if outputs.get("adaptive_steps") is not None:
@@ -612,7 +613,10 @@ def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, run
     n = H.shape[0] if hasattr(H, "shape") else 0
     forced = liouville.startswith("mfma")
     split = {"mfma_f16": "f16", "mfma_bf16": "bf16"}.get(liouville)
-    wanted = forced or (liouville == "auto" and n >= 16)
+    # measured on gfx950 (random N-level, 256^2, 2000 steps): from 16 levels the matrix-core
+    # forms are fastest in both precisions; at 8 levels in FP64 the FP64 matrix-core form (52 ms)
+    # beats 8 lanes per point (77 ms), while FP32 stays on the per-point kernels
+    wanted = forced or (liouville == "auto" and (n >= 16 or (fp64 and n >= 8)))
     reasons = []
     if n not in hilbert.SUPPORTED_N:
         reasons.append(f"N = {n} (supported: {hilbert.SUPPORTED_N})")

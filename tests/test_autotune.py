@@ -87,8 +87,12 @@ def test_hilbert_lds_formula_matches_the_compiler():
                                   auto_runtime_consts=False, rho0=None, ode_size=None)
     hm = hilbert.extract(model, solver._drive_map(model["Drive"]))
     _, offs = hilbert.constants(hm, [])
-    src = hilbert.emit(hm, offs, "mean", 256, True).replace("GQ_RHO0_FROM_ARGS", "0")
-    est = hilbert.lds_bytes(32, True, 256, hilbert.jump_width(hm))
-    assert est > hilbert.LDS_BYTES_PER_CU
-    with pytest.raises(runtime.GqError, match=rf"local memory \({est}\) exceeds limit"):
-        runtime.compile_check(src, codegen.HIPRTC_BASE_OPTIONS)
+    # FP64 VALU form at 256 threads and FP64 matrix-core form at 512 threads: both over the
+    # limit, and the compiler's figure is the formula's
+    for block, valu in ((256, True), (512, False)):
+        src = hilbert.emit(hm, offs, "mean", block, True, valu=valu).replace(
+            "GQ_RHO0_FROM_ARGS", "0")
+        est = hilbert.lds_bytes(32, True, block, hilbert.jump_width(hm), valu=valu)
+        assert est > hilbert.LDS_BYTES_PER_CU
+        with pytest.raises(runtime.GqError, match=rf"local memory \({est}\) exceeds limit"):
+            runtime.compile_check(src, codegen.HIPRTC_BASE_OPTIONS)

@@ -93,3 +93,62 @@ def test_auto_uses_fp16_split_products_from_16_levels(gpu):
     np.testing.assert_array_equal(auto, f16)
     with pytest.raises(ValueError, match="FP32-mode option"):
         mesolve_2D(*args, var_arrays=w["var_arrays"], liouville="mfma_f16", fp64=True)
+
+
+def _hilbert_model(w):
+    from gqis import hilbert
+    from gqis.solver import _drive_map, _prepare_model
+    model = _prepare_model(w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"],
+                           sweep_symbols=tuple(w["var_arrays"]), const_values=None,
+                           runtime_consts=None, keep_symbolic_consts=None,
+                           auto_runtime_consts=False, rho0=None, ode_size=None)
+    return hilbert.extract(model, _drive_map(model["Drive"]))
+
+
+@pytest.mark.parametrize("n,steps", [(8, 300), (16, 150), (32, 60)])
+def test_fp64_matrix_core_form_matches_fp64_valu_form(gpu, n, steps):
+    # the FP64 matrix-core kernel (default FP64 form) against the FP64 VALU dot-product kernel:
+    # every output mode, a ragged point count (3 x 3: N = 8 puts two points in a wavefront) and
+    # a continuation from explicit initial states
+    from gqis import hilbert
+    w = workloads.random_n_level(n, seed=SEED + n, n_eps=3, n_amp=3, n_t=steps + 1,
+                                 t_end=0.01 * steps)
+    hm = _hilbert_model(w)
+    kw = dict(var_arrays=w["var_arrays"], runtime_values=[], dt=0.01, t_in=0.0,
+              num_steps=steps, warmup_steps=steps // 4, fp64=True)
+    finals = {}
+    for mode in ("mean", "final", "final_rho"):
+        got = hilbert.run(hm, mode=mode, rho0_values=None, **kw)
+        ref = hilbert.run(hm, mode=mode, rho0_values=None, valu=True, **kw)
+        g, r = got["result"].cpu().numpy(), ref["result"].cpu().numpy()
+        print(n, mode, float(np.max(np.abs(g - r))), flush=True)
+        assert g.shape == r.shape and np.max(np.abs(g - r)) <= 1e-12 * max(1.0, np.max(np.abs(r)))
+        finals[mode] = r
+    again = hilbert.run(hm, mode="final_rho", rho0_values=finals["final_rho"], **kw)
+    ref = hilbert.run(hm, mode="final_rho", rho0_values=finals["final_rho"], valu=True, **kw)
+    assert np.max(np.abs(again["result"].cpu().numpy() - ref["result"].cpu().numpy())) <= 1e-12
+
+
+@pytest.mark.parametrize("n,steps", [(8, 200), (16, 100)])
+def test_fp64_matrix_core_final_state_matches_oracle(gpu, n, steps):
+    w = workloads.random_n_level(n, seed=SEED, n_eps=2, n_amp=3, n_t=steps + 1,
+                                 t_end=0.01 * steps)
+    eps_v, amp_v = w["var_arrays"].values()
+    rho = mesolve_2D(w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"],
+                     var_arrays=w["var_arrays"], liouville="mfma", fp64=True,
+                     output_mode="final_rho")
+    m = models.random_n_level(n, SEED, eps_v, amp_v)
+    fin, _ = integrate.rk4(m.f, m.ground_state(), 0.01, steps, mode="final_rho")
+    assert rho.shape == (2, 3, n * n - 1)
+    assert np.max(np.abs(rho.reshape(6, -1) - fin)) <= 1e-12
+
+
+def test_fp64_matrix_core_chain_matches_per_lane_kernel(gpu):
+    # relaxation (single-entry jumps) and dephasing (diagonal collapse) of the 8-level chain
+    w = workloads.chain(8, n_eps=5, n_amp=3, n_t=401, t_end=20.0)
+    kw = dict(var_arrays=w["var_arrays"], fp64=True)
+    args = (w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"])
+    for mode in ("mean", "final_rho"):
+        mfma = mesolve_2D(*args, liouville="mfma", output_mode=mode, **kw)
+        lanes = mesolve_2D(*args, liouville="lanes", output_mode=mode, **kw)
+        assert np.max(np.abs(mfma - lanes)) <= 1e-12 * max(1.0, np.max(np.abs(lanes)))
