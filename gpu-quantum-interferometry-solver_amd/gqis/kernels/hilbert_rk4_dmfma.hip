@@ -28,6 +28,9 @@ typedef double gq_f64x4 __attribute__((ext_vector_type(4)));
 #define HV (HN / 2)              // state values per lane and column tile
 #define HWAVES (GQ_BLOCK / 64)
 #define HLD (HN + 1)             // padded LDS row
+// N = 32 keeps 4 x 32 doubles of RK4 state per lane: there the Y parts and dephasing weights
+// of the second RHS pass are read back from LDS instead of held in registers
+#define HLOWREG (HN == 32)
 
 extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4_dmfma)(const gq_sweep_args a) {
     __shared__ double s_apack[HTR * HTR * 4 * 64];
@@ -107,7 +110,9 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4_
     // Per-lane dissipator constants: dephasing weights of the lane's elements and, for the lane
     // holding a diagonal element (at most one: column tile ctd, slot md), its jump list and
     // observable weight, so the RHS has no per-element branch.
+#if !HLOWREG
     double depw[HTC][HQ];
+#endif
     int md[HTC];
     int ctd = -1, jd = 0;
 #pragma unroll
@@ -116,7 +121,9 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4_
 #pragma unroll
         for (int u = 0; u < HQ; ++u) {
             const int i = g * HQ + u;
+#if !HLOWREG
             depw[ct][u] = s_dep[i * HN + jc[ct]];
+#endif
             if (i == jc[ct]) {
                 md[ct] = u;
                 ctd = ct;
@@ -183,7 +190,9 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4_
                         acc[to][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, x[ct][4 * T + r],
                                                                            acc[to][ct], 0, 0, 0);
                 }
+#if !HLOWREG
         double yr[HTC][HQ], yi[HTC][HQ];
+#endif
 #pragma unroll
         for (int ct = 0; ct < HTC; ++ct) {
             const int j = jc[ct];
@@ -192,11 +201,15 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4_
                 const double xr = acc[u >> 2][ct][u & 3];
                 const double xi = acc[(u + HQ) >> 2][ct][(u + HQ) & 3];
                 // Y = -i X - i v rho
-                yr[ct][u] = fma(v[u], x[ct][u + HQ], xi);
-                yi[ct][u] = -fma(v[u], x[ct][u], xr);
+                const double ry = fma(v[u], x[ct][u + HQ], xi);
+                const double iy = -fma(v[u], x[ct][u], xr);
+#if !HLOWREG
+                yr[ct][u] = ry;
+                yi[ct][u] = iy;
+#endif
                 const int i = g * HQ + u;
-                ybuf_r[i * HLD + j] = yr[ct][u];
-                ybuf_i[i * HLD + j] = yi[ct][u];
+                ybuf_r[i * HLD + j] = ry;
+                ybuf_i[i * HLD + j] = iy;
             }
         }
         if (ctd >= 0) dbuf[jd] = diag_of(x);
@@ -210,10 +223,17 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4_
 #pragma unroll
             for (int u = 0; u < HQ; ++u) {
                 const int i = g * HQ + u;
-                double dr = yr[ct][u] + ybuf_r[j * HLD + i];
-                double di = yi[ct][u] - ybuf_i[j * HLD + i];
-                dr = fma(depw[ct][u], x[ct][u], dr);
-                di = fma(depw[ct][u], x[ct][u + HQ], di);
+#if HLOWREG
+                const double ry = ybuf_r[i * HLD + j], iy = ybuf_i[i * HLD + j];
+                const double dw = s_dep[i * HN + j];
+#else
+                const double ry = yr[ct][u], iy = yi[ct][u];
+                const double dw = depw[ct][u];
+#endif
+                double dr = ry + ybuf_r[j * HLD + i];
+                double di = iy - ybuf_i[j * HLD + i];
+                dr = fma(dw, x[ct][u], dr);
+                di = fma(dw, x[ct][u + HQ], di);
                 k[ct][u] = (u == md[ct]) ? dr + jsum : dr;
                 k[ct][u + HQ] = di;
             }
