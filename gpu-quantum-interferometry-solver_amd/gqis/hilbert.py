@@ -348,9 +348,12 @@ _SOURCES = {}
 
 
 def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps, warmup_steps,
-        block_size=256, collect=False, options=(), fp64=False, split=None, valu=False):
+        block_size=256, collect=False, options=(), fp64=False, split=None, valu=False,
+        prepare=False):
     """Launch the Hilbert-space kernel; returns the same dict as ``device.run_sweep``
-    (``valu``: the FP64 VALU form instead of the FP64 matrix-core form)."""
+    (``valu``: the FP64 VALU form instead of the FP64 matrix-core form).  ``prepare``: build
+    buffers and kernels but launch nothing; returns the launch plan of ``device.run_sweep``
+    (for ``prepare_sweep``), whose ``set_consts`` rewrites the runtime constants in place."""
     import torch
     from .device import _axis, _initial_states
     setup_start = time.time()
@@ -431,9 +434,33 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
         module, _, _ = ctx.compile(autotune.with_block(src, block_size), options)
         ctx.read_status(status, stream, clear=True)
         tune_s = time.time() - t0
+    per_block = (block_size // 64) * points_per_wave(m.n, fp64, valu)
+    if prepare:
+        launches = []
+        if npts > 0:
+            if n_atoms > 0:
+                launches.append((codegen.kernel_name("gq_rk4_time_table", fp64),
+                                 (rows + 255) // 256, 256))
+            launches.append((kname, (npts + per_block - 1) // per_block, block_size))
+        n_rc = len(m.runtime_syms)
+
+        def set_consts(values):
+            # the runtime constants lead the constant buffer ("RC"); the packed operators
+            # do not depend on them
+            cbuf[:n_rc].copy_(torch.as_tensor(np.asarray(values, np.float64), dtype=rdt,
+                                              device=dev))
+        return {"ctx": ctx, "module": module, "args": a, "launches": launches, "fp64": fp64,
+                "mode": mode, "nx": nx, "ny": ny, "size": d, "num_trace": 0,
+                "state_trace": False, "adaptive": False, "set_consts": set_consts,
+                "runtime_syms": tuple(m.runtime_syms),
+                "buffers": {"px": px, "py": py, "consts": cbuf, "rho0": rho0, "result": result,
+                            "trace": None, "status": status, "cursor": None, "table": table,
+                            "steps": None},
+                "info": {"build_s": build_s, "cached_kernel": cached,
+                         "block_size": int(block_size), "lanes_per_point": 1,
+                         "autotune_s": tune_s, "autotune": tune_times, "kernel": kname}}
     timer = runtime.EventTimer(ctx, stream) if collect else None
     setup_s = time.time() - setup_start
-    per_block = (block_size // 64) * points_per_wave(m.n, fp64, valu)
     if npts > 0:
         if timer:
             timer.record_start()

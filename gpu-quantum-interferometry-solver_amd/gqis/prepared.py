@@ -1,4 +1,5 @@
-"""A sweep kept resident on the device for repeated runs, replayed as one HIP graph.
+"""A sweep kept resident on the device for repeated runs, replayed as one HIP graph (per-point
+kernels and the matrix-core Hilbert-space kernels alike).
 
 Fitting loops call the same sweep many times with new sweep values, runtime constants or
 initial states (the reference re-enters ``mesolve_2D`` each time, ``gqis/solver.py:1712-2391``:
@@ -58,9 +59,19 @@ class PreparedSweep:
             b[key].copy_(new)
 
     def _set_consts(self, runtime_consts):
-        from .solver import _runtime_const_values
         merged = dict(self.runtime_consts)
         merged.update(runtime_consts)
+        if "set_consts" in self.plan:
+            # matrix-core plan: its runtime constants lead a buffer of packed operators
+            syms = self.plan["runtime_syms"]
+            if not syms:
+                raise ValueError("This model has no runtime constants.")
+            values = dict(self.plan["runtime_defaults"])
+            values.update(merged)
+            self.plan["set_consts"]([float(values[s]) for s in syms])
+            self.runtime_consts = merged
+            return
+        from .solver import _runtime_const_values
         vals = _runtime_const_values(self.compiled, merged, self.const_values, self.use_defaults)
         if not vals:
             raise ValueError("This model has no runtime constants.")

@@ -443,17 +443,27 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
         raise ValueError(f"liouville='{liouville}' (split-precision matrix-core products) is an "
                          "FP32-mode option.")
     hilbert_out = None
-    if _prepare and liouville.startswith("mfma"):
-        raise ValueError("prepare_sweep runs the per-point kernels; use liouville='auto' or "
-                         "'lanes'.")
-    if (compiled_rhs is None and _ode_size is None and liouville != "lanes" and not _prepare
+    if (compiled_rhs is None and _ode_size is None and liouville != "lanes"
             and not (lanes_per_point != "auto" and lanes_per_point > 1
                      and liouville == "auto")):
         hilbert_out = _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values,
                                    runtime_consts, keep_symbolic_consts, auto_runtime_consts,
                                    rho0, rho0_values, spec, mode, fp64, return_time_trace,
                                    liouville, dt, t_in, num_steps, warmup_steps, block_size,
-                                   collect, tuple(hiprtc_options) + _as_tuple(nvrtc_options))
+                                   collect, tuple(hiprtc_options) + _as_tuple(nvrtc_options),
+                                   prepare=_prepare)
+    if hilbert_out is not None and _prepare:
+        from .prepared import PreparedSweep
+        plan, stage_s, rhs_status = hilbert_out
+
+        def finish(outs, *, return_device, ignore_non_finite_output, total_start):
+            return _finish(outs, stage_s, rhs_status, spec, solver_frequency, unroll, False,
+                           False, return_device, False, False, ignore_non_finite_output,
+                           None, 0, 1, 0, fp64, t_in, dt, total_start)
+        return PreparedSweep(plan, compiled=None, spec=spec, const_values=const_values,
+                             runtime_consts=runtime_consts, use_defaults=False,
+                             sweep_symbols=tuple(var_arrays),
+                             uses_rho0_values=uses_rho0_values, finish=finish)
     if hilbert_out is not None:
         outputs, stage_s, rhs_status = hilbert_out
         return _finish(outputs, stage_s, rhs_status, spec, solver_frequency, unroll, collect,
@@ -607,8 +617,10 @@ SPLIT_F16_MAX_ENTRY = 1.0e4     # largest |H_eff| entry routed to FP16 hi/lo pro
 
 def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, runtime_consts,
                  keep_symbolic_consts, auto_runtime_consts, rho0, rho0_values, spec, mode, fp64,
-                 trace, liouville, dt, t_in, num_steps, warmup_steps, block_size, collect, options):
-    """Run the matrix-core (Hilbert-space) path when it applies; None otherwise."""
+                 trace, liouville, dt, t_in, num_steps, warmup_steps, block_size, collect, options,
+                 prepare=False):
+    """Run the matrix-core (Hilbert-space) path when it applies; None otherwise (``prepare``:
+    return its launch plan instead of running it)."""
     from . import hilbert
     n = H.shape[0] if hasattr(H, "shape") else 0
     forced = liouville.startswith("mfma")
@@ -655,7 +667,9 @@ def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, run
     outputs = hilbert.run(hm, mode=mode, var_arrays=var_arrays, rho0_values=rho0_values,
                           runtime_values=rt, dt=dt, t_in=t_in, num_steps=num_steps,
                           warmup_steps=warmup_steps, block_size=block_size, collect=collect,
-                          options=options, fp64=fp64, split=split)
+                          options=options, fp64=fp64, split=split, prepare=prepare)
+    if prepare:
+        outputs["runtime_defaults"] = dict(model["defaults"])
     stage_s = {"ode_prepare_s": prep, "ode_optimize_s": 0.0, "emit_s": 0.0}
     return outputs, stage_s, "miss"
 

@@ -207,7 +207,7 @@ def test_prepare_sweep_argument_checks():
         prepare_sweep(*args, var_arrays=w["var_arrays"], calibrate_timesteps=True)
     with pytest.raises(ValueError, match="not available for a prepared sweep"):
         prepare_sweep(*args, var_arrays=w["var_arrays"], return_timing_info=True)
-    with pytest.raises(ValueError, match="per-point kernels"):
+    with pytest.raises(ValueError, match="does not apply: N = 2"):
         prepare_sweep(*args, var_arrays=w["var_arrays"], liouville="mfma")
     with pytest.raises(ValueError, match="output_mode"):
         prepare_sweep(*args, var_arrays=w["var_arrays"], output_mode="mean_state")

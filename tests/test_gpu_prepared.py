@@ -115,3 +115,44 @@ def test_replay_host_time_is_below_the_full_call(gpu):
     replay = (time.perf_counter() - t0) / 20
     print(f"per call: mesolve_2D {full * 1e3:.3f} ms, replay {replay * 1e3:.3f} ms", flush=True)
     assert replay < full
+
+
+@pytest.mark.parametrize("n,fp64", [(8, True), (16, False)])
+def test_matrix_core_sweep_replays(gpu, n, fp64):
+    # the Hilbert-space kernels (FP64 matrix cores at N = 8, FP16 hi/lo products at N = 16)
+    # behind prepare_sweep: new axes, a new drive frequency (a runtime constant inside the
+    # tabulated time term) and explicit initial states, each replay equal to an eager call
+    omega = sp.Symbol("omega", real=True)
+    w = workloads.random_n_level(n, n_eps=6, n_amp=5, n_t=201, t_end=2.0)
+    drive = {k: v.subs(sp.Symbol("t", real=True), omega * sp.Symbol("t", real=True))
+             for k, v in w["Drive"].items()}
+    args = (w["H"], drive, w["Col_Ops"], w["mean_operator"], w["tlist"])
+    e, a = list(w["var_arrays"])
+    var = dict(w["var_arrays"])
+    kw = dict(fp64=fp64, runtime_consts={omega: 1.0})
+    prep = prepare_sweep(*args, var_arrays=var, **kw)
+    assert prep.plan["info"]["kernel"].startswith("gq_hilbert_rk4")
+    state_var, state_consts = dict(var), {omega: 1.0}
+    cases = [({}, {}), ({e: np.linspace(-2, 2, 6), a: np.linspace(1, 2, 5)}, {}),
+             ({}, {omega: 1.7}), ({}, {})]
+    for new_var, new_consts in cases:
+        got = prep.run(var_arrays=new_var or None, runtime_consts=new_consts or None)
+        state_var.update(new_var)
+        state_consts.update(new_consts)
+        ref = mesolve_2D(*args, var_arrays=state_var, fp64=fp64, runtime_consts=state_consts)
+        np.testing.assert_array_equal(got, ref)
+        _busy()
+    prep.close()
+    rho = mesolve_2D(*args, var_arrays=state_var, fp64=fp64, runtime_consts=state_consts,
+                     output_mode="final_rho")
+    prep = prepare_sweep(*args, var_arrays=state_var, output_mode="final_rho",
+                         rho0_values=rho, **dict(kw, runtime_consts=state_consts))
+    got = prep.run()
+    ref = mesolve_2D(*args, var_arrays=state_var, fp64=fp64, runtime_consts=state_consts,
+                     output_mode="final_rho", rho0_values=rho)
+    np.testing.assert_array_equal(got, ref)
+    got = prep.run(rho0_values=ref)
+    ref2 = mesolve_2D(*args, var_arrays=state_var, fp64=fp64, runtime_consts=state_consts,
+                      output_mode="final_rho", rho0_values=ref)
+    np.testing.assert_array_equal(got, ref2)
+    prep.close()
