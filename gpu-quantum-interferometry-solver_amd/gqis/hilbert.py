@@ -143,6 +143,17 @@ def pack_a(m):
     return out
 
 
+def dense_enough(m):
+    """Whether the dense matrix-core product beats the per-point kernels below 16 levels: their
+    work grows with the nonzeros of H_eff (a reduced element couples to the elements its row and
+    column of H_eff reach), the product's does not.  Measured in FP64 at N = 8 (256^2, 2000 RK4
+    steps): dense random H_eff 52 ms on the matrix cores against 76 ms with 8 lanes per point;
+    the tridiagonal chain (3 nonzeros per row) runs about 3x faster per point on the lanes.
+    Break-even is about 5.5 nonzeros per row at N = 8."""
+    per_row = np.count_nonzero(np.abs(m.h_eff) > 0) / m.n
+    return per_row >= 0.75 * m.n
+
+
 def dmfma_order(n):
     """Logical Rbig row (Re rho_i: i, Im rho_i: n + i) held at each row 16 T + 4 r + g of the
     FP64 matrix-core kernel's tiles: slot m = 4 T + r of lane group g is Re rho_{g n/4 + m} for

@@ -216,9 +216,9 @@ class SweepServer:
                                       flags[1:])
 
     def _hilbert_model(self, r):
-        """Matrix-core model of a Lindblad request with N >= 16 (N >= 8 in FP64) whose
-        operators fit the Hilbert-space kernel, else None (smaller models run faster on the
-        per-point kernels); the same choice mesolve_2D makes."""
+        """Matrix-core model of a Lindblad request with N >= 16 (N >= 8 with a dense H_eff in
+        FP64) whose operators fit the Hilbert-space kernel, else None (smaller or sparser models
+        run faster on the per-point kernels); the same choice mesolve_2D makes."""
         if r.compiled.kind != "lindblad" or r.output_mode not in ("mean", "final", "final_rho"):
             return None
         n = int(round(np.sqrt(r.compiled.state_size + 1)))
@@ -229,7 +229,10 @@ class SweepServer:
             return None
         from . import hilbert
         from .solver import _drive_map
-        return hilbert.extract(src, _drive_map(src["Drive"]))
+        hm = hilbert.extract(src, _drive_map(src["Drive"]))
+        if hm is not None and n < 16 and not hilbert.dense_enough(hm):
+            return None
+        return hm
 
     def warm(self, requests):
         """Compile every kernel variant the requests need (before the stream starts)."""

@@ -245,7 +245,7 @@ def mesolve_2D(H, Drive=None, Col_Ops=None, mean_operator=None, tlist=None,
     ``liouville`` ("auto", "lanes", "mfma", or "mfma_f16" / "mfma_bf16": the commutator product
     as three FP16 / BF16 matrix-core products of hi/lo parts with FP32 accumulation; the
     Hilbert-space path for N = 8, 16, 32 on the FP32 or FP64 matrix cores, used automatically
-    in RK4 sweeps whose operators fit it from N = 16, and from N = 8 in FP64).  ``return_device=True`` returns torch tensors on
+    in RK4 sweeps whose operators fit it from N = 16, and from N = 8 in FP64 for a dense H).  ``return_device=True`` returns torch tensors on
     the GPU.  ``H`` may be a ``CompiledRHS`` from ``compile_lindblad_rhs``.
 
     Returns ``(num_X, num_Y)`` complex for ``"mean"``/``"final"``, ``(num_X, num_Y, N*N-1)``
@@ -627,7 +627,8 @@ def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, run
     split = {"mfma_f16": "f16", "mfma_bf16": "bf16"}.get(liouville)
     # measured on gfx950 (random N-level, 256^2, 2000 steps): from 16 levels the matrix-core
     # forms are fastest in both precisions; at 8 levels in FP64 the FP64 matrix-core form (52 ms)
-    # beats 8 lanes per point (77 ms), while FP32 stays on the per-point kernels
+    # beats 8 lanes per point (77 ms) when H_eff is dense (hilbert.dense_enough), while FP32
+    # stays on the per-point kernels
     wanted = forced or (liouville == "auto" and (n >= 16 or (fp64 and n >= 8)))
     reasons = []
     if n not in hilbert.SUPPORTED_N:
@@ -656,6 +657,8 @@ def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, run
                              "terms, constant diagonal or single-entry collapse operators and a "
                              "diagonal observable.")
         return None
+    if liouville == "auto" and n < 16 and not hilbert.dense_enough(hm):
+        return None          # a sparse H_eff runs faster on the per-point kernels
     values = dict(model["defaults"])
     values.update(runtime_consts or {})
     rt = [float(values[s]) for s in model["runtime_const_syms"]]

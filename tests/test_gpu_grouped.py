@@ -160,13 +160,14 @@ def test_autotuner_considers_grouped_adaptive_kernels(gpu):
 
 
 def test_auto_lanes_per_point_default(gpu):
-    # 63 states on the per-point kernels: the default takes 8 lanes per point; 15 states stay
-    # on one lane.  (The default FP64 path for this model is the matrix-core one.)
+    # 63 states: the default takes 8 lanes per point for the tridiagonal chain; 15 states stay
+    # on one lane; a dense 8-level H in FP64 takes the matrix-core path instead
     args, var = _chain8(n_eps=8, n_amp=8, n_t=101, t_end=5.0)
-    _, info = mesolve_2D(*args, var_arrays=var, fp64=True, liouville="lanes",
-                         return_timing_info=True)
-    assert info["lanes_per_point"] == 8 and info["kernel"] == "gq_rk4_grouped_f64"
     _, info = mesolve_2D(*args, var_arrays=var, fp64=True, return_timing_info=True)
+    assert info["lanes_per_point"] == 8 and info["kernel"] == "gq_rk4_grouped_f64"
+    w = workloads.random_n_level(8, n_eps=8, n_amp=8, n_t=101, t_end=1.0)
+    _, info = mesolve_2D(w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"],
+                         var_arrays=w["var_arrays"], fp64=True, return_timing_info=True)
     assert info["kernel"] == "gq_hilbert_rk4_dmfma_f64"
     w = workloads.four_level(n_eps=8, n_amp=8, n_t=101, t_end=5.0)
     _, info = mesolve_2D(w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"],
