@@ -12,6 +12,11 @@
 // decision, is bitwise the same in all waves, so all waves take the same branches and reach
 // the same barriers.  Points are assigned one per lane for the whole integration (no work
 // stealing across groups).
+//
+// The FP64 time atoms of a step's stage times (about 95 instructions per sin / cos) are the
+// same in every wave: wave (s - 1) % GQ_G evaluates those of stage s at the step start and
+// writes them to LDS before the step's first barrier, after which every wave reads them
+// (s_tt, double-buffered by attempt parity).  The values are those of the direct evaluation.
 
 #if GQ_STATE_FP64
 typedef double st_t;
@@ -33,6 +38,19 @@ __device__ __forceinline__ void gq_stage_at(double t_abs, real px, real py,
     for (int i = 0; i < GQ_NATOMS_ALLOC; ++i) TT[i] = (real)TTd[i];
     gq_grp<GI>::stage(px, py, C, Sg, TT, Gg);
 }
+
+// stage values from the atoms of one stage time shared in LDS (tt[k * 64 + lane])
+template <int GI>
+__device__ __forceinline__ void gq_stage_lds(const double* __restrict__ tt, int lane, real px,
+                                             real py, const real* __restrict__ C,
+                                             const real* __restrict__ Sg, real* __restrict__ Gg) {
+    real TT[GQ_NATOMS_ALLOC];
+#pragma unroll
+    for (int i = 0; i < GQ_NATOMS; ++i) TT[i] = (real)tt[i * 64 + lane];
+    gq_grp<GI>::stage(px, py, C, Sg, TT, Gg);
+}
+
+#define GQ_TT_STRIDE (GQ_NATOMS_ALLOC * 64)
 
 template <int GI>
 __device__ __forceinline__ void gq_share(real* __restrict__ Y, const st_t* __restrict__ v,
@@ -57,6 +75,7 @@ __device__ __forceinline__ double gq_group_sum(double* __restrict__ red, double 
 template <int GI>
 __device__ __forceinline__ void gq_adaptive_body(const gq_sweep_args& a,
                                                  real (*s_y)[GQ_D * 64], double* s_red,
+                                                 double* s_tt,
                                                  const int lane, const long long q,
                                                  const bool valid, const real px, const real py,
                                                  const gq_point& pt) {
@@ -108,12 +127,26 @@ __device__ __forceinline__ void gq_adaptive_body(const gq_sweep_args& a,
     for (int r = 0; r < P::NR; ++r) savg[r] = 0;
 #endif
 
+    int attempt = 0;   // the same in every wave: selects the s_tt half of this attempt
     // lanes with nothing to do (past the end, or zero steps) take part in the barriers only
     while (__any(live)) {
         const double remaining = dt_out - tau;
         const bool last = h * 1.01 >= remaining;
         const double hs = last ? remaining : h;
         const double t0 = (double)step * dt_out + tau;
+        double* __restrict__ tt = s_tt + (attempt & 1) * GQ_NS * GQ_TT_STRIDE;
+        ++attempt;
+#if GQ_NATOMS > 0
+#pragma unroll
+        for (int s = 1; s < GQ_NS; ++s) {
+            if ((s - 1) % GQ_G == GI) {
+                double TTd[GQ_NATOMS_ALLOC];
+                gq_atoms_f64(a.t_in + (t0 + GQ_C[s] * hs), C, TTd);
+#pragma unroll
+                for (int i = 0; i < GQ_NATOMS; ++i) tt[s * GQ_TT_STRIDE + i * 64 + lane] = TTd[i];
+            }
+        }
+#endif
         real k[GQ_NS][P::NR];
 #pragma unroll
         for (int r = 0; r < P::NR; ++r) k[0][r] = k0[r];
@@ -128,9 +161,9 @@ __device__ __forceinline__ void gq_adaptive_body(const gq_sweep_args& a,
                     if (GQ_A[s][j] != 0.0) acc = fma((st_t)GQ_A[s][j], (st_t)k[j][r], acc);
                 ys[r] = fma((st_t)hs, acc, y[r]);
             }
-            gq_stage_at<GI>(a.t_in + (t0 + GQ_C[s] * hs), px, py, C, Sg, Gg);
             gq_share<GI>(s_y[buf], ys, lane);
-            __syncthreads();
+            __syncthreads();   // the first one of the step also publishes tt
+            gq_stage_lds<GI>(tt + s * GQ_TT_STRIDE, lane, px, py, C, Sg, Gg);
             P::rhs(s_y[buf], lane, px, py, C, Sg, Gg, k[s]);
             buf ^= 1;
         }
@@ -331,6 +364,7 @@ __device__ __forceinline__ void gq_adaptive_body(const gq_sweep_args& a,
 extern "C" __global__ __launch_bounds__(GQ_GW) void GQ_KERNEL(gq_adaptive_grouped)(const gq_sweep_args a) {
     __shared__ real s_y[2][GQ_D * 64];
     __shared__ double s_red[GQ_G * 64];
+    __shared__ double s_tt[2 * GQ_NS * GQ_TT_STRIDE];
     const int g = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = (int)(threadIdx.x & 63u);
     const long long q_raw = (long long)blockIdx.x * 64 + lane;
@@ -339,5 +373,5 @@ extern "C" __global__ __launch_bounds__(GQ_GW) void GQ_KERNEL(gq_adaptive_groupe
     const gq_point pt = gq_locate(a, q);
     const real px = ((const real*)a.par_x)[pt.ix];
     const real py = ((const real*)a.par_y)[pt.iy];
-    GQ_GROUP_DISPATCH(g, gq_adaptive_body, (a, s_y, s_red, lane, q, valid, px, py, pt))
+    GQ_GROUP_DISPATCH(g, gq_adaptive_body, (a, s_y, s_red, s_tt, lane, q, valid, px, py, pt))
 }

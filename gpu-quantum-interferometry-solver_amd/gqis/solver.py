@@ -707,14 +707,17 @@ GROUPED_TEMPLATES.update({name: "grouped_tableau.hip" for name in
                            "ab5")})
 
 
-def grouped_applies(spec, d, fp64, groups):
+def grouped_applies(spec, d, fp64, groups, natoms=1):
     """A grouped kernel exists for this solver, the state splits into ``groups`` row blocks and
     the double-buffered LDS copy of the state (2 x d x 64 reals, plus the adaptive kernels'
-    error-norm partials) fits one compute unit."""
+    error-norm partials and shared stage-time atoms, up to 12 stages x ``natoms``, double
+    buffered) fits one compute unit."""
     if spec.name not in GROUPED_TEMPLATES or groups < 2:
         return False
     rw = -(-d // groups)
-    lds = 2 * d * 64 * (8 if fp64 else 4) + (groups * 64 * 8 if spec.adaptive else 0)
+    lds = 2 * d * 64 * (8 if fp64 else 4)
+    if spec.adaptive:
+        lds += groups * 64 * 8 + 2 * 12 * max(1, natoms) * 64 * 8
     return (groups - 1) * rw < d and lds <= LDS_BYTES_PER_CU
 
 
