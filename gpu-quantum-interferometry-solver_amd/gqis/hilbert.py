@@ -297,7 +297,7 @@ def block_for(m, fp64, block_size=256, valu=False):
     return 128 if (fp64 and valu and m.n == 32) else block_size
 
 
-def emit(m, offs, mode, block_size=256, fp64=False, split=None, valu=False):
+def emit(m, offs, mode, block_size=256, fp64=False, split=None, valu=False, trace=False):
     """hiprtc source of the Hilbert-space RK4 kernel for model m (``split``: the commutator
     GEMM as three FP16 or BF16 products of hi/lo parts with FP32 accumulation; FP64 runs on the
     FP64 matrix cores, or with ``valu`` as VALU dot products)."""
@@ -338,7 +338,8 @@ def emit(m, offs, mode, block_size=256, fp64=False, split=None, valu=False):
         header = fh.read()
     defines = {"GQ_HILBERT": 1, "GQ_OUT_MEAN": int(mode == "mean"),
                "GQ_OUT_FINAL": int(mode == "final"), "GQ_OUT_FINAL_RHO": int(mode == "final_rho"),
-               "GQ_OUT_MEAN_STATE": 0, "GQ_TRACE": 0, "GQ_RHO0_VALUES": "GQ_RHO0_FROM_ARGS",
+               "GQ_OUT_MEAN_STATE": 0, "GQ_TRACE": int(bool(trace)),
+               "GQ_RHO0_VALUES": "GQ_RHO0_FROM_ARGS",
                "GQ_MFMA_SPLIT": split_kind}
     parts = [header, codegen.prelude(fp64, True, block_size, defines), model_src,
              codegen.read_kernel_template("common.hip"), codegen.read_kernel_template("rk4_table.hip"),
@@ -360,7 +361,7 @@ _SOURCES = {}
 
 def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps, warmup_steps,
         block_size=256, collect=False, options=(), fp64=False, split=None, valu=False,
-        prepare=False):
+        prepare=False, num_trace=0, trace_stride=1, trace_offset=0):
     """Launch the Hilbert-space kernel; returns the same dict as ``device.run_sweep``
     (``valu``: the FP64 VALU form instead of the FP64 matrix-core form).  ``prepare``: build
     buffers and kernels but launch nothing; returns the launch plan of ``device.run_sweep``
@@ -378,10 +379,12 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
     tune = block_size == "auto"
     if tune:
         block_size = block_for(m, fp64, 256, valu)
-    skey = (m.key, mode, block_size, uses_rho0, tuple(options), bool(fp64), split, bool(valu))
+    trace = num_trace > 0
+    skey = (m.key, mode, block_size, uses_rho0, tuple(options), bool(fp64), split, bool(valu),
+            trace)
     src = _SOURCES.get(skey)
     if src is None:
-        src = emit(m, offs, mode, block_size, fp64, split, valu).replace(
+        src = emit(m, offs, mode, block_size, fp64, split, valu, trace).replace(
             "GQ_RHO0_FROM_ARGS", "1" if uses_rho0 else "0")
         _SOURCES[skey] = src
     options = codegen.HIPRTC_BASE_OPTIONS + tuple(options)
@@ -401,6 +404,9 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
         result = torch.empty(max(npts, 1), dtype=torch.complex128 if fp64 else torch.complex64,
                              device=dev)
     status = torch.zeros(1, dtype=torch.int32, device=dev)
+    trace_buf = (torch.zeros((num_trace, max(npts, 1)),
+                             dtype=torch.complex128 if fp64 else torch.complex64, device=dev)
+                 if trace else None)
     rows = 2 * num_steps + 1
     n_atoms = int(src.split("#define GQ_NATOMS ")[1].split()[0])
     table = torch.empty(max(1, rows * max(1, n_atoms)), dtype=rdt, device=dev)
@@ -409,7 +415,9 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
     a.num_x, a.num_y = nx, ny
     a.point_begin, a.point_count = 0, npts
     a.num_steps, a.warmup_steps = int(num_steps), int(warmup_steps)
-    a.trace_stride, a.trace_point_stride = 1, npts
+    a.trace_stride, a.trace_offset, a.num_trace = int(trace_stride), int(trace_offset), int(num_trace)
+    a.trace_point_stride = npts
+    a.trace = trace_buf.data_ptr() if trace_buf is not None else None
     a.par_x, a.par_y, a.consts = px.data_ptr(), py.data_ptr(), cbuf.data_ptr()
     a.rho0 = rho0.data_ptr() if rho0 is not None else None
     a.time_table = table.data_ptr()
@@ -432,7 +440,7 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
 
         def trial(mod, block):
             t = runtime.SweepArgs.from_buffer_copy(a)
-            t.num_steps, t.warmup_steps = trial_steps, 0
+            t.num_steps, t.warmup_steps, t.num_trace, t.trace = trial_steps, 0, 0, None
             per = (block // 64) * points_per_wave(m.n, fp64, valu)
             return _timed_launches(ctx, mod, kname, (npts + per - 1) // per, block, t, stream,
                                    spare)
@@ -461,12 +469,12 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
             cbuf[:n_rc].copy_(torch.as_tensor(np.asarray(values, np.float64), dtype=rdt,
                                               device=dev))
         return {"ctx": ctx, "module": module, "args": a, "launches": launches, "fp64": fp64,
-                "mode": mode, "nx": nx, "ny": ny, "size": d, "num_trace": 0,
+                "mode": mode, "nx": nx, "ny": ny, "size": d, "num_trace": int(num_trace),
                 "state_trace": False, "adaptive": False, "set_consts": set_consts,
                 "runtime_syms": tuple(m.runtime_syms),
                 "buffers": {"px": px, "py": py, "consts": cbuf, "rho0": rho0, "result": result,
-                            "trace": None, "status": status, "cursor": None, "table": table,
-                            "steps": None},
+                            "trace": trace_buf, "status": status, "cursor": None,
+                            "table": table, "steps": None},
                 "info": {"build_s": build_s, "cached_kernel": cached,
                          "block_size": int(block_size), "lanes_per_point": 1,
                          "autotune_s": tune_s, "autotune": tune_times, "kernel": kname}}
@@ -489,7 +497,9 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
         out = result[:, :npts].t().reshape(nx, ny, d).contiguous()
     else:
         out = result[:npts].reshape(nx, ny)
-    return {"result": out, "trace": None, "status": flags, "gpu_s": gpu_s, "build_s": build_s,
+    tr = (trace_buf[:, :npts].t().reshape(nx, ny, num_trace).contiguous()
+          if trace_buf is not None else None)
+    return {"result": out, "trace": tr, "status": flags, "gpu_s": gpu_s, "build_s": build_s,
             "cached_kernel": cached, "setup_s": setup_s, "adaptive_steps": None,
             "block_size": int(block_size), "autotune_s": tune_s, "autotune": tune_times,
             "kernel": kname}

@@ -294,6 +294,19 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     };
 
+    // sum over the 2N lanes of each point (valid in the lanes with h == 0, j == 0); every lane
+    // of the wavefront takes part
+    auto point_sum = [&](double part) -> double {
+        s_red[wave][lane] = part;
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        double s = 0.0;
+        if (h == 0 && j == 0)
+            for (int l = 0; l < HN; ++l) s += s_red[wave][pw * HN + l] + s_red[wave][32 + pw * HN + l];
+        return s;
+    };
+    bool bad_trace = false;
+    (void)bad_trace;
+
     stage_values(0);
     real k[HN];
     rhs(y, k);
@@ -335,6 +348,23 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
             avg += (double)(od * yd);
         }
 #endif
+#if GQ_TRACE
+        // observable trace after steps offset + k * stride (gqis/solver.py:2177-2193)
+        if (step >= a.trace_offset && (step - a.trace_offset) % a.trace_stride == 0) {
+            const int kk = (step - a.trace_offset) / a.trace_stride;
+            if (kk < a.num_trace) {
+                real yd = R(0.0);
+#pragma unroll
+                for (int m = 0; m < HM; ++m) yd = (m == mdiag) ? y[m] : yd;
+                const double s = point_sum((double)(od * yd));
+                if (h == 0 && j == 0 && valid) {
+                    const real r = (real)s;
+                    bad_trace |= !gq_finite(r);
+                    gq_store_complex(a.trace, (long long)kk * a.trace_point_stride + q, r, R(0.0));
+                }
+            }
+        }
+#endif
     }
 
     bool bad = false;
@@ -348,11 +378,8 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
     }
 #endif
     // sum the partial traces of the 2N lanes of each point
-    s_red[wave][lane] = avg;
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    double s = point_sum(avg);
     if (h == 0 && j == 0) {
-        double s = 0.0;
-        for (int l = 0; l < HN; ++l) s += s_red[wave][pw * HN + l] + s_red[wave][32 + pw * HN + l];
 #if GQ_OUT_MEAN
         s /= (double)(num_steps - a.warmup_steps);
 #endif
@@ -381,4 +408,7 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
     }
 #endif
     gq_flag(a.status, bad, GQ_STATUS_NONFINITE_RESULT);
+#if GQ_TRACE
+    gq_flag(a.status, bad_trace, GQ_STATUS_NONFINITE_TRACE);
+#endif
 }

@@ -241,6 +241,21 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4_
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     };
 
+    // sum over the lanes of each point (HPW points per wave: lanes with c / 8 == pw for N = 8,
+    // all 64 lanes otherwise), valid in the point's leader lane; every lane takes part
+    const bool leader = (g == 0) && (HPW == 1 ? c == 0 : (c & 7) == 0);
+    auto point_sum = [&](double part) -> double {
+        s_red[wave][lane] = part;
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        double s = 0.0;
+        if (leader)
+            for (int l = 0; l < 64; ++l)
+                if (HPW == 1 || (((l & 15) >> 3) == pw)) s += s_red[wave][l];
+        return s;
+    };
+    bool bad_trace = false;
+    (void)bad_trace;
+
     stage_values(0);
     double k[HTC][HV];
     rhs(y, k);
@@ -284,6 +299,19 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4_
 #if GQ_OUT_MEAN
         if (step >= a.warmup_steps) avg += od * diag_of(y);
 #endif
+#if GQ_TRACE
+        // observable trace after steps offset + k * stride (gqis/solver.py:2177-2193)
+        if (step >= a.trace_offset && (step - a.trace_offset) % a.trace_stride == 0) {
+            const int kk = (step - a.trace_offset) / a.trace_stride;
+            if (kk < a.num_trace) {
+                const double s = point_sum(od * diag_of(y));
+                if (leader && valid) {
+                    bad_trace |= !gq_finite(s);
+                    gq_store_complex(a.trace, (long long)kk * a.trace_point_stride + q, s, 0.0);
+                }
+            }
+        }
+#endif
     }
 
     bool bad = false;
@@ -291,14 +319,8 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4_
 #if GQ_OUT_FINAL
     avg = od * diag_of(y);
 #endif
-    // sum the partial traces of the lanes of each point (HPW points per wave: lanes with
-    // c / 8 == pw for N = 8, all 64 lanes otherwise)
-    s_red[wave][lane] = avg;
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-    if (g == 0 && (HPW == 1 ? c == 0 : (c & 7) == 0)) {
-        double s = 0.0;
-        for (int l = 0; l < 64; ++l)
-            if (HPW == 1 || (((l & 15) >> 3) == pw)) s += s_red[wave][l];
+    double s = point_sum(avg);
+    if (leader) {
 #if GQ_OUT_MEAN
         s /= (double)(num_steps - a.warmup_steps);
 #endif
@@ -329,4 +351,7 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4_
     }
 #endif
     gq_flag(a.status, bad, GQ_STATUS_NONFINITE_RESULT);
+#if GQ_TRACE
+    gq_flag(a.status, bad_trace, GQ_STATUS_NONFINITE_TRACE);
+#endif
 }

@@ -451,15 +451,17 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
                                    rho0, rho0_values, spec, mode, fp64, return_time_trace,
                                    liouville, dt, t_in, num_steps, warmup_steps, block_size,
                                    collect, tuple(hiprtc_options) + _as_tuple(nvrtc_options),
-                                   prepare=_prepare)
+                                   prepare=_prepare, num_trace=num_trace,
+                                   trace_stride=trace_stride, trace_offset=trace_offset)
     if hilbert_out is not None and _prepare:
         from .prepared import PreparedSweep
         plan, stage_s, rhs_status = hilbert_out
 
         def finish(outs, *, return_device, ignore_non_finite_output, total_start):
             return _finish(outs, stage_s, rhs_status, spec, solver_frequency, unroll, False,
-                           False, return_device, False, False, ignore_non_finite_output,
-                           None, 0, 1, 0, fp64, t_in, dt, total_start)
+                           False, return_device, return_time_trace, False,
+                           ignore_non_finite_output, None, num_trace, trace_stride, trace_offset,
+                           fp64, t_in, dt, total_start)
         return PreparedSweep(plan, compiled=None, spec=spec, const_values=const_values,
                              runtime_consts=runtime_consts, use_defaults=False,
                              sweep_symbols=tuple(var_arrays),
@@ -618,7 +620,7 @@ SPLIT_F16_MAX_ENTRY = 1.0e4     # largest |H_eff| entry routed to FP16 hi/lo pro
 def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, runtime_consts,
                  keep_symbolic_consts, auto_runtime_consts, rho0, rho0_values, spec, mode, fp64,
                  trace, liouville, dt, t_in, num_steps, warmup_steps, block_size, collect, options,
-                 prepare=False):
+                 prepare=False, num_trace=0, trace_stride=1, trace_offset=0):
     """Run the matrix-core (Hilbert-space) path when it applies; None otherwise (``prepare``:
     return its launch plan instead of running it)."""
     from . import hilbert
@@ -635,8 +637,6 @@ def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, run
         reasons.append(f"N = {n} (supported: {hilbert.SUPPORTED_N})")
     if spec.name != "rk4":
         reasons.append(f"solver {spec.name} (RK4 only)")
-    if trace:
-        reasons.append("time traces")
     if mode not in ("mean", "final", "final_rho"):
         reasons.append(f"output mode {mode}")
     if not wanted:
@@ -670,7 +670,9 @@ def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, run
     outputs = hilbert.run(hm, mode=mode, var_arrays=var_arrays, rho0_values=rho0_values,
                           runtime_values=rt, dt=dt, t_in=t_in, num_steps=num_steps,
                           warmup_steps=warmup_steps, block_size=block_size, collect=collect,
-                          options=options, fp64=fp64, split=split, prepare=prepare)
+                          options=options, fp64=fp64, split=split, prepare=prepare,
+                          num_trace=num_trace if trace else 0, trace_stride=trace_stride,
+                          trace_offset=trace_offset)
     if prepare:
         outputs["runtime_defaults"] = dict(model["defaults"])
     stage_s = {"ode_prepare_s": prep, "ode_optimize_s": 0.0, "emit_s": 0.0}

@@ -152,3 +152,26 @@ def test_fp64_matrix_core_chain_matches_per_lane_kernel(gpu):
         mfma = mesolve_2D(*args, liouville="mfma", output_mode=mode, **kw)
         lanes = mesolve_2D(*args, liouville="lanes", output_mode=mode, **kw)
         assert np.max(np.abs(mfma - lanes)) <= 1e-12 * max(1.0, np.max(np.abs(lanes)))
+
+
+@pytest.mark.parametrize("n,lv,fp64,tol", [(8, "mfma", True, 1e-12), (8, "mfma", False, 2e-5),
+                                           (16, "mfma_f16", False, 2e-5)])
+def test_matrix_core_time_traces(gpu, n, lv, fp64, tol):
+    # observable traces after steps offset + k * stride, with the mean and with the final
+    # state, against the per-point FP64 kernel (whose traces are pinned to the oracle's) at
+    # N = 8, and at N = 16 against the FP64 matrix-core trace (the 255-state per-point FP64
+    # kernel takes minutes to compile)
+    w = workloads.random_n_level(n, seed=SEED, n_eps=3, n_amp=2, n_t=121, t_end=1.2)
+    args = (w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"])
+    ref_path = "lanes" if n == 8 else "mfma"
+    for mode, every in (("mean", 7), ("final_rho", 20)):
+        kw = dict(var_arrays=w["var_arrays"], output_mode=mode, return_time_trace=True,
+                  time_trace_every=every, return_timing_info=True)
+        res, tr, times, info = mesolve_2D(*args, liouville=lv, fp64=fp64, **kw)
+        assert info["kernel"].startswith("gq_hilbert_rk4")
+        rres, rtr, rtimes, rinfo = mesolve_2D(*args, liouville=ref_path, fp64=True, **kw)
+        assert tr.shape == rtr.shape == (3, 2, len(times))
+        np.testing.assert_allclose(times, rtimes, rtol=1e-6)
+        scale = max(1.0, float(np.max(np.abs(rtr))))
+        assert float(np.max(np.abs(tr - rtr))) <= tol * scale
+        assert float(np.max(np.abs(res - rres))) <= tol * max(1.0, float(np.max(np.abs(rres))))
