@@ -31,6 +31,11 @@ class HilbertModel:
     basis: list              # K SymPy expressions g_k(ParX, ParY, t, consts)
     runtime_syms: tuple
     key: tuple
+    # collapse operators with at most one nonzero per row that are neither diagonal nor a single
+    # entry (a qubit's lowering operator in a multi-qubit space): L[i][mcol[k, i]] = ml[k, i],
+    # mcol -1 for a zero row; their jump term (L rho L^+)_ij = ml_i conj(ml_j) rho[mcol_i][mcol_j]
+    mcol: np.ndarray = None  # int M x N
+    ml: np.ndarray = None    # complex M x N
 
 
 def _numeric(e):
@@ -85,6 +90,7 @@ def extract(model, drive_map):
     vcoef = np.array([[r.get(m, 0.0) for r in rows] for m in basis], np.float64)
     dep = np.zeros((n, n))
     jump = np.zeros((n, n))
+    mono = []
     h_eff = h0.astype(complex)
     for op in model["Col_Ops"]:
         mat = np.zeros((n, n), complex)
@@ -106,6 +112,12 @@ def extract(model, drive_map):
         elif len(nz) == 1:
             a, b = nz[0]
             jump[a, b] += abs(mat[a, b]) ** 2
+        elif np.all(np.count_nonzero(np.abs(mat) > 0, axis=1) <= 1):
+            col = np.full(n, -1)
+            coef = np.zeros(n, complex)
+            for i, k in nz:
+                col[i], coef[i] = k, mat[i, k]
+            mono.append((col, coef))
         else:
             return None
     o = np.zeros(n)
@@ -116,10 +128,12 @@ def extract(model, drive_map):
                 return None
             if i == k:
                 o[i] = v.real
+    mcol = np.array([c for c, _ in mono], int).reshape(len(mono), n)
+    ml = np.array([l for _, l in mono], complex).reshape(len(mono), n)
     key = ("hilbert", n, h_eff.tobytes(), dep.tobytes(), jump.tobytes(), o.tobytes(),
-           vcoef.tobytes(), tuple(sp.srepr(b) for b in basis))
+           vcoef.tobytes(), tuple(sp.srepr(b) for b in basis), mcol.tobytes(), ml.tobytes())
     return HilbertModel(n, h_eff, dep, jump, o, vcoef, basis,
-                        tuple(model["runtime_const_syms"]), key)
+                        tuple(model["runtime_const_syms"]), key, mcol, ml)
 
 
 def pack_a(m):
@@ -144,12 +158,13 @@ def pack_a(m):
 
 
 def dense_enough(m):
-    """Whether the dense matrix-core product beats the per-point kernels below 16 levels: their
-    work grows with the nonzeros of H_eff (a reduced element couples to the elements its row and
-    column of H_eff reach), the product's does not.  Measured in FP64 at N = 8 (256^2, 2000 RK4
-    steps): dense random H_eff 52 ms on the matrix cores against 76 ms with 8 lanes per point;
-    the tridiagonal chain (3 nonzeros per row) runs about 3x faster per point on the lanes.
-    Break-even is about 5.5 nonzeros per row at N = 8."""
+    """Whether the dense matrix-core product beats the per-point kernels (below 32 levels):
+    their work grows with the nonzeros of H_eff (a reduced element couples to the elements its
+    row and column of H_eff reach), the product's does not.  Measured (2000 RK4 steps): dense
+    random H_eff at N = 8 in FP64, 52 ms on the matrix cores against 76 ms with 8 lanes per
+    point (256^2); the tridiagonal 8-level chain about 3x faster on the lanes; the 4-qubit chain
+    (5 nonzeros per row of 16) 17.9 ms on 8 lanes per point against 103 ms with FP16 hi/lo
+    products (128^2).  Break-even is about 5.5 nonzeros per row at N = 8."""
     per_row = np.count_nonzero(np.abs(m.h_eff) > 0) / m.n
     return per_row >= 0.75 * m.n
 
@@ -234,6 +249,11 @@ def pack_a_split(m, kind):
     return hi, lo
 
 
+def n_mono(m):
+    """Number of collapse operators handled by the general gather (one nonzero per row)."""
+    return 0 if m.mcol is None else int(m.mcol.shape[0])
+
+
 def jump_width(m):
     """Largest number of populations feeding one population (at least 1)."""
     return max(1, int(np.max(np.count_nonzero(m.jump, axis=1))))
@@ -253,9 +273,14 @@ def constants(m, runtime_values, split=None):
     big = np.zeros((2 * n, 2 * n))
     big[:n, :n], big[:n, n:] = m.h_eff.real, -m.h_eff.imag
     big[n:, :n], big[n:, n:] = m.h_eff.imag, m.h_eff.real
+    mono = n_mono(m)
+    mcol = m.mcol if mono else np.full((1, n), -1)
+    ml = m.ml if mono else np.zeros((1, n), complex)
     parts = {"RC": rc, "APACK": pack_a(m).ravel(), "HBIG": big.ravel(),
              "DPACK": pack_a_dmfma(m).ravel(), "DEP": m.dep.ravel(),
-             "JW": jw.ravel(), "JB": jb.ravel(), "VCOEF": m.vcoef.ravel(), "OBS": m.obs}
+             "JW": jw.ravel(), "JB": jb.ravel(), "VCOEF": m.vcoef.ravel(), "OBS": m.obs,
+             "MCOL": mcol.ravel().astype(np.float64), "MRE": ml.real.ravel(),
+             "MIM": ml.imag.ravel()}
     if split:
         hi, lo = pack_a_split(m, split)
         parts.update(AHI=hi.ravel(), ALO=lo.ravel())
@@ -270,14 +295,22 @@ def constants(m, runtime_values, split=None):
 LDS_BYTES_PER_CU = 160 * 1024      # gfx950
 
 
-def lds_bytes(n, fp64, block, jmax, split=None, valu=False):
+def lds_bytes(n, fp64, block, jmax, split=None, valu=False, mono=0):
     """Static LDS of the kernel (the ``__shared__`` arrays of ``kernels/hilbert_rk4.hip`` and,
-    for FP64 without ``valu``, ``kernels/hilbert_rk4_dmfma.hip``)."""
+    for FP64 without ``valu``, ``kernels/hilbert_rk4_dmfma.hip``); ``mono`` general jump
+    operators add a per-wavefront copy of the state and their row tables."""
     if fp64 and not valu:
         waves, pts = block // 64, (2 if n == 8 else 1)
-        return (n // 8) ** 2 * 4 * 64 * 8 + waves * (2 * pts * n * (n + 1) + pts * n + 64) * 8 \
+        total = (n // 8) ** 2 * 4 * 64 * 8 + waves * (2 * pts * n * (n + 1) + pts * n + 64) * 8 \
             + n * n * 8 + n * jmax * 12
+        if mono:
+            total += waves * 2 * pts * n * (n + 1) * 8 + mono * n * (4 + 16)
+        return total
     r = 8 if fp64 else 4
+    if mono:
+        waves, pts = block // 64, 32 // n
+        extra = waves * 2 * pts * n * (n + 1) * r + mono * n * (4 + 2 * r)
+        return lds_bytes(n, fp64, block, jmax, split, valu) + extra
     waves, pts, tiles = block // 64, 32 // n, (2 * n + 31) // 32
     total = waves * 2 * pts * n * (n + 1) * r + waves * pts * n * r + n * n * r
     total += n * jmax * (r + 4) + waves * 64 * 8
@@ -290,11 +323,17 @@ def lds_bytes(n, fp64, block, jmax, split=None, valu=False):
     return total
 
 
-def block_for(m, fp64, block_size=256, valu=False):
-    """Work-group size: the FP64 VALU N = 32 kernel keeps its LDS under 160 KB with two waves."""
+def block_for(m, fp64, block_size=256, valu=False, split=None):
+    """Work-group size: the requested one, halved until the kernel's LDS fits a compute unit
+    (the FP64 VALU kernel at N = 32, and the FP64 matrix-core kernel at N = 32 with general
+    jump operators, run two waves per work-group)."""
     if block_size == "auto":
         return block_size
-    return 128 if (fp64 and valu and m.n == 32) else block_size
+    b = int(block_size)
+    while b > 64 and lds_bytes(m.n, fp64, b, jump_width(m), split, valu,
+                               n_mono(m)) > LDS_BYTES_PER_CU:
+        b //= 2
+    return b
 
 
 def emit(m, offs, mode, block_size=256, fp64=False, split=None, valu=False, trace=False):
@@ -313,7 +352,7 @@ def emit(m, offs, mode, block_size=256, fp64=False, split=None, valu=False, trac
     pr = HipPrinter(names)
     ftype = "double" if fp64 else "float"
     lines = [f"#define GQ_HN {m.n}", f"#define GQ_NBASIS {len(m.basis)}",
-             f"#define GQ_JMAX {jump_width(m)}",
+             f"#define GQ_JMAX {jump_width(m)}", f"#define GQ_NMONO {n_mono(m)}",
              f"#define GQ_NATOMS {len(atom_list)}", f"#define GQ_NATOMS_ALLOC {max(1, len(atom_list))}"]
     for k, v in offs.items():
         lines.append(f"#define GQ_OFF_{k} {v}")
@@ -375,10 +414,12 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
         raise ValueError("split-precision matrix-core products are an FP32-mode option")
     consts, offs = constants(m, runtime_values, split)
     uses_rho0 = rho0_values is not None
-    block_size = block_for(m, fp64, block_size, valu)
+    if valu and n_mono(m):
+        raise ValueError("the FP64 VALU form has no general jump operators")
+    block_size = block_for(m, fp64, block_size, valu, split)
     tune = block_size == "auto"
     if tune:
-        block_size = block_for(m, fp64, 256, valu)
+        block_size = block_for(m, fp64, 256, valu, split)
     trace = num_trace > 0
     skey = (m.key, mode, block_size, uses_rho0, tuple(options), bool(fp64), split, bool(valu),
             trace)
@@ -435,7 +476,8 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
         trial_steps = min(int(num_steps), 64)
         jmax = jump_width(m)
         fits = tuple(b for b in autotune.CANDIDATES
-                     if lds_bytes(m.n, fp64, b, jmax, split, valu) <= LDS_BYTES_PER_CU)
+                     if lds_bytes(m.n, fp64, b, jmax, split, valu, n_mono(m))
+                     <= LDS_BYTES_PER_CU)
         spare = torch.zeros(1, dtype=torch.int32, device=dev)
 
         def trial(mod, block):

@@ -69,6 +69,14 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
     __shared__ float s_apack[HT * HN * 64];
 #endif
     __shared__ real s_y[HWAVES][2][HP * HN * HLD];
+#if GQ_NMONO
+    // general jump operators (one nonzero per row): a per-wavefront copy of the state and the
+    // operators' row tables, (L rho L^+)_ij = l_i conj(l_j) rho[col_i][col_j]
+    __shared__ real s_xm[HWAVES][2][HP * HN * HLD];
+    __shared__ int s_mcol[GQ_NMONO * HN];
+    __shared__ real s_mre[GQ_NMONO * HN];
+    __shared__ real s_mim[GQ_NMONO * HN];
+#endif
     __shared__ real s_diag[HWAVES][HP * HN];
     __shared__ real s_dep[HN * HN];           // dephasing weights sum_k d_k(i) d_k(j)
     __shared__ real s_jw[HN * GQ_JMAX];       // population transfer into a: weights |l|^2 ...
@@ -92,6 +100,13 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
         s_jw[e] = cst[GQ_OFF_JW + e];
         s_jb[e] = (int)cst[GQ_OFF_JB + e];
     }
+#if GQ_NMONO
+    for (int e = threadIdx.x; e < GQ_NMONO * HN; e += GQ_BLOCK) {
+        s_mcol[e] = (int)cst[GQ_OFF_MCOL + e];
+        s_mre[e] = cst[GQ_OFF_MRE + e];
+        s_mim[e] = cst[GQ_OFF_MIM + e];
+    }
+#endif
     __syncthreads();
 
     const int wave = threadIdx.x >> 6;
@@ -153,6 +168,18 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
     // Per-lane constants of the dissipator, kept in registers: the dephasing weights of the
     // lane's elements and, for the lane holding its column's diagonal element (m = mdiag), that
     // population's jump list (zero weights elsewhere), so the RHS has no per-element branch.
+#if GQ_NMONO
+    real* __restrict__ xm_r = s_xm[wave][0] + pw * HN * HLD;
+    real* __restrict__ xm_i = s_xm[wave][1] + pw * HN * HLD;
+    int mcj[GQ_NMONO];         // the general jump operators at the lane's column j
+    real mlr[GQ_NMONO], mli[GQ_NMONO];
+#pragma unroll
+    for (int kk = 0; kk < GQ_NMONO; ++kk) {
+        mcj[kk] = s_mcol[kk * HN + j];
+        mlr[kk] = s_mre[kk * HN + j];
+        mli[kk] = s_mim[kk * HN + j];
+    }
+#endif
     real depw[HM];
     int mdiag = -1;
 #pragma unroll
@@ -274,6 +301,10 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
             const int i = gq_row_of(m, h);
             ybuf_r[i * HLD + j] = yr[m];
             ybuf_i[i * HLD + j] = yi[m];
+#if GQ_NMONO
+            xm_r[i * HLD + j] = x[m];
+            xm_i[i * HLD + j] = x[m + HM];
+#endif
             xd = (m == mdiag) ? x[m] : xd;
         }
         if (mdiag >= 0) dbuf[j] = xd;
@@ -288,6 +319,21 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
             real di = yi[m] - ybuf_i[j * HLD + i];
             dr = fma(depw[m], x[m], dr);
             di = fma(depw[m], x[m + HM], di);
+#if GQ_NMONO
+#pragma unroll
+            for (int kk = 0; kk < GQ_NMONO; ++kk) {
+                const int ci = s_mcol[kk * HN + i];
+                if (ci >= 0 && mcj[kk] >= 0) {
+                    // w = l_i conj(l_j), times rho[col_i][col_j]
+                    const real lr = s_mre[kk * HN + i], li = s_mim[kk * HN + i];
+                    const real wr = fma(lr, mlr[kk], li * mli[kk]);
+                    const real wi = fma(li, mlr[kk], -lr * mli[kk]);
+                    const real pr = xm_r[ci * HLD + mcj[kk]], pim = xm_i[ci * HLD + mcj[kk]];
+                    dr = fma(wr, pr, fma(-wi, pim, dr));
+                    di = fma(wr, pim, fma(wi, pr, di));
+                }
+            }
+#endif
             k[m] = (m == mdiag) ? dr + jsum : dr;
             k[m + HM] = di;
         }

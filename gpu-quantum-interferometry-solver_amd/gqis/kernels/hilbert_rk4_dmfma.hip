@@ -40,6 +40,14 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4_
     __shared__ double s_jw[HN * GQ_JMAX];       // population transfer into a: weights |l|^2 ...
     __shared__ int s_jb[HN * GQ_JMAX];          // ... from the populations b (sparse rows)
     __shared__ double s_red[HWAVES][64];
+#if GQ_NMONO
+    // general jump operators (one nonzero per row): a per-wavefront copy of the state and the
+    // operators' row tables, (L rho L^+)_ij = l_i conj(l_j) rho[col_i][col_j]
+    __shared__ double s_xm[HWAVES][2][HPW * HN * HLD];
+    __shared__ int s_mcol[GQ_NMONO * HN];
+    __shared__ double s_mre[GQ_NMONO * HN];
+    __shared__ double s_mim[GQ_NMONO * HN];
+#endif
 
     const double* __restrict__ cst = (const double*)a.consts;   // packed operator constants
     for (int e = threadIdx.x; e < HTR * HTR * 4 * 64; e += GQ_BLOCK) s_apack[e] = cst[GQ_OFF_DPACK + e];
@@ -48,6 +56,13 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4_
         s_jw[e] = cst[GQ_OFF_JW + e];
         s_jb[e] = (int)cst[GQ_OFF_JB + e];
     }
+#if GQ_NMONO
+    for (int e = threadIdx.x; e < GQ_NMONO * HN; e += GQ_BLOCK) {
+        s_mcol[e] = (int)cst[GQ_OFF_MCOL + e];
+        s_mre[e] = cst[GQ_OFF_MRE + e];
+        s_mim[e] = cst[GQ_OFF_MIM + e];
+    }
+#endif
     __syncthreads();
 
     const int wave = threadIdx.x >> 6;
@@ -66,6 +81,20 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4_
     double* __restrict__ ybuf_r = s_y[wave][0] + pw * HN * HLD;
     double* __restrict__ ybuf_i = s_y[wave][1] + pw * HN * HLD;
     double* __restrict__ dbuf = s_diag[wave] + pw * HN;
+#if GQ_NMONO
+    double* __restrict__ xm_r = s_xm[wave][0] + pw * HN * HLD;
+    double* __restrict__ xm_i = s_xm[wave][1] + pw * HN * HLD;
+    int mcj[HTC][GQ_NMONO];      // the general jump operators at the lane's columns
+    double mlr[HTC][GQ_NMONO], mli[HTC][GQ_NMONO];
+#pragma unroll
+    for (int ct = 0; ct < HTC; ++ct)
+#pragma unroll
+        for (int kk = 0; kk < GQ_NMONO; ++kk) {
+            mcj[ct][kk] = s_mcol[kk * HN + jc[ct]];
+            mlr[ct][kk] = s_mre[kk * HN + jc[ct]];
+            mli[ct][kk] = s_mim[kk * HN + jc[ct]];
+        }
+#endif
 
     // ---- initial state: rho0 values (reduced layout) or |0><0| ------------------------------
     double y[HTC][HV];
@@ -210,6 +239,10 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4_
                 const int i = g * HQ + u;
                 ybuf_r[i * HLD + j] = ry;
                 ybuf_i[i * HLD + j] = iy;
+#if GQ_NMONO
+                xm_r[i * HLD + j] = x[ct][u];
+                xm_i[i * HLD + j] = x[ct][u + HQ];
+#endif
             }
         }
         if (ctd >= 0) dbuf[jd] = diag_of(x);
@@ -234,6 +267,22 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_rk4_
                 double di = iy - ybuf_i[j * HLD + i];
                 dr = fma(dw, x[ct][u], dr);
                 di = fma(dw, x[ct][u + HQ], di);
+#if GQ_NMONO
+#pragma unroll
+                for (int kk = 0; kk < GQ_NMONO; ++kk) {
+                    const int ci = s_mcol[kk * HN + i];
+                    if (ci >= 0 && mcj[ct][kk] >= 0) {
+                        // w = l_i conj(l_j), times rho[col_i][col_j]
+                        const double lr = s_mre[kk * HN + i], li = s_mim[kk * HN + i];
+                        const double wr = fma(lr, mlr[ct][kk], li * mli[ct][kk]);
+                        const double wi = fma(li, mlr[ct][kk], -lr * mli[ct][kk]);
+                        const double pr = xm_r[ci * HLD + mcj[ct][kk]];
+                        const double pim = xm_i[ci * HLD + mcj[ct][kk]];
+                        dr = fma(wr, pr, fma(-wi, pim, dr));
+                        di = fma(wr, pim, fma(wi, pr, di));
+                    }
+                }
+#endif
                 k[ct][u] = (u == md[ct]) ? dr + jsum : dr;
                 k[ct][u + HQ] = di;
             }

@@ -115,13 +115,15 @@ class SweepServer:
             split = ("f16" if not r.fp64 and np.max(np.abs(hm.h_eff)) <= SPLIT_F16_MAX_ENTRY
                      else None)
             cvals, offs = hilbert.constants(hm, list(key[-1]), split)
-            block = hilbert.block_for(hm, r.fp64, 256 if self.block == "auto" else self.block)
+            block = hilbert.block_for(hm, r.fp64, 256 if self.block == "auto" else self.block,
+                                      split=split)
             src = hilbert.emit(hm, offs, r.output_mode, block, r.fp64, split).replace(
                 "GQ_RHO0_FROM_ARGS", "0")
             kname = codegen.kernel_name(hilbert.kernel_base(split, r.fp64), r.fp64)
             natoms = int(src.split("#define GQ_NATOMS ")[1].split()[0])
             cands = tuple((1, b) for b in autotune.CANDIDATES
-                          if hilbert.lds_bytes(hm.n, r.fp64, b, hilbert.jump_width(hm), split)
+                          if hilbert.lds_bytes(hm.n, r.fp64, b, hilbert.jump_width(hm), split,
+                                               mono=hilbert.n_mono(hm))
                           <= hilbert.LDS_BYTES_PER_CU)
 
             def variant(c):
@@ -216,9 +218,10 @@ class SweepServer:
                                       flags[1:])
 
     def _hilbert_model(self, r):
-        """Matrix-core model of a Lindblad request with N >= 16 (N >= 8 with a dense H_eff in
-        FP64) whose operators fit the Hilbert-space kernel, else None (smaller or sparser models
-        run faster on the per-point kernels); the same choice mesolve_2D makes."""
+        """Matrix-core model of a Lindblad request whose operators fit the Hilbert-space kernel
+        (N = 32; N = 16 with a dense H_eff; N = 8 with a dense H_eff in FP64), else None
+        (smaller or sparser models run faster on the per-point kernels); the same choice
+        mesolve_2D makes."""
         if r.compiled.kind != "lindblad" or r.output_mode not in ("mean", "final", "final_rho"):
             return None
         n = int(round(np.sqrt(r.compiled.state_size + 1)))
@@ -230,7 +233,7 @@ class SweepServer:
         from . import hilbert
         from .solver import _drive_map
         hm = hilbert.extract(src, _drive_map(src["Drive"]))
-        if hm is not None and n < 16 and not hilbert.dense_enough(hm):
+        if hm is not None and n < 32 and not hilbert.dense_enough(hm):
             return None
         return hm
 

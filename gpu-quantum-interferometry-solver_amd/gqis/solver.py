@@ -245,7 +245,8 @@ def mesolve_2D(H, Drive=None, Col_Ops=None, mean_operator=None, tlist=None,
     ``liouville`` ("auto", "lanes", "mfma", or "mfma_f16" / "mfma_bf16": the commutator product
     as three FP16 / BF16 matrix-core products of hi/lo parts with FP32 accumulation; the
     Hilbert-space path for N = 8, 16, 32 on the FP32 or FP64 matrix cores, used automatically
-    in RK4 sweeps whose operators fit it from N = 16, and from N = 8 in FP64 for a dense H).  ``return_device=True`` returns torch tensors on
+    in RK4 sweeps whose operators fit it: at N = 32, at N = 16 for a dense H_eff, at N = 8 for a
+    dense H_eff in FP64).  ``return_device=True`` returns torch tensors on
     the GPU.  ``H`` may be a ``CompiledRHS`` from ``compile_lindblad_rhs``.
 
     Returns ``(num_X, num_Y)`` complex for ``"mean"``/``"final"``, ``(num_X, num_Y, N*N-1)``
@@ -627,10 +628,12 @@ def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, run
     n = H.shape[0] if hasattr(H, "shape") else 0
     forced = liouville.startswith("mfma")
     split = {"mfma_f16": "f16", "mfma_bf16": "bf16"}.get(liouville)
-    # measured on gfx950 (random N-level, 256^2, 2000 steps): from 16 levels the matrix-core
-    # forms are fastest in both precisions; at 8 levels in FP64 the FP64 matrix-core form (52 ms)
-    # beats 8 lanes per point (77 ms) when H_eff is dense (hilbert.dense_enough), while FP32
-    # stays on the per-point kernels
+    # measured on gfx950 (256^2, 2000 steps): for a dense H_eff (hilbert.dense_enough) the
+    # matrix-core forms are fastest from 16 levels in both precisions and at 8 levels in FP64
+    # (52 ms against 77 ms with 8 lanes per point); a sparse one (the 8-level chain, the
+    # multi-qubit chain: 3-5 nonzeros per row) runs 3-6x faster on the per-point kernels below
+    # 32 levels.  At 32 levels the per-point form needs 1023 symbolic equations and a very large
+    # kernel, so the matrix-core form is taken whenever the model fits it.
     wanted = forced or (liouville == "auto" and (n >= 16 or (fp64 and n >= 8)))
     reasons = []
     if n not in hilbert.SUPPORTED_N:
@@ -657,7 +660,7 @@ def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, run
                              "terms, constant diagonal or single-entry collapse operators and a "
                              "diagonal observable.")
         return None
-    if liouville == "auto" and n < 16 and not hilbert.dense_enough(hm):
+    if liouville == "auto" and n < 32 and not hilbert.dense_enough(hm):
         return None          # a sparse H_eff runs faster on the per-point kernels
     values = dict(model["defaults"])
     values.update(runtime_consts or {})

@@ -123,6 +123,49 @@ def chain(n, n_eps=64, n_amp=64, eps_range=(-10.0, 10.0), amp_max=20.0, n_t=2001
     }
 
 
+def multi_qubit(n_qubits, n_eps=64, n_amp=64, n_t=2001, t_end=20.0, delta=1.0, w_over_delta=1.14,
+                gamma1_per_t=0.05, gamma_phi_per_t=0.04, gap_step=0.05, j_over_delta=0.1,
+                max_over_w=10.0):
+    """The reference's multi-qubit scaling model (``Benchmarks/Benchmark_04_hilbert_space_
+    scaling.py:128-173``, defaults of ``:1084-1093``): n qubits on a chain (dimension 2^n) with
+    gaps delta (1 + gap_step i) sx_i / 2, ZZ coupling J sz_i sz_j on neighbours, the drive
+    (eps + A sin(w t)) sz_0 / 2 on qubit 0, collapse sqrt(gamma1) s-_i (s- = [[0,1],[0,0]]) and
+    sqrt(gamma2) sz_i with gamma1 = gamma1_per_T / T, gamma2 = gamma1/2 + gamma_phi_per_T / T,
+    T = 2 pi / w; observable: qubit 0 in |1><1|.  eps in [-eps_max, eps_max], A in [0, A_max]
+    with both maxima max_over_w * w."""
+    sx = np.array([[0.0, 1.0], [1.0, 0.0]])
+    sz = np.array([[1.0, 0.0], [0.0, -1.0]])
+    sm = np.array([[0.0, 1.0], [0.0, 0.0]])
+    excited = np.array([[0.0, 0.0], [0.0, 1.0]])
+
+    def on_qubit(op, index):
+        out = np.ones((1, 1))
+        for k in range(n_qubits):
+            out = np.kron(out, op if k == index else np.eye(2))
+        return out
+    w = w_over_delta * delta
+    period = 2.0 * np.pi / w
+    gamma1 = gamma1_per_t / period
+    gamma2 = gamma1 / 2.0 + gamma_phi_per_t / period
+    h = np.zeros((2 ** n_qubits, 2 ** n_qubits))
+    col = []
+    for i in range(n_qubits):
+        h += 0.5 * delta * (1.0 + gap_step * i) * on_qubit(sx, i)
+        col += [np.sqrt(gamma1) * on_qubit(sm, i), np.sqrt(gamma2) * on_qubit(sz, i)]
+    for i in range(n_qubits - 1):
+        h += j_over_delta * delta * on_qubit(sz, i) @ on_qubit(sz, i + 1)
+    big = max_over_w * w
+    return {
+        "H": sp.Matrix(h) + DRIVE * sp.Matrix(on_qubit(sz, 0)) / 2,
+        "Drive": {DRIVE: EPS + AMP * sp.sin(w * T)},
+        "Col_Ops": [sp.Matrix(c) for c in col], "mean_operator": sp.Matrix(on_qubit(excited, 0)),
+        "tlist": np.linspace(0.0, t_end, n_t),
+        "var_arrays": {EPS: np.linspace(-big, big, n_eps), AMP: np.linspace(0.0, big, n_amp)},
+        "numeric": {"h0": h, "hz": on_qubit(sz, 0) / 2, "col": col,
+                    "obs": on_qubit(excited, 0), "w": w},
+    }
+
+
 def serving_stream(count=1000, rate=20.0, seed=7, sides=(32, 128, 512, 1024)):
     """Seeded Poisson stream of sweep-request descriptions: model (two-level LZS or an N-level
     chain, N in {2, 4, 8}), square grid side, amplitude range, solver (rk4 / tsit5a) and

@@ -116,6 +116,36 @@ def random_n_level(n, seed, px, py, rate=0.01):
     return AffineSweep(terms, obs, px, py)
 
 
+def multi_qubit(n_qubits, px, py, delta=1.0, w_over_delta=1.14, gamma1_per_t=0.05,
+                gamma_phi_per_t=0.04, gap_step=0.05, j_over_delta=0.1):
+    """The reference's multi-qubit chain (``Benchmarks/Benchmark_04_hilbert_space_scaling.py:
+    128-173``): gaps delta (1 + gap_step i) sx_i / 2, J sz_i sz_{i+1}, drive (eps + A sin(w t))
+    sz_0 / 2, collapse sqrt(gamma1) s-_i with s- = [[0, 1], [0, 0]] and sqrt(gamma2) sz_i,
+    observable qubit 0 in |1><1|."""
+    sx = np.array([[0, 1], [1, 0]], complex)
+    sm = np.array([[0, 1], [0, 0]], complex)
+
+    def on(op, index):
+        out = np.ones((1, 1), complex)
+        for k in range(n_qubits):
+            out = np.kron(out, op if k == index else np.eye(2))
+        return out
+    w = w_over_delta * delta
+    period = 2.0 * np.pi / w
+    gamma1 = gamma1_per_t / period
+    gamma2 = gamma1 / 2.0 + gamma_phi_per_t / period
+    h0 = sum(0.5 * delta * (1.0 + gap_step * i) * on(sx, i) for i in range(n_qubits))
+    h0 = h0 + sum(j_over_delta * delta * on(SZ, i) @ on(SZ, i + 1) for i in range(n_qubits - 1))
+    col = []
+    for i in range(n_qubits):
+        col += [np.sqrt(gamma1) * on(sm, i), np.sqrt(gamma2) * on(SZ, i)]
+    hz = on(SZ, 0) / 2
+    terms = [(one, h0, col),
+             (lambda t, x, y: x, hz, []),
+             (lambda t, x, y: y * np.sin(w * t), hz, [])]
+    return AffineSweep(terms, on(np.diag([0.0, 1.0]).astype(complex), 0), px, py)
+
+
 class Duffing:
     """x'' + delta x' + alpha x + beta x^3 = gamma cos(w t) as (x, v); sweep (gamma, x0)."""
 

@@ -175,3 +175,44 @@ def test_matrix_core_time_traces(gpu, n, lv, fp64, tol):
         scale = max(1.0, float(np.max(np.abs(rtr))))
         assert float(np.max(np.abs(tr - rtr))) <= tol * scale
         assert float(np.max(np.abs(res - rres))) <= tol * max(1.0, float(np.max(np.abs(rres))))
+
+
+@pytest.mark.parametrize("nq,steps", [(3, 200), (4, 100), (5, 30)])
+def test_multi_qubit_model_on_the_matrix_cores(gpu, nq, steps):
+    # the reference's multi-qubit chain (Benchmark 04): each qubit's lowering operator has one
+    # nonzero per row, so its jump term is the kernels' general gather; FP64 matrix-core form
+    # against the oracle to 1e-10 of the map's scale, the FP32 and FP16 hi/lo forms to 2e-5
+    n = 2 ** nq
+    w = workloads.multi_qubit(nq, n_eps=2, n_amp=2, n_t=steps + 1, t_end=0.01 * steps)
+    eps_v, amp_v = w["var_arrays"].values()
+    args = (w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"])
+    m = models.multi_qubit(nq, eps_v, amp_v)
+    ref, _ = integrate.rk4(m.f, m.ground_state(), 0.01, steps, obs=m.obs)
+    fin, _ = integrate.rk4(m.f, m.ground_state(), 0.01, steps, mode="final_rho")
+    scale = max(1.0, float(np.max(np.abs(ref))))
+    forms = [("mfma", True, 1e-10)] + ([("mfma", False, 2e-5), ("mfma_f16", False, 2e-5)]
+                                       if nq < 5 else [])
+    for lv, fp64, tol in forms:
+        got, info = mesolve_2D(*args, var_arrays=w["var_arrays"], liouville=lv, fp64=fp64,
+                               return_timing_info=True)
+        assert info["kernel"].startswith("gq_hilbert_rk4")
+        print(nq, lv, fp64, float(np.max(np.abs(got.ravel() - ref))), flush=True)
+        assert np.max(np.abs(got.ravel() - ref)) <= tol * scale
+        rho = mesolve_2D(*args, var_arrays=w["var_arrays"], liouville=lv, fp64=fp64,
+                         output_mode="final_rho")
+        assert rho.shape == (2, 2, n * n - 1)
+        assert np.max(np.abs(rho.reshape(4, -1) - fin)) <= tol
+
+
+def test_automatic_choice_follows_the_density_of_h(gpu):
+    # 16 levels: the dense random model takes the FP16 hi/lo matrix-core form, the sparse
+    # 4-qubit chain (5 nonzeros per row) the per-point kernels, which are faster for it
+    dense = workloads.random_n_level(16, seed=SEED, n_eps=2, n_amp=2, n_t=21, t_end=0.2)
+    sparse = workloads.multi_qubit(4, n_eps=2, n_amp=2, n_t=21, t_end=0.2)
+    kernels = []
+    for w in (dense, sparse):
+        _, info = mesolve_2D(w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"],
+                             var_arrays=w["var_arrays"], return_timing_info=True)
+        kernels.append(info["kernel"])
+    assert kernels[0] == "gq_hilbert_rk4_f16x3_f32"
+    assert not kernels[1].startswith("gq_hilbert")
