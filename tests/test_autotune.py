@@ -96,3 +96,23 @@ def test_hilbert_lds_formula_matches_the_compiler():
         assert est > hilbert.LDS_BYTES_PER_CU
         with pytest.raises(runtime.GqError, match=rf"local memory \({est}\) exceeds limit"):
             runtime.compile_check(src, codegen.HIPRTC_BASE_OPTIONS)
+
+
+def test_hilbert_lds_formula_with_general_jump_operators():
+    # the 5-qubit chain (lowering operators with one nonzero per row) adds the per-wavefront
+    # state copy and row tables: at 256 threads the FP64 kernel exceeds the limit by the
+    # formula's count, and block_for halves the work-group
+    w = workloads.multi_qubit(5, n_eps=2, n_amp=2)
+    model = solver._prepare_model(w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"],
+                                  sweep_symbols=tuple(w["var_arrays"]), const_values=None,
+                                  runtime_consts=None, keep_symbolic_consts=None,
+                                  auto_runtime_consts=False, rho0=None, ode_size=None)
+    hm = hilbert.extract(model, solver._drive_map(model["Drive"]))
+    assert hilbert.n_mono(hm) == 5
+    _, offs = hilbert.constants(hm, [])
+    src = hilbert.emit(hm, offs, "mean", 256, True).replace("GQ_RHO0_FROM_ARGS", "0")
+    est = hilbert.lds_bytes(32, True, 256, hilbert.jump_width(hm), mono=5)
+    assert est > hilbert.LDS_BYTES_PER_CU
+    with pytest.raises(runtime.GqError, match=rf"local memory \({est}\) exceeds limit"):
+        runtime.compile_check(src, codegen.HIPRTC_BASE_OPTIONS)
+    assert hilbert.block_for(hm, True, 256) == 128
