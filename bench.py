@@ -435,6 +435,19 @@ def bench_lzs(args):
                             "cores": threads, "kind": "port",
                             "sample": f"full 64x64 grid, 2000 RK4 steps, FP64 C oracle "
                                       f"(OpenMP, {threads} threads), {cpu_s:.3f} s"}}
+    if not args.no_cpu:
+        # the reference's own CPU path for this map (per-point Python RK4 on a lambdified RHS,
+        # spawn pool over eps columns, cores - 1 workers), restated, over the full grid
+        from oracle import python_cpu
+        log("reference CPU path over the full 64x64 map ...")
+        py_res, py_s, workers = python_cpu.sweep("lzs_two_level", eps, amp, 0.1, 2000,
+                                                 workers=max(1, threads - 1))
+        out["cpu_reference_path"] = {
+            "value": point_steps / py_s, "unit": "param-point RK steps/s", "cores": workers,
+            "kind": "port",
+            "sample": f"full 64x64 grid, 2000 RK4 steps, per-point Python loop on a lambdified "
+                      f"RHS, {workers} spawn workers, {py_s:.1f} s after worker preparation",
+            "max_abs_dev_vs_gpu": float(np.max(np.abs(py_res.ravel() - got.real.ravel())))}
     print(json.dumps(out), flush=True)
 
 
