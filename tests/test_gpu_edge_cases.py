@@ -53,3 +53,27 @@ def test_a_single_time_step(gpu, liouville, fp64):
     m = models.random_n_level(n, 5, eps_v, amp_v)
     ref, _ = integrate.rk4(m.f, m.ground_state(), 0.01, 1, obs=m.obs)
     assert np.max(np.abs(got.ravel() - ref)) <= 1e-12
+
+
+@pytest.mark.parametrize("solver,groups", [("dp5", 4), ("ab5", 2), ("rk4", 8), ("tsit5a", 4)])
+def test_ode_api_with_lanes_per_point(gpu, solver, groups):
+    # a 32-state ODE (a ring of driven, damped oscillators) through odesolve_2D: the grouped
+    # kernels split its rows over several wavefronts and give the one-lane result
+    import sympy as sp
+    from gqis import odesolve_2D
+    n = 16
+    xs = sp.symbols(f"x0:{n}", real=True)
+    vs = sp.symbols(f"v0:{n}", real=True)
+    k, g, t = sp.symbols("k g t", real=True)
+    rhs = list(vs) + [-(2 + k) * xs[i] + xs[(i - 1) % n] + xs[(i + 1) % n] - g * vs[i]
+                      + (sp.cos(t) if i == 0 else 0) for i in range(n)]
+    y0 = [1.0 if i == 3 else 0.0 for i in range(n)] + [0.0] * n
+    axes = {k: np.linspace(0.5, 1.5, 5), g: np.linspace(0.05, 0.2, 3)}
+    kw = dict(var_arrays=axes, solver=solver, fp64=True)
+    if solver == "tsit5a":
+        kw["adaptive_options"] = {"step_rtol": 1e-9, "step_atol": 1e-11, "max_substeps": 4000}
+    tl = np.linspace(0.0, 4.0, 401)
+    ref = odesolve_2D(rhs, list(xs) + list(vs), y0, tl, lanes_per_point=1, **kw)
+    got = odesolve_2D(rhs, list(xs) + list(vs), y0, tl, lanes_per_point=groups, **kw)
+    assert got.shape == ref.shape == (5, 3, 2 * n)
+    assert _rel(got, ref) <= (1e-8 if solver == "tsit5a" else 1e-13)
