@@ -129,8 +129,9 @@ def reference_cpu_path():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=30,
+                    help="timed sweeps (at least 30 for the latency percentiles)")
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--grid", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
