@@ -139,11 +139,16 @@ def main():
                     help="run only `--steps` sweeps of one form (for rocprofv3), no JSON line")
     ap.add_argument("--requests", type=int, default=1000, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="serving: Poisson rate (req/s)")
-    ap.add_argument("--workload", choices=["four_level", "lzs", "duffing", "serving", "scaling"],
+    ap.add_argument("--workload", choices=["four_level", "lzs", "duffing", "serving", "scaling",
+                                           "resonator"],
                     default="four_level",
                     help="four_level: the headline sweep; lzs: the 64x64 two-level map with the "
                          "CPU oracle beside it; duffing: adaptive Tsit5/DOP853 sweep; serving: "
-                         "Poisson request stream; scaling: Hilbert-space N-level models")
+                         "Poisson request stream; scaling: Hilbert-space N-level models; "
+                         "resonator: the reference's published qubit-resonator benchmarks")
+    ap.add_argument("--photons", type=int, nargs="*", default=[2, 3, 4, 5, 6, 7, 8, 9, 10],
+                    help="resonator: Fock states of the cavity (2 to 10)")
+    ap.add_argument("--no-adaptive", action="store_true", help="resonator: RK4 rows only")
     ap.add_argument("--levels", type=int, nargs="*", default=None,
                     help="scaling: only these N (4, 8, 16, 32)")
     ap.add_argument("--forms", default="",
@@ -168,6 +173,8 @@ def main():
         return bench_scaling(args)
     if args.workload == "lzs":
         return bench_lzs(args)
+    if args.workload == "resonator":
+        return bench_resonator(args)
 
     w = workloads.four_level(n_eps=args.grid, n_amp=args.grid)
     npts = args.grid * args.grid
@@ -450,6 +457,101 @@ def bench_lzs(args):
                       f"RHS, {workers} spawn workers, {py_s:.1f} s after worker preparation",
             "max_abs_dev_vs_gpu": float(np.max(np.abs(py_res.ravel() - got.real.ravel())))}
     print(json.dumps(out), flush=True)
+
+
+# The reference's published timings on its own qubit-resonator benchmark (RTX 3080, FP32):
+# Benchmarks/results/Benchmark_02_full_benchmark.csv:62 (2048 x 2048, RK4 5840 steps, solve-call
+# wall time) and Benchmarks/results/Benchmark_04_hilbert_space_scaling_calibrated.csv:57-65, 75-83
+# (1024 x 1024, kernel time: RK4 8611 steps; Tsit5A at rtol = atol = 1.77828e-4, 160 output
+# intervals, mean accepted steps per point in the second entry).
+RESONATOR_2048_RK4_S = 0.971001387
+RESONATOR_1024_RK4_S = {2: 0.345004028, 3: 0.849800171, 4: 1.70617957, 5: 17.3547266,
+                        6: 71.9960313, 7: 179.748641, 8: 302.470219, 9: 614.485,
+                        10: 662.597125}
+RESONATOR_1024_TSIT5A = {2: (0.272495605, 1294.33534), 3: (0.551332886, 1260.34076),
+                         4: (5.4231543, 1241.68607), 5: (27.9381484, 1229.23429),
+                         6: (85.9629922, 1219.59529), 7: (117.030633, 1211.54963),
+                         8: (151.767547, 1204.46933), 9: (204.656266, 1198.04485),
+                         10: (250.943922, 1192.12925)}
+
+
+def bench_resonator(args):
+    """The reference's own qubit-resonator benchmarks re-run here, same model, grid, step
+    counts and precision (FP32), beside its published RTX 3080 times: the 2048 x 2048 four-level
+    map (RK4, 5840 steps; call wall time and kernel time) and the Hilbert-space scaling rows at
+    1024 x 1024 (RK4 8611 steps, Tsit5A at the published tolerance) for ``--photons``."""
+    import numpy as np
+    import torch
+    from gqis import mesolve_2D, workloads
+    rows = {}
+
+    def call(w, tl, **kw):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out, info = mesolve_2D(w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], tl,
+                               var_arrays={k: v.astype(np.float32)
+                                           for k, v in w["var_arrays"].items()},
+                               return_timing_info=True, **kw)
+        return out, info, time.perf_counter() - t0
+
+    if 2 in args.photons and not args.no_plain:
+        w = workloads.qubit_resonator(2, n_eps=2048, n_amp=2048)
+        tl = w["tlist"].astype(np.float32)
+        log("qubit-resonator 2048x2048, RK4 5840 steps, FP32 ...")
+        call(w, tl)                                         # symbolic model, compile
+        walls, kern = [], []
+        for _ in range(max(3, min(args.steps, 10))):
+            out, info, wall = call(w, tl)
+            walls.append(wall)
+            kern.append(info["gpu_kernel_s"])
+        _, info64, _ = call(w, tl, fp64=True)
+        rows["2048x2048 rk4 5840 steps"] = {
+            "call_wall_s": float(np.median(walls)), "kernel_s": float(np.median(kern)),
+            "kernel_s_fp64": info64["gpu_kernel_s"], "published_rtx3080_s": RESONATOR_2048_RK4_S,
+            "speedup_call_wall": RESONATOR_2048_RK4_S / float(np.median(walls)),
+            "point_steps_per_s": 2048 * 2048 * 5840 / float(np.median(kern))}
+        log(json.dumps(rows["2048x2048 rk4 5840 steps"]))
+    for nph in args.photons:
+        # 8611 RK4 steps over 40 periods; Tsit5A with four output points per period
+        w = workloads.qubit_resonator(nph, n_eps=1024, n_amp=1024, steps_per_period=8611 / 40)
+        tl = w["tlist"].astype(np.float32)
+        log(f"qubit-resonator {2 * nph} levels ({(2 * nph) ** 2 - 1} states), 1024x1024 ...")
+        t0 = time.perf_counter()
+        call(w, tl[:2], output_mode="final")                # symbolic model only
+        prep = time.perf_counter() - t0
+        out, info, _ = call(w, tl)                          # compile
+        out, info, _ = call(w, tl)
+        row = {"levels": 2 * nph, "states": (2 * nph) ** 2 - 1, "symbolic_s": prep,
+               "rk4_kernel_s": info["gpu_kernel_s"], "rk4_kernel": info.get("kernel"),
+               "rk4_lanes_per_point": info.get("lanes_per_point"),
+               "rk4_published_rtx3080_s": RESONATOR_1024_RK4_S[nph],
+               "rk4_speedup": RESONATOR_1024_RK4_S[nph] / info["gpu_kernel_s"]}
+        log(json.dumps(row))
+        if not args.no_adaptive:
+            tla = np.linspace(0.0, float(w["tlist"][-1]), 161)
+            opts = {"step_rtol": 1.77828e-4, "step_atol": 1.77828e-4, "max_substeps": 128,
+                    "count_steps": True}
+            call(w, tla, solver="tsit5a", adaptive_options=opts, ignore_non_finite_output=True)
+            out, info, _ = call(w, tla, solver="tsit5a", adaptive_options=opts,
+                                ignore_non_finite_output=True)
+            acc = info["adaptive_steps"]["accepted"]
+            rej = info["adaptive_steps"]["rejected"]
+            pub_s, pub_steps = RESONATOR_1024_TSIT5A[nph]
+            row.update({"tsit5a_kernel_s": info["gpu_kernel_s"],
+                        "tsit5a_mean_accepted": float(acc.mean()),
+                        "tsit5a_rejected_frac": float(rej.sum() / max(acc.sum(), 1)),
+                        "tsit5a_finite": bool(np.all(np.isfinite(out))),
+                        "tsit5a_published_rtx3080_s": pub_s,
+                        "tsit5a_published_mean_accepted": pub_steps,
+                        "tsit5a_speedup": pub_s / info["gpu_kernel_s"]})
+            log(json.dumps(row))
+        rows[f"1024x1024 {2 * nph} levels"] = row
+    print(json.dumps({"metric": "kernel time on the reference's qubit-resonator benchmarks",
+                      "unit": "s", "higher_is_better": False, "n_gpus": 1, "dtype": "f32",
+                      "data": "synthetic", "vs_baseline": None,
+                      "config": {"workload": "qubit-resonator (reference Benchmark 02 / 04), "
+                                             "wd500 regime, 40 drive periods, mean of a"},
+                      "results": rows}), flush=True)
 
 
 def bench_serving(args):

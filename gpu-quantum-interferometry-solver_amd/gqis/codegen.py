@@ -343,6 +343,9 @@ def emit_grouped(plan, groups, rho0_exprs=None):
             f"#define GQ_NATOMS_ALLOC {max(1, len(plan.time_atoms))}"]
     cases = " ".join(f"case {g}: fn<{g}> args; break;" for g in range(len(blocks)))
     head.append(f"#define GQ_GROUP_DISPATCH(g, fn, args) switch (g) {{ {cases} default: break; }}")
+    # GQ_NBUF LDS copies of the state (set by the caller): the one after buffer b
+    head.append("#ifndef GQ_NBUF\n#define GQ_NBUF 2\n#endif")
+    head.append("#define GQ_NEXT_BUF(b) (GQ_NBUF == 2 ? ((b) ^ 1) : 0)")
     # time atoms, shared by every wave
     anames = {"t": "t_now"}
     for i, c in enumerate(plan.const_syms):

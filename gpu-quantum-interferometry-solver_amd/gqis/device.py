@@ -141,7 +141,8 @@ def run_sweep(*, source, variant, spec, compiled, mode, var_arrays, rho0_values,
         from .solver import grouped_applies
         cands = [(1, b) for b in autotune.CANDIDATES]
         if emit is not None and size >= MIN_GROUPED_STATES:
-            cands += [(g, 64 * g) for g in GROUP_CHOICES if grouped_applies(spec, size, fp64, g)]
+            cands += [(g, 64 * g) for g in GROUP_CHOICES
+                      if grouped_applies(spec, size, fp64, g, natoms)]
 
         def source_of(cand):
             return autotune.with_block(source if cand[0] == 1 else emit(cand[0]), cand[1])

@@ -52,10 +52,15 @@ __device__ __forceinline__ void gq_stage_lds(const double* __restrict__ tt, int 
 
 #define GQ_TT_STRIDE (GQ_NATOMS_ALLOC * 64)
 
+// The wave's rows of v into the LDS copy of the state; with a single copy (GQ_NBUF 1) after a
+// barrier that waits for the readers of the previous contents.
 template <int GI>
 __device__ __forceinline__ void gq_share(real* __restrict__ Y, const st_t* __restrict__ v,
                                          int lane) {
     using P = gq_grp<GI>;
+#if GQ_NBUF == 1
+    __syncthreads();
+#endif
 #pragma unroll
     for (int r = 0; r < P::NR; ++r) Y[(P::R0 + r) * 64 + lane] = (real)v[r];
 }
@@ -120,7 +125,7 @@ __device__ __forceinline__ void gq_adaptive_body(const gq_sweep_args& a,
     real op_re = R(0.0), op_im = R(0.0);
     if constexpr (GI == 0) P::obs(s_y[buf], lane, px, py, C, Sg, Gg, op_re, op_im);
 #endif
-    buf ^= 1;
+    buf = GQ_NEXT_BUF(buf);
 #if GQ_OUT_MEAN_STATE
     acc_t savg[P::NR];
 #pragma unroll
@@ -165,7 +170,7 @@ __device__ __forceinline__ void gq_adaptive_body(const gq_sweep_args& a,
             __syncthreads();   // the first one of the step also publishes tt
             gq_stage_lds<GI>(tt + s * GQ_TT_STRIDE, lane, px, py, C, Sg, Gg);
             P::rhs(s_y[buf], lane, px, py, C, Sg, Gg, k[s]);
-            buf ^= 1;
+            buf = GQ_NEXT_BUF(buf);
         }
         st_t yn[P::NR];
 #pragma unroll
@@ -180,7 +185,7 @@ __device__ __forceinline__ void gq_adaptive_body(const gq_sweep_args& a,
         const real* __restrict__ Yn = s_y[buf];
         gq_share<GI>(s_y[buf], yn, lane);
         __syncthreads();
-        buf ^= 1;
+        buf = GQ_NEXT_BUF(buf);
         real kf[P::NR];
         double err_norm;
 #if GQ_METHOD_DOP853
@@ -288,7 +293,7 @@ __device__ __forceinline__ void gq_adaptive_body(const gq_sweep_args& a,
                     gq_store_complex(a.trace, (long long)kk * ld + q, t_re, t_im);
                 }
             }
-            buf ^= 1;
+            buf = GQ_NEXT_BUF(buf);
 #else
             if (sample) {
 #pragma unroll
@@ -352,7 +357,7 @@ __device__ __forceinline__ void gq_adaptive_body(const gq_sweep_args& a,
                 live = false;
             }
 #if GQ_OUT_FINAL
-            buf ^= 1;
+            buf = GQ_NEXT_BUF(buf);
 #endif
             gq_flag(a.status, bad, GQ_STATUS_NONFINITE_RESULT);
             if constexpr (GI == 0) gq_flag(a.status, done && failed, GQ_STATUS_SUBSTEP_OVERFLOW);
@@ -362,7 +367,7 @@ __device__ __forceinline__ void gq_adaptive_body(const gq_sweep_args& a,
 }
 
 extern "C" __global__ __launch_bounds__(GQ_GW) void GQ_KERNEL(gq_adaptive_grouped)(const gq_sweep_args a) {
-    __shared__ real s_y[2][GQ_D * 64];
+    __shared__ real s_y[GQ_NBUF][GQ_D * 64];
     __shared__ double s_red[GQ_G * 64];
     __shared__ double s_tt[2 * GQ_NS * GQ_TT_STRIDE];
     const int g = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));

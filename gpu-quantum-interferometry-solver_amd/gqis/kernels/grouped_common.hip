@@ -8,11 +8,17 @@
 
 #define GQ_GW (GQ_G * 64)
 
-// The wave's rows of v into the LDS copy of the full vector.
+// The wave's rows of v into the LDS copy of the full vector.  With two alternating copies
+// (GQ_NBUF 2) a copy is rewritten only after every wave has passed the barrier that follows the
+// write of the other one.  With one copy (models whose two copies would not fit the LDS) a
+// barrier first waits until every wave has finished reading the previous contents.
 template <int GI>
 __device__ __forceinline__ void gq_share(real* __restrict__ Y, const real* __restrict__ v,
                                          int lane) {
     using P = gq_grp<GI>;
+#if GQ_NBUF == 1
+    __syncthreads();
+#endif
 #pragma unroll
     for (int r = 0; r < P::NR; ++r) Y[(P::R0 + r) * 64 + lane] = v[r];
 }

@@ -1,6 +1,7 @@
 // Classical RK4 with the state of each parameter point split over GQ_G lanes ("lanes per
 // point", grouped_common.hip).  Two LDS buffers alternate, so one barrier per stage suffices
-// (a buffer is rewritten only after every wave has passed the next stage's barrier).  Each wave
+// (a buffer is rewritten only after every wave has passed the next stage's barrier); a state too
+// large for two copies keeps one, with a second barrier before each write.  Each wave
 // runs the body specialised to its group (GQ_GROUP_DISPATCH on the wave-uniform index), so its
 // statics, stage values and rows are compile-time register arrays.  This keeps large models (63
 // states in FP64, 255 in FP32) in registers, at the cost of one LDS round trip per stage.
@@ -48,7 +49,7 @@ __device__ __forceinline__ void gq_rk4_body(const gq_sweep_args& a, real (*s_y)[
     __syncthreads();
     real k[P::NR];
     P::rhs(s_y[buf], lane, px, py, C, Sg, Gg, k);
-    buf ^= 1;
+    buf = GQ_NEXT_BUF(buf);
     gq_gout<GI> out;
     gq_gout_init<GI>(out);
 
@@ -65,7 +66,7 @@ __device__ __forceinline__ void gq_rk4_body(const gq_sweep_args& a, real (*s_y)[
         gq_share<GI>(s_y[buf], yt, lane);
         __syncthreads();
         P::rhs(s_y[buf], lane, px, py, C, Sg, Gg, k);
-        buf ^= 1;
+        buf = GQ_NEXT_BUF(buf);
 #pragma unroll
         for (int r = 0; r < P::NR; ++r) {
             yt[r] = fma(dt2, k[r], y[r]);
@@ -74,7 +75,7 @@ __device__ __forceinline__ void gq_rk4_body(const gq_sweep_args& a, real (*s_y)[
         gq_share<GI>(s_y[buf], yt, lane);
         __syncthreads();
         P::rhs(s_y[buf], lane, px, py, C, Sg, Gg, k);
-        buf ^= 1;
+        buf = GQ_NEXT_BUF(buf);
 #pragma unroll
         for (int r = 0; r < P::NR; ++r) {
             yt[r] = fma(dt, k[r], y[r]);
@@ -86,7 +87,7 @@ __device__ __forceinline__ void gq_rk4_body(const gq_sweep_args& a, real (*s_y)[
         gq_share<GI>(s_y[buf], yt, lane);
         __syncthreads();
         P::rhs(s_y[buf], lane, px, py, C, Sg, Gg, k);
-        buf ^= 1;
+        buf = GQ_NEXT_BUF(buf);
 #pragma unroll
         for (int r = 0; r < P::NR; ++r) {
             acc[r] += k[r];
@@ -97,15 +98,15 @@ __device__ __forceinline__ void gq_rk4_body(const gq_sweep_args& a, real (*s_y)[
         gq_share<GI>(s_y[buf], y, lane);
         __syncthreads();
         if (step + 1 < num_steps) P::rhs(Ycur, lane, px, py, C, Sg, Gg, k);
-        buf ^= 1;
+        buf = GQ_NEXT_BUF(buf);
         gq_gout_step<GI>(out, a, step, q, valid, Ycur, y, lane, px, py, C, Sg, Gg);
     }
     // the final state is in the LDS buffer written last (the loop flipped past it)
-    gq_gout_final<GI>(out, a, q, valid, s_y[buf ^ 1], y, lane, px, py, C, Sg, Gg);
+    gq_gout_final<GI>(out, a, q, valid, s_y[GQ_NEXT_BUF(buf)], y, lane, px, py, C, Sg, Gg);
 }
 
 extern "C" __global__ __launch_bounds__(GQ_GW) void GQ_KERNEL(gq_rk4_grouped)(const gq_sweep_args a) {
-    __shared__ real s_y[2][GQ_D * 64];
+    __shared__ real s_y[GQ_NBUF][GQ_D * 64];
     const int g = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = (int)(threadIdx.x & 63u);
     const long long q_raw = (long long)blockIdx.x * 64 + lane;

@@ -6,7 +6,7 @@
 //   GQ_SCHEME 1: 2N-storage LSRK4;
 //   GQ_SCHEME 2: Adams-Bashforth 5 with four RK4 start-up steps.
 // Every RHS evaluation shares its input rows through one of two alternating LDS buffers and one
-// barrier.  The state after a step is shared as the input of the next step's first RHS (schemes
+// barrier (or through a single buffer, with a barrier before the write, when two do not fit).  The state after a step is shared as the input of the next step's first RHS (schemes
 // 0 and 2) or, for LSRK4, only when an observable needs it.
 
 template <int GI>
@@ -29,13 +29,13 @@ __device__ __forceinline__ void gq_tab_body(const gq_sweep_args& a, real (*s_y)[
     gq_gout<GI> out;
     gq_gout_init<GI>(out);
     int buf = 0;
-    // RHS of the wave's rows at the shared state v (one LDS buffer, one barrier)
+    // RHS of the wave's rows at the shared state v (one LDS buffer)
     auto eval = [&](const real* __restrict__ v, real* __restrict__ k) -> const real* {
         real* __restrict__ Y = s_y[buf];
         gq_share<GI>(Y, v, lane);
         __syncthreads();
         P::rhs(Y, lane, px, py, C, Sg, Gg, k);
-        buf ^= 1;
+        buf = GQ_NEXT_BUF(buf);
         return Y;
     };
 
@@ -89,7 +89,7 @@ __device__ __forceinline__ void gq_tab_body(const gq_sweep_args& a, real (*s_y)[
         gq_share<GI>(s_y[buf], y, lane);
         __syncthreads();
         Yst = s_y[buf];
-        buf ^= 1;
+        buf = GQ_NEXT_BUF(buf);
     }
 #endif
     for (int step = 0; step < num_steps; ++step) {
@@ -113,7 +113,7 @@ __device__ __forceinline__ void gq_tab_body(const gq_sweep_args& a, real (*s_y)[
         gq_share<GI>(s_y[buf], y, lane);
         __syncthreads();
         Yst = s_y[buf];
-        buf ^= 1;
+        buf = GQ_NEXT_BUF(buf);
 #endif
         gq_gout_step<GI>(out, a, step, q, valid, Yst, y, lane, px, py, C, Sg, Gg);
     }
@@ -182,7 +182,7 @@ __device__ __forceinline__ void gq_tab_body(const gq_sweep_args& a, real (*s_y)[
 }
 
 extern "C" __global__ __launch_bounds__(GQ_GW) void GQ_KERNEL(gq_fixed_grouped)(const gq_sweep_args a) {
-    __shared__ real s_y[2][GQ_D * 64];
+    __shared__ real s_y[GQ_NBUF][GQ_D * 64];
     const int g = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = (int)(threadIdx.x & 63u);
     const long long q_raw = (long long)blockIdx.x * 64 + lane;

@@ -142,7 +142,7 @@ class SweepServer:
             cands = tuple((1, b) for b in autotune.CANDIDATES)
             if size >= device.MIN_GROUPED_STATES:
                 cands += tuple((g, 64 * g) for g in device.GROUP_CHOICES
-                               if grouped_applies(spec, size, r.fp64, g))
+                               if grouped_applies(spec, size, r.fp64, g, natoms))
 
             def variant(c):
                 g, b = c

@@ -512,9 +512,11 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
         # block_size="auto" the tuner measures every grouping instead
         lanes_per_point = (8 if block_size != "auto"
                            and compiled.state_size >= GROUPED_AUTO_MIN_STATES
-                           and grouped_applies(spec, compiled.state_size, fp64, 8) else 1)
+                           and grouped_applies(spec, compiled.state_size, fp64, 8,
+                                               len(compiled.plan.time_atoms)) else 1)
     if lanes_per_point > 1 and not grouped_applies(spec, compiled.state_size, fp64,
-                                                   lanes_per_point):
+                                                   lanes_per_point,
+                                                   len(compiled.plan.time_atoms)):
         raise ValueError(f"lanes_per_point={lanes_per_point} needs a solver with a grouped "
                          f"kernel ({', '.join(sorted(GROUPED_TEMPLATES))}) and a state "
                          f"that splits into {lanes_per_point} row blocks whose LDS copy fits "
@@ -712,18 +714,29 @@ GROUPED_TEMPLATES.update({name: "grouped_tableau.hip" for name in
                            "ab5")})
 
 
-def grouped_applies(spec, d, fp64, groups, natoms=1):
-    """A grouped kernel exists for this solver, the state splits into ``groups`` row blocks and
-    the double-buffered LDS copy of the state (2 x d x 64 reals, plus the adaptive kernels'
-    error-norm partials and shared stage-time atoms, up to 12 stages x ``natoms``, double
-    buffered) fits one compute unit."""
+def grouped_buffers(spec, d, fp64, groups, natoms=0):
+    """LDS copies of the state a grouped kernel keeps: 2 (alternating, one barrier per stage)
+    when both fit one compute unit, 1 (a barrier before each write as well) when one does, 0
+    when no grouped kernel applies -- the solver has no grouped template, the state does not
+    split into ``groups`` non-empty row blocks, or one copy (d x 64 reals) does not fit.  The
+    adaptive kernels also keep their error-norm partials and the FP64 time atoms (``natoms``)
+    of up to 12 stages, double buffered."""
     if spec.name not in GROUPED_TEMPLATES or groups < 2:
-        return False
+        return 0
     rw = -(-d // groups)
-    lds = 2 * d * 64 * (8 if fp64 else 4)
-    if spec.adaptive:
-        lds += groups * 64 * 8 + 2 * 12 * max(1, natoms) * 64 * 8
-    return (groups - 1) * rw < d and lds <= LDS_BYTES_PER_CU
+    if (groups - 1) * rw >= d:
+        return 0
+    copy = d * 64 * (8 if fp64 else 4)
+    extra = (groups * 64 * 8 + 2 * 12 * max(1, natoms) * 64 * 8) if spec.adaptive else 0
+    for n in (2, 1):
+        if n * copy + extra <= LDS_BYTES_PER_CU:
+            return n
+    return 0
+
+
+def grouped_applies(spec, d, fp64, groups, natoms=0):
+    """A grouped kernel exists for this solver and state (``grouped_buffers`` > 0)."""
+    return grouped_buffers(spec, d, fp64, groups, natoms) > 0
 
 
 def _emit_source(compiled, spec, mode, uses_rho0_values, trace, state_trace, fp64, time_table,
@@ -760,6 +773,12 @@ def _emit_source(compiled, spec, mode, uses_rho0_values, trace, state_trace, fp6
         extra = tableaux.emit_fixed(spec.name)
     if groups > 1:
         block_size = 64 * groups
+        nbuf = grouped_buffers(spec, compiled.state_size, fp64, groups,
+                               len(compiled.plan.time_atoms))
+        if not nbuf:
+            raise ValueError(f"no grouped {spec.name} kernel for {compiled.state_size} states in "
+                             f"{groups} row blocks")
+        defines["GQ_NBUF"] = nbuf
     return codegen.assemble(compiled.plan, templates, fp64=fp64, acc64=True,
                             block_size=256 if block_size == "auto" else block_size,
                             defines=defines,

@@ -166,6 +166,59 @@ def multi_qubit(n_qubits, n_eps=64, n_amp=64, n_t=2001, t_end=20.0, delta=1.0, w
     }
 
 
+def qubit_resonator_parameters(regime="wd500"):
+    """Physical constants of the reference's qubit-resonator benchmark
+    (``Benchmarks/Benchmark_02_four_level_Interferometry.py:851-861`` with the defaults of
+    ``:1074-1091``; the same conversion in ``Benchmarks/Benchmark_04_hilbert_space_scaling.py:
+    97-109``): wq1 = 4.71 * 1.15 scales the normalised inputs."""
+    wq1 = 4.71 * 1.15
+    w_over_delta, eps_over_w, amp_over_w = {
+        "wd500": (0.0923105326317733, 22.74020028, 27.8315904255),
+        "wd1500": (0.27693159789532, 7.58006676, 12.0603558511)}[regime]
+    delta = wq1 * 1.0
+    w = w_over_delta * delta
+    gamma1 = 2.0e-3
+    return {"delta": delta, "w": w, "wr2": 7.6767, "gamma1": gamma1, "gamma2": gamma1 / 2.0,
+            "kappa": 5.0e-3, "ap": 2.0e-5 * wq1, "g1": 0.04033970276 / 1.23 * wq1,
+            "eps_max": eps_over_w * w, "amp_max": amp_over_w * w}
+
+
+def qubit_resonator(n_photon=2, n_eps=2048, n_amp=2048, periods=40, steps_per_period=146,
+                    regime="wd500"):
+    """The reference's qubit-resonator model with ``n_photon`` Fock states (dimension
+    2 n_photon; ``Benchmarks/Benchmark_02_four_level_Interferometry.py:140-168`` for
+    n_photon = 2, ``Benchmarks/Benchmark_04_hilbert_space_scaling.py:73-94`` in general):
+    with E = sqrt(delta^2 + Drive^2) and Drive = eps + A cos(w t),
+    H = Ap (a + a^+) + g1 delta (s+ a + s- a^+) / E + sz (E - wr2) / 2 on qubit (x) cavity,
+    s- = [[0,0],[1,0]] on the qubit, a the cavity's annihilation operator; collapse
+    sqrt(gamma1) s-, sqrt(gamma2) sz, sqrt(kappa) a; observable a (complex mean).  The grid is
+    eps in [-eps_max, eps_max], A in [0, A_max]; ``periods`` drive periods of
+    ``steps_per_period`` RK4 steps (146 is the reference's published 2048 x 2048 row,
+    ``Benchmarks/results/Benchmark_02_full_benchmark.csv:26``)."""
+    p = qubit_resonator_parameters(regime)
+    sz = sp.Matrix([[1, 0], [0, -1]])
+    lower = sp.Matrix([[0, 0], [1, 0]])
+    ann = sp.zeros(n_photon)
+    for k in range(1, n_photon):
+        ann[k - 1, k] = float(np.sqrt(k))
+    s_minus = sp.kronecker_product(lower, sp.eye(n_photon))
+    s_z = sp.kronecker_product(sz, sp.eye(n_photon))
+    a = sp.kronecker_product(sp.eye(2), ann)
+    energy = sp.sqrt(p["delta"] ** 2 + DRIVE ** 2)
+    h = (p["ap"] * (a.H + a) + p["g1"] * p["delta"] * (s_minus.H * a + s_minus * a.H) / energy
+         + s_z / 2 * (energy - p["wr2"]))
+    col = [np.sqrt(p["gamma1"]) * s_minus, np.sqrt(p["gamma2"]) * s_z, np.sqrt(p["kappa"]) * a]
+    period = 2.0 * np.pi / p["w"]
+    steps = int(round(periods * steps_per_period))
+    return {
+        "H": h, "Drive": {DRIVE: EPS + AMP * sp.cos(p["w"] * T)}, "Col_Ops": col,
+        "mean_operator": a, "tlist": np.linspace(0.0, periods * period, steps + 1),
+        "var_arrays": {EPS: np.linspace(-p["eps_max"], p["eps_max"], n_eps),
+                       AMP: np.linspace(0.0, p["amp_max"], n_amp)},
+        "params": p,
+    }
+
+
 def serving_stream(count=1000, rate=20.0, seed=7, sides=(32, 128, 512, 1024)):
     """Seeded Poisson stream of sweep-request descriptions: model (two-level LZS or an N-level
     chain, N in {2, 4, 8}), square grid side, amplitude range, solver (rk4 / tsit5a) and

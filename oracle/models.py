@@ -146,6 +146,34 @@ def multi_qubit(n_qubits, px, py, delta=1.0, w_over_delta=1.14, gamma1_per_t=0.0
     return AffineSweep(terms, on(np.diag([0.0, 1.0]).astype(complex), 0), px, py)
 
 
+def qubit_resonator_operators(n_photon):
+    """Qubit (x) cavity operators of the reference's qubit-resonator model
+    (``Benchmarks/Benchmark_04_hilbert_space_scaling.py:63-94``): s- = [[0, 0], [1, 0]] on the
+    qubit, sz on the qubit, a with a[k-1, k] = sqrt(k) on the cavity."""
+    lower = np.array([[0, 0], [1, 0]], complex)
+    ann = np.diag(np.sqrt(np.arange(1, n_photon)), 1).astype(complex)
+    eye = np.eye(n_photon)
+    return np.kron(lower, eye), np.kron(SZ, eye), np.kron(np.eye(2), ann)
+
+
+def qubit_resonator(n_photon, px, py, p):
+    """The qubit-resonator sweep (``Benchmarks/Benchmark_02_four_level_Interferometry.py:
+    140-168``) as three affine terms: Ap (a + a^+) with the collapse operators sqrt(gamma1) s-,
+    sqrt(gamma2) sz, sqrt(kappa) a; g1 delta (s+ a + s- a^+) scaled by 1 / E; sz scaled by
+    (E - wr2) / 2; E = sqrt(delta^2 + (eps + A cos(w t))^2).  ``p`` holds delta, w, wr2,
+    gamma1, gamma2, kappa, ap, g1.  Observable a."""
+    sm, sz, a = qubit_resonator_operators(n_photon)
+
+    def energy(t, x, y):
+        return np.sqrt(p["delta"] ** 2 + (x + y * np.cos(p["w"] * t)) ** 2)
+    col = [np.sqrt(p["gamma1"]) * sm, np.sqrt(p["gamma2"]) * sz, np.sqrt(p["kappa"]) * a]
+    jc = p["g1"] * p["delta"] * (sm.conj().T @ a + sm @ a.conj().T)
+    terms = [(one, p["ap"] * (a + a.conj().T), col),
+             (lambda t, x, y: 1.0 / energy(t, x, y), jc, []),
+             (lambda t, x, y: 0.5 * (energy(t, x, y) - p["wr2"]), sz, [])]
+    return AffineSweep(terms, a, px, py)
+
+
 class Duffing:
     """x'' + delta x' + alpha x + beta x^3 = gamma cos(w t) as (x, v); sweep (gamma, x0)."""
 
