@@ -36,7 +36,7 @@ def _initial_states(rho0_values, nx, ny, size, dtype, device):
     return r.reshape(nx * ny, size).contiguous()
 
 
-GROUP_CHOICES = (2, 4, 8)
+GROUP_CHOICES = (2, 4, 8, 16)
 MIN_GROUPED_STATES = 24     # below this one lane holds a point's state without spills
 
 

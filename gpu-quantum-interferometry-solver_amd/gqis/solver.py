@@ -239,7 +239,7 @@ def mesolve_2D(H, Drive=None, Col_Ops=None, mean_operator=None, tlist=None,
     ``hiprtc_options`` (extra compiler flags), ``time_table`` (FP64 table of the time-only drive
     terms for fixed-step solvers; False evaluates them per point as the reference does) and
     ``block_size`` (threads per work-group, or ``"auto"``: measured once per kernel and device)
-    ``lanes_per_point`` (1, 2, 4 or 8 lanes, one per wavefront, share each point's state rows;
+    ``lanes_per_point`` (1, 2, 4, 8 or 16 lanes, one per wavefront, share each point's state rows;
     "auto": 8 for states of 48 or more reals where the grouped kernel applies, else 1;
     ``block_size="auto"`` also measures 2, 4, 8 for models of 24 or more states) and
     ``liouville`` ("auto", "lanes", "mfma", or "mfma_f16" / "mfma_bf16": the commutator product
@@ -361,8 +361,8 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
             raise ValueError("block_size must be 64, 128, 256, 512, 1024 or 'auto'.")
     if lanes_per_point != "auto":
         lanes_per_point = int(lanes_per_point)
-        if lanes_per_point not in (1, 2, 4, 8):
-            raise ValueError("lanes_per_point must be 1, 2, 4, 8 or 'auto'.")
+        if lanes_per_point not in (1, 2, 4, 8, 16):
+            raise ValueError("lanes_per_point must be 1, 2, 4, 8, 16 or 'auto'.")
 
     var_arrays = {} if var_arrays is None else dict(var_arrays)
     for sym, arr in ({} if rho0_var_arrays is None else dict(rho0_var_arrays)).items():
@@ -507,13 +507,15 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
         return src
 
     if lanes_per_point == "auto":
-        # measured on gfx950 at 63 states: 8 lanes per point 1.2x (RK4 FP32), 1.85x (RK4
-        # FP64) and 8.7x (Tsit5A FP64) faster than one lane, which spills; with
-        # block_size="auto" the tuner measures every grouping instead
-        lanes_per_point = (8 if block_size != "auto"
-                           and compiled.state_size >= GROUPED_AUTO_MIN_STATES
-                           and grouped_applies(spec, compiled.state_size, fp64, 8,
-                                               len(compiled.plan.time_atoms)) else 1)
+        # with block_size="auto" the tuner measures every grouping instead
+        points = 1
+        for v in var_arrays.values():
+            points *= int(v.numel()) if hasattr(v, "numel") else int(np.asarray(v).size)
+        lanes_per_point = (1 if block_size == "auto" else
+                           auto_lanes_per_point(spec, compiled.state_size, fp64,
+                                                compiled.plan.nnz if compiled.plan.affine
+                                                else None, points,
+                                                len(compiled.plan.time_atoms)))
     if lanes_per_point > 1 and not grouped_applies(spec, compiled.state_size, fp64,
                                                    lanes_per_point,
                                                    len(compiled.plan.time_atoms)):
@@ -712,6 +714,35 @@ GROUPED_TEMPLATES = {"rk4": "grouped_rk4.hip", "tsit5a": "grouped_adaptive.hip",
 GROUPED_TEMPLATES.update({name: "grouped_tableau.hip" for name in
                           ("dp5", "tsit5", "dop853", "alshina6", "anas5", "lserk4", "lserk4f",
                            "ab5")})
+
+
+def auto_lanes_per_point(spec, d, fp64, nnz, points, natoms=0):
+    """Default wavefronts per point (1 = the one-lane kernel), from measurements on gfx950 of RK4
+    and Tsit5A over chain, random and qubit-resonator models of 63 to 399 states:
+
+    * rows with many coefficients (``nnz`` >= 16 d, e.g. a dense 8-level H) or a non-affine
+      RHS (``nnz`` None): 8 lanes from 48 states (the one-lane kernel spills; 1.2x FP32, 1.8x
+      FP64 at 63 states);
+    * sparse rows, RK4: one lane while a point's state is at most 128 words (FP32 words; FP64
+      counts two) and the sweep gives every SIMD a wavefront (65536 points) -- the grouped
+      kernels' LDS round trips cost more than they save there (1.4x at 63 FP32 states) --
+      otherwise the fewest lanes that leave each wavefront at most 36 words of a vector (4
+      lanes at 143 FP32 states, 8 at 255, 16 at 399: 1.75x over 8 there);
+    * sparse rows, other solvers (more stage vectors): 8 lanes from 48 states, 16 where 8 would
+      leave a wavefront more than 36 words."""
+    words = 2 if fp64 else 1
+    if d < GROUPED_AUTO_MIN_STATES:
+        return 1
+    sparse = nnz is not None and nnz < 16 * d
+    if not sparse or spec.name != "rk4":
+        groups = 16 if sparse and -(-d // 8) * words > 36 else 8
+    elif d * words <= 128 and points >= 65536:
+        groups = 1
+    else:
+        groups = next((g for g in (2, 4, 8) if -(-d // g) * words <= 36), 16)
+    while groups > 1 and not grouped_applies(spec, d, fp64, groups, natoms):
+        groups //= 2
+    return groups
 
 
 def grouped_buffers(spec, d, fp64, groups, natoms=0):

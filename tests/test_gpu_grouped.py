@@ -22,7 +22,7 @@ def _rel(got, ref):
 
 
 @pytest.mark.parametrize("mode", ["mean", "final", "final_rho"])
-@pytest.mark.parametrize("groups", [2, 4, 8])
+@pytest.mark.parametrize("groups", [2, 4, 8, 16])
 def test_grouped_matches_per_lane_fp64(gpu, mode, groups):
     args, var = _chain8()
     ref = mesolve_2D(*args, var_arrays=var, fp64=True, output_mode=mode, liouville="lanes",
@@ -54,7 +54,7 @@ def test_grouped_fp32_matches_per_lane(gpu):
     assert _rel(got, ref) <= 1e-5
 
 
-@pytest.mark.parametrize("groups", [2, 4, 8])
+@pytest.mark.parametrize("groups", [2, 4, 8, 16])
 def test_grouped_random_8_level_golden_fixture(gpu, groups):
     g = np.load(os.path.join(GOLDEN, "c3_random_8_level_8x8_subset_rk4_mean_fp64.npz"))
     w = workloads.random_n_level(8, seed=int(g["seed"]), n_eps=8, n_amp=8)
@@ -87,7 +87,7 @@ def test_autotuner_considers_grouped_kernels(gpu):
                            block_size="auto", return_timing_info=True)
     print("autotune", info["lanes_per_point"], info["block_size"], info.get("autotune"),
           flush=True)
-    assert {c[0] for c in info["autotune"]} == {1, 2, 4, 8}
+    assert {c[0] for c in info["autotune"]} == {1, 2, 4, 8, 16}
     assert _rel(got, ref) <= 1e-13
 
 
@@ -155,16 +155,21 @@ def test_autotuner_considers_grouped_adaptive_kernels(gpu):
     got, info = mesolve_2D(*args, block_size="auto", return_timing_info=True, **kw)
     print("autotune adaptive", info["lanes_per_point"], info["block_size"],
           info.get("autotune"), flush=True)
-    assert {c[0] for c in info["autotune"]} == {1, 2, 4, 8}
+    assert {c[0] for c in info["autotune"]} == {1, 2, 4, 8, 16}
     assert _rel(got, ref) <= 1e-7
 
 
 def test_auto_lanes_per_point_default(gpu):
-    # 63 states: the default takes 8 lanes per point for the tridiagonal chain; 15 states stay
-    # on one lane; a dense 8-level H in FP64 takes the matrix-core path instead
+    # 63 states of the tridiagonal chain (5 coefficients per row): on a small grid the default
+    # spreads each point over 4 lanes (16 FP64 rows per wavefront); on a grid that gives every
+    # SIMD a wavefront, FP32 stays on one lane.  15 states stay on one lane; a dense 8-level H
+    # in FP64 takes the matrix-core path
     args, var = _chain8(n_eps=8, n_amp=8, n_t=101, t_end=5.0)
     _, info = mesolve_2D(*args, var_arrays=var, fp64=True, return_timing_info=True)
-    assert info["lanes_per_point"] == 8 and info["kernel"] == "gq_rk4_grouped_f64"
+    assert info["lanes_per_point"] == 4 and info["kernel"] == "gq_rk4_grouped_f64"
+    args, var = _chain8(n_eps=256, n_amp=256, n_t=11, t_end=0.25)
+    _, info = mesolve_2D(*args, var_arrays=var, return_timing_info=True)
+    assert info["lanes_per_point"] == 1 and info["kernel"] == "gq_rk4_f32"
     w = workloads.random_n_level(8, n_eps=8, n_amp=8, n_t=101, t_end=1.0)
     _, info = mesolve_2D(w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"],
                          var_arrays=w["var_arrays"], fp64=True, return_timing_info=True)
