@@ -156,6 +156,44 @@ def _names_for(plan):
     return names
 
 
+def _atoms_stage_lines(plan, const_double, const_real):
+    """``gq_atoms_stage``: the time atoms at an FP64 time, in the kernel's real type (the
+    adaptive kernels' stage times).  FP64 kernels evaluate them in FP64.  FP32 kernels reduce
+    each sin / cos of an affine function of t to [-pi, pi] in FP64 (two-part 2 pi, Cody and
+    Waite) and take the FP32 function of the reduced angle: FP32 accuracy, as the FP32 RHS,
+    for a fraction of the FP64 evaluation.  Atoms with t outside such a function stay FP64."""
+    lines = ["__device__ __forceinline__ void gq_atoms_stage(const double t_now, "
+             "const real* __restrict__ C, real* __restrict__ TT) {",
+             "    (void)t_now; (void)C; (void)TT;", "#if GQ_FP64",
+             "    gq_atoms_f64(t_now, C, TT);", "#else"]
+    dnames = {"t": "t_now", **const_double}
+    dpr = HipPrinter(dnames, literal="DBL")
+    rnames = dict(const_real)
+    reduced = {}
+    for i, (_, e) in enumerate(plan.time_atoms):
+        tsym = next((x for x in e.free_symbols if x.name == "t"), None)
+        subs = {}
+        for f in e.atoms(sp.sin, sp.cos):
+            arg = f.args[0]
+            if (tsym is not None and arg.is_polynomial(tsym)
+                    and sp.degree(arg, tsym) <= 1):
+                if f not in reduced:
+                    name = f"q{len(reduced)}"
+                    reduced[f] = sp.Symbol(name, real=True)
+                    rnames[name] = name
+                    fn = "cos" if isinstance(f, sp.cos) else "sin"
+                    lines.append(f"    const real {name} = {fn}((real)gq_reduce_2pi("
+                                 f"{dpr.doprint(arg)}));")
+                subs[f] = reduced[f]
+        rest = e.xreplace(subs)
+        if subs and not any(x.name == "t" for x in rest.free_symbols):
+            lines.append(f"    TT[{i}] = {HipPrinter(rnames).doprint(rest)};")
+        else:
+            lines.append(f"    TT[{i}] = (real)({dpr.doprint(e)});")
+    lines += ["#endif", "}"]
+    return lines
+
+
 def emit_model(plan, rho0_exprs=None):
     """Device functions of one model (the part the reference prints from SymPy,
     ``gqis/solver.py:633-700,1649-1692``)."""
@@ -192,6 +230,9 @@ def emit_model(plan, rho0_exprs=None):
         lines.append("    (void)t_now; (void)C; (void)TT;")
         lines += [f"    TT[{i}] = {apr.doprint(e)};" for i, (_, e) in enumerate(plan.time_atoms)]
         lines.append("}")
+    lines += _atoms_stage_lines(plan, {s.name: f"(double)C[{i}]" for i, s in
+                                       enumerate(plan.const_syms)},
+                                {s.name: f"C[{i}]" for i, s in enumerate(plan.const_syms)})
     # stage values: once per distinct stage time
     lines.append("__device__ __forceinline__ void gq_stage(const real px, const real py, "
                  "const real* __restrict__ C, const gq_static_t& S, const real* __restrict__ TT, "
@@ -365,6 +406,9 @@ def emit_grouped(plan, groups, rho0_exprs=None):
     head.append("    (void)t_now; (void)C; (void)TT;")
     head += [f"    TT[{i}] = {rpr.doprint(e)};" for i, (_, e) in enumerate(plan.time_atoms)]
     head.append("}")
+    head += _atoms_stage_lines(plan, {c.name: f"(double)C[{i}]" for i, c in
+                                      enumerate(plan.const_syms)},
+                               {c.name: f"C[{i}]" for i, c in enumerate(plan.const_syms)})
     return "\n".join(head + parts)
 
 
@@ -391,7 +435,12 @@ def prelude(fp64, acc64, block_size, defines):
            f"#define GQ_KSUFFIX {kernel_suffix(fp64)}",
            "#define GQ_KERNEL(name) GQ_CAT(name, GQ_KSUFFIX)",
            f"#define GQ_BLOCK {int(block_size)}",
-           "__device__ __forceinline__ real gq_sign(real x) { return (x > R(0)) - (x < R(0)); }"]
+           "__device__ __forceinline__ real gq_sign(real x) { return (x > R(0)) - (x < R(0)); }",
+           "// x - 2 pi rint(x / 2 pi) in FP64, 2 pi split in two parts (Cody and Waite)",
+           "__device__ __forceinline__ double gq_reduce_2pi(double x) {",
+           "    const double k = rint(x * 0.15915494309189535);",
+           "    return fma(-k, 2.4492935982947064e-16, fma(-k, 6.283185307179586, x));",
+           "}"]
     for k, v in defines.items():
         out.append(f"#define {k} {v}")
     out += ["#ifndef GQ_HILBERT", "#define GQ_HILBERT 0", "#endif"]

@@ -24,14 +24,13 @@ typedef real st_t;
 
 #define GQ_NS GQ_NSTAGES
 
+// stage values at an FP64 time (the atoms from gq_atoms_stage: FP64, or FP32 functions of
+// FP64-reduced angles in FP32 kernels)
 __device__ __forceinline__ void gq_stage_at(double t_abs, real px, real py,
                                             const real* __restrict__ C, const gq_static_t& S,
                                             gq_stage_t& G) {
-    double TTd[GQ_NATOMS_ALLOC];
-    gq_atoms_f64(t_abs, C, TTd);
     real TT[GQ_NATOMS_ALLOC];
-#pragma unroll
-    for (int i = 0; i < GQ_NATOMS_ALLOC; ++i) TT[i] = (real)TTd[i];
+    gq_atoms_stage(t_abs, C, TT);
     gq_stage(px, py, C, S, TT, G);
 }
 

@@ -13,10 +13,11 @@
 // the same barriers.  Points are assigned one per lane for the whole integration (no work
 // stealing across groups).
 //
-// The FP64 time atoms of a step's stage times (about 95 instructions per sin / cos) are the
-// same in every wave: wave (s - 1) % GQ_G evaluates those of stage s at the step start and
-// writes them to LDS before the step's first barrier, after which every wave reads them
-// (s_tt, double-buffered by attempt parity).  The values are those of the direct evaluation.
+// The time atoms of a step's stage times (gq_atoms_stage: about 95 instructions per FP64
+// sin / cos; an FP64 reduction and an FP32 function in FP32 kernels) are the same in every
+// wave: wave (s - 1) % GQ_G evaluates those of stage s at the step start and writes them to
+// LDS before the step's first barrier, after which every wave reads them (s_tt,
+// double-buffered by attempt parity).  The values are those of the one-lane kernel.
 
 #if GQ_STATE_FP64
 typedef double st_t;
@@ -31,11 +32,8 @@ template <int GI>
 __device__ __forceinline__ void gq_stage_at(double t_abs, real px, real py,
                                             const real* __restrict__ C,
                                             const real* __restrict__ Sg, real* __restrict__ Gg) {
-    double TTd[GQ_NATOMS_ALLOC];
-    gq_atoms_f64(t_abs, C, TTd);
     real TT[GQ_NATOMS_ALLOC];
-#pragma unroll
-    for (int i = 0; i < GQ_NATOMS_ALLOC; ++i) TT[i] = (real)TTd[i];
+    gq_atoms_stage(t_abs, C, TT);
     gq_grp<GI>::stage(px, py, C, Sg, TT, Gg);
 }
 
@@ -145,10 +143,11 @@ __device__ __forceinline__ void gq_adaptive_body(const gq_sweep_args& a,
 #pragma unroll
         for (int s = 1; s < GQ_NS; ++s) {
             if ((s - 1) % GQ_G == GI) {
-                double TTd[GQ_NATOMS_ALLOC];
-                gq_atoms_f64(a.t_in + (t0 + GQ_C[s] * hs), C, TTd);
+                real TTs[GQ_NATOMS_ALLOC];
+                gq_atoms_stage(a.t_in + (t0 + GQ_C[s] * hs), C, TTs);
 #pragma unroll
-                for (int i = 0; i < GQ_NATOMS; ++i) tt[s * GQ_TT_STRIDE + i * 64 + lane] = TTd[i];
+                for (int i = 0; i < GQ_NATOMS; ++i)
+                    tt[s * GQ_TT_STRIDE + i * 64 + lane] = (double)TTs[i];
             }
         }
 #endif

@@ -1,0 +1,103 @@
+"""The adaptive kernels' stage-time atoms (``gq_atoms_stage``), compiled for the CPU from the
+emitted model code: in FP32 kernels each sin / cos of an affine function of t is reduced to
+[-pi, pi] in FP64 and evaluated in FP32, so the atoms keep FP32 accuracy at any t; FP64 kernels
+evaluate them in FP64.  A wrong part of 2 pi, a missing reduction or a sin / cos swap fails
+these."""
+import ctypes
+import subprocess
+
+import numpy as np
+import pytest
+import sympy as sp
+
+from gqis import codegen, workloads
+from gqis.solver import _compile_rhs
+
+HARNESS = r"""
+#include <cmath>
+using std::cos; using std::sin; using std::exp; using std::sqrt; using std::pow;
+using std::fabs; using std::fmin; using std::fmax; using std::rint; using std::fma;
+#define __device__
+#define __forceinline__ inline
+#define __restrict__
+typedef %(real)s real;
+#define R(x) ((real)(x))
+#define DBL(x) ((double)(x))
+#define GQ_FP64 %(fp64)d
+inline real gq_sign(real x) { return (x > R(0)) - (x < R(0)); }
+%(reduce)s
+%(model)s
+extern "C" void atoms(const double* t, int n, const double* C, double* out) {
+    real Cr[8];
+    for (int i = 0; i < 8; ++i) Cr[i] = (real)C[i];
+    for (int j = 0; j < n; ++j) {
+        real TT[GQ_NATOMS_ALLOC];
+        gq_atoms_stage(t[j], Cr, TT);
+        for (int i = 0; i < GQ_NATOMS; ++i) out[j * GQ_NATOMS + i] = (double)TT[i];
+    }
+}
+"""
+
+
+def _reduce_source():
+    # the device definition, as the kernels' prelude emits it
+    pre = codegen.prelude(False, True, 256, {})
+    body = pre[pre.index("__device__ __forceinline__ double gq_reduce_2pi"):]
+    return body.replace("__device__ __forceinline__", "inline")
+
+
+def _atoms_lib(tmp_path, plan, fp64):
+    src = tmp_path / f"atoms_{int(fp64)}.cpp"
+    src.write_text(HARNESS % {"real": "double" if fp64 else "float", "fp64": int(fp64),
+                              "reduce": _reduce_source(), "model": codegen.emit_model(plan)})
+    lib = tmp_path / f"atoms_{int(fp64)}.so"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-ffp-contract=off", "-shared", "-fPIC", "-w",
+                    str(src), "-o", str(lib)], check=True)
+    fn = ctypes.CDLL(str(lib)).atoms
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    return fn
+
+
+def _run(fn, t, natoms, consts=()):
+    c = np.zeros(8)
+    c[:len(consts)] = consts
+    out = np.zeros((len(t), natoms))
+    fn(t.ctypes.data, len(t), c.ctypes.data, out.ctypes.data)
+    return out
+
+
+def test_fp32_atoms_keep_fp32_accuracy_at_large_times(tmp_path):
+    # the qubit resonator's atoms cos(w t)^2 and cos(w t), w = 0.5, over 40 drive periods and
+    # far beyond (t up to 1e5): FP32 accuracy of the reduced angle, where cos of the FP32
+    # time would be off by up to 4e-3
+    w = workloads.qubit_resonator(2, n_eps=2, n_amp=2)
+    compiled, _ = _compile_rhs(w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"],
+                               sweep_symbols=tuple(w["var_arrays"]))
+    plan = compiled.plan
+    assert [str(e) for _, e in plan.time_atoms] == ["cos(0.5*t)**2", "cos(0.5*t)"]
+    t = np.concatenate([np.linspace(0.0, 40 * 4 * np.pi, 4001),
+                        np.random.default_rng(5).uniform(1e4, 1e5, 2000)])
+    want = np.stack([np.cos(0.5 * t) ** 2, np.cos(0.5 * t)], axis=1)
+    got32 = _run(_atoms_lib(tmp_path, plan, False), t, 2)
+    assert np.max(np.abs(got32 - want)) < 3e-7
+    naive = np.cos(t.astype(np.float32).astype(np.float64) * 0.5)
+    assert np.max(np.abs(naive - want[:, 1])) > 1e-3          # what the reduction avoids
+    got64 = _run(_atoms_lib(tmp_path, plan, True), t, 2)
+    assert np.max(np.abs(got64 - want)) < 1e-12
+
+
+def test_fp32_atoms_sin_with_runtime_constant_and_non_trig_atoms(tmp_path):
+    # sin(c t + 1) with c a runtime constant, and exp(-t) (t outside a trig function: FP64,
+    # rounded to FP32).  An FP32 kernel holds c in FP32, so the atoms are those of float(c).
+    from gqis.symbolic import TIME, make_plan
+    t_sym, c, y = TIME, sp.Symbol("c", real=True), sp.Symbol("y0", real=True)
+    plan = make_plan([sp.sin(c * t_sym + 1) * y + sp.exp(-t_sym / 7)], sp.Integer(0),
+                     sp.Integer(0), [y], {}, (c,))
+    atoms = {str(e) for _, e in plan.time_atoms}
+    assert any("sin" in a for a in atoms) and any("exp" in a for a in atoms)
+    t = np.random.default_rng(6).uniform(0.0, 3e4, 3000)
+    got = _run(_atoms_lib(tmp_path, plan, False), t, len(plan.time_atoms), consts=(2.3,))
+    for i, (_, e) in enumerate(plan.time_atoms):
+        f = sp.lambdify((t_sym, c), e, "numpy")
+        want = np.broadcast_to(f(t, float(np.float32(2.3))), t.shape)
+        assert np.max(np.abs(got[:, i] - want)) < 3e-7 * max(1.0, float(np.max(np.abs(want))))

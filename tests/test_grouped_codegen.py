@@ -26,7 +26,9 @@ using std::fabs; using std::fmin; using std::fmax;
 typedef double real;
 #define R(x) ((real)(x))
 #define DBL(x) ((double)(x))
+#define GQ_FP64 1
 inline real gq_sign(real x) { return (x > R(0)) - (x < R(0)); }
+inline double gq_reduce_2pi(double x) { return x; }
 namespace lane {
 %(lane)s
 }
