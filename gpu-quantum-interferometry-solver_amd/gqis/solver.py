@@ -728,13 +728,17 @@ def auto_lanes_per_point(spec, d, fp64, nnz, points, natoms=0):
       kernels' LDS round trips cost more than they save there (1.4x at 63 FP32 states) --
       otherwise the fewest lanes that leave each wavefront at most 36 words of a vector (4
       lanes at 143 FP32 states, 8 at 255, 16 at 399: 1.75x over 8 there);
+    * sparse rows, adaptive solvers: one lane up to 64 words on such a sweep (0.55 s against
+      0.85 s with 8 lanes for Tsit5A at 63 FP32 states), else as the fixed-step tableaux;
     * sparse rows, other solvers (more stage vectors): 8 lanes from 48 states, 16 where 8 would
       leave a wavefront more than 36 words."""
     words = 2 if fp64 else 1
     if d < GROUPED_AUTO_MIN_STATES:
         return 1
     sparse = nnz is not None and nnz < 16 * d
-    if not sparse or spec.name != "rk4":
+    if sparse and spec.adaptive and d * words <= 64 and points >= 65536:
+        groups = 1
+    elif not sparse or spec.name != "rk4":
         groups = 16 if sparse and -(-d // 8) * words > 36 else 8
     elif d * words <= 128 and points >= 65536:
         groups = 1
