@@ -350,9 +350,21 @@ def emit_grouped(plan, groups, rho0_exprs=None):
                      "const real py, const real* __restrict__ C, const real* __restrict__ Sg, "
                      "const real* __restrict__ Gg, real* __restrict__ dy) {")
         parts.append("        (void)Y; (void)lane; (void)px; (void)py; (void)C; (void)Sg; (void)Gg;")
-        parts += [f"        const real y{j} = Y[{j} * 64 + lane];" for j in cols]
-        parts += [f"        const real {sym.name} = {pr.doprint(e)};" for sym, e in temps]
-        parts += [f"        dy[{i - lo}] = {pr.doprint(e)};" for i, e in zip(range(lo, hi), exprs)]
+        if temps:
+            parts += [f"        const real y{j} = Y[{j} * 64 + lane];" for j in cols]
+            parts += [f"        const real {sym.name} = {pr.doprint(e)};" for sym, e in temps]
+            parts += [f"        dy[{i - lo}] = {pr.doprint(e)};"
+                      for i, e in zip(range(lo, hi), exprs)]
+        else:
+            # each LDS value is read just before the first row that uses it, so a value is
+            # live from its first to its last use rather than through the whole evaluation
+            loaded = set()
+            for i, e in zip(range(lo, hi), exprs):
+                first = sorted({state_names[s.name] for s in e.free_symbols
+                                if s.name in state_names} - loaded)
+                loaded.update(first)
+                parts += [f"        const real y{j} = Y[{j} * 64 + lane];" for j in first]
+                parts.append(f"        dy[{i - lo}] = {pr.doprint(e)};")
         parts.append("    }")
         if g == 0:
             obs_cols = sorted({state_names[s.name] for e in obs + [t for _, t in obs_temps]

@@ -726,8 +726,10 @@ def auto_lanes_per_point(spec, d, fp64, nnz, points, natoms=0):
     * sparse rows, RK4: one lane while a point's state is at most 128 words (FP32 words; FP64
       counts two) and the sweep gives every SIMD a wavefront (65536 points) -- the grouped
       kernels' LDS round trips cost more than they save there (1.4x at 63 FP32 states) --
-      otherwise the fewest lanes that leave each wavefront at most 36 words of a vector (4
-      lanes at 143 FP32 states, 8 at 255, 16 at 399: 1.75x over 8 there);
+      otherwise the fewest lanes that leave each wavefront at most 50 words of a vector and
+      give a compute unit at least 8 wavefronts (work-groups per CU are limited by their LDS
+      copies of the state): 4 lanes at 143 FP32 states, 8 from 195 to 399 (at 195, 4 lanes
+      fit one work-group per CU and take 3.74 s against 2.81 s);
     * sparse rows, adaptive solvers: one lane up to 64 words on such a sweep (0.55 s against
       0.85 s with 8 lanes for Tsit5A at 63 FP32 states), else as the fixed-step tableaux;
     * sparse rows, other solvers (more stage vectors): 8 lanes from 48 states, 16 where 8 would
@@ -743,7 +745,11 @@ def auto_lanes_per_point(spec, d, fp64, nnz, points, natoms=0):
     elif d * words <= 128 and points >= 65536:
         groups = 1
     else:
-        groups = next((g for g in (2, 4, 8) if -(-d // g) * words <= 36), 16)
+        def fits(g):
+            nbuf = grouped_buffers(spec, d, fp64, g, natoms)
+            per_cu = LDS_BYTES_PER_CU // max(1, nbuf * d * 64 * 4 * words)
+            return nbuf and -(-d // g) * words <= 50 and g * per_cu >= 8
+        groups = next((g for g in (2, 4, 8) if fits(g)), 16)
     while groups > 1 and not grouped_applies(spec, d, fp64, groups, natoms):
         groups //= 2
     return groups
