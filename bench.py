@@ -77,30 +77,95 @@ def measured_traffic():
         return None
 
 
-def cpu_baseline(seconds_budget=15.0):
+# The GPU box gives one GPU's process a share of 16 host cores, whatever the machine has; CPU
+# baselines use at most that many threads and say so.
+CPU_SHARE = 16
+
+
+def host_cpu():
+    """CPU model, logical and physical core counts, and the cores this process may use."""
+    model, cores = None, set()
+    phys = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                key, _, val = line.partition(":")
+                key, val = key.strip(), val.strip()
+                if key == "model name" and model is None:
+                    model = val
+                elif key == "physical id":
+                    phys = val
+                elif key == "core id":
+                    cores.add((phys, val))
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0))
+    return {"model": model, "logical_cores": os.cpu_count(), "physical_cores": len(cores) or None,
+            "usable_cores": usable, "threads": min(usable, CPU_SHARE),
+            "thread_cap": f"min(usable cores, {CPU_SHARE}): the per-GPU host share of the box"}
+
+
+def gpu_state():
+    """Clocks, power and temperature of GPU 0 from rocm-smi (None where it is unavailable)."""
+    import subprocess
+    try:
+        out = subprocess.run(["rocm-smi", "--showclocks", "--showpower", "--showtemp", "--json"],
+                             capture_output=True, text=True, timeout=30).stdout
+        cards = json.loads(out)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        return None
+    card = cards.get("card0") if isinstance(cards, dict) else None
+    if not isinstance(card, dict):
+        return None
+    keep = ("clock", "power", "temperature")
+    return {k: v for k, v in card.items() if any(w in k.lower() for w in keep)}
+
+
+def cpu_baseline(grid=2048, seconds_budget=15.0):
     """FP64 C oracle (RK4, OpenMP over points) on a bounded sample of the same workload:
-    whole eps columns (2048 A values each) at the full 2000 steps, as many columns as fit the
-    time budget (at least one)."""
+    eps columns spread evenly over the grid (2048 A values each) at the full 2000 steps, as
+    many as fit the time budget (1 to 64).  Returns the baseline record and the sample
+    (column indices, oracle map) for the GPU maps' error."""
     import numpy as np
     from oracle import cbridge, models
-    threads = len(os.sched_getaffinity(0))
-    threads = min(threads, 16)
-    eps = np.linspace(-15.0, 15.0, 2048)
-    amp = np.linspace(0.0, 30.0, 2048)
+    host = host_cpu()
+    threads = host["threads"]
+    eps = np.linspace(-15.0, 15.0, grid)
+    amp = np.linspace(0.0, 30.0, grid)
     probe = models.four_level_chain(eps[:1], amp[:64])
     t0 = time.perf_counter()
     cbridge.rk4_affine(probe, 0.05, 2000, threads=1)
     per_point = (time.perf_counter() - t0) / 64
-    cols = max(1, int(seconds_budget * threads / (per_point * 2048)))
-    cols = min(cols, 64)
-    m = models.four_level_chain(eps[:cols], amp)
+    cols = max(1, int(seconds_budget * threads / (per_point * grid)))
+    cols = min(cols, 64, grid)
+    idx = np.unique(np.linspace(0, grid - 1, cols).round().astype(int))
+    m = models.four_level_chain(eps[idx], amp)
     t0 = time.perf_counter()
-    cbridge.rk4_affine(m, 0.05, 2000, threads=threads)
+    ref = np.asarray(cbridge.rk4_affine(m, 0.05, 2000, threads=threads)).reshape(len(idx), grid)
     dt = time.perf_counter() - t0
-    return {"value": cols * 2048 * 2000 / dt, "unit": "param-point RK steps/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{cols} eps column(s) x 2048 A values x 2000 RK4 steps, FP64 C oracle "
-                      f"(OpenMP, {threads} threads), {dt:.1f} s"}
+    record = {"value": len(idx) * grid * 2000 / dt, "unit": "param-point RK steps/s",
+              "cores": threads, "kind": "port",
+              "sample": f"{len(idx)} eps columns spread over the {grid} x {grid} grid x {grid} "
+                        f"A values "
+                        f"x 2000 RK4 steps, FP64 C oracle (OpenMP, {threads} threads), {dt:.1f} s",
+              "host": host}
+    return record, idx, ref
+
+
+def map_error(gpu_map, idx, ref, stable):
+    """Max absolute and relative (to the map's largest value) deviation of a GPU map's columns
+    ``idx`` from the oracle's, over the RK4-stable points and over all finite points."""
+    import numpy as np
+    got = np.asarray(gpu_map)[idx].astype(np.complex128)
+    err = np.abs(got - ref)
+    fin = np.isfinite(err)
+    gate = fin & np.asarray(stable)[idx]
+    scale = float(np.max(np.abs(ref[gate]))) if gate.any() else float("nan")
+    worst = float(np.max(err[gate])) if gate.any() else float("nan")
+    return {"max_abs": worst, "max_rel_to_map_max": worst / scale, "map_max": scale,
+            "frac_over_1e-6": float(np.mean(err[gate] > 1e-6)) if gate.any() else None,
+            "max_abs_all_finite": float(np.max(err[fin])) if fin.any() else None,
+            "points": int(gate.sum()), "columns": len(idx)}
 
 
 def reference_cpu_path():
@@ -110,7 +175,7 @@ def reference_cpu_path():
     The sample's results are compared with the C oracle's."""
     import numpy as np
     from oracle import cbridge, models, python_cpu
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    cores = host_cpu()["threads"]
     eps = np.linspace(-15.0, 15.0, 2048)[::128]
     amp = np.linspace(0.0, 30.0, 2048)[::32]
     res, compute_s, workers = python_cpu.sweep("four_level", eps, amp, 0.05, 2000,
@@ -223,11 +288,14 @@ def main():
     log(f"FP64 headline form: warmup x{args.warmup} ...")
     run_sweeps(w, True, args.warmup)
     torch.cuda.synchronize()
+    gpu_before = gpu_state()
     log(f"FP64 timed x{args.steps} ...")
     t0 = time.perf_counter()
-    _, info_t, lat_t = run_sweeps(w, True, args.steps)
+    res_t, info_t, lat_t = run_sweeps(w, True, args.steps)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    gpu_after = gpu_state()
+    res_t = res_t.cpu().numpy()
 
     finite = np.isfinite(ref64) & np.isfinite(res32)
     gate = finite & stable
@@ -251,7 +319,7 @@ def main():
                                f"2000 steps dt=0.05, mean of |0><0|",
                    "points": npts, "rk_steps": nsteps, "solver": "rk4",
                    "precision": "fp64 (time-only drive terms tabulated per launch)"},
-        "error_vs_fp64_oracle": 0.0,
+        "error_vs_fp64_oracle": None,
         "latency_s": {"p50": float(np.percentile(lat_t, 50)),
                       "p99": float(np.percentile(lat_t, 99)), "samples": len(lat_t)},
         "kernel_s": {"fp64_tabulated": kernel_s, "plain_fp64": plain, "fp32": k32},
@@ -267,10 +335,15 @@ def main():
                      "unit": "TFLOP/s", "frac": achieved / PEAK_TFLOPS["fp64"],
                      "traffic": measured_traffic() if args.grid == 2048 else None,
                      "flops_per_point_step": flops_ps, "kernel": "gq_rk4_f64"},
+        "gpu_state": {"before_timed": gpu_before, "after_timed": gpu_after},
     }
     if not args.no_cpu:
-        log("CPU baseline ...")
-        out["cpu_baseline"] = cpu_baseline()
+        log("CPU baseline (FP64 C oracle sample; also the error reference) ...")
+        out["cpu_baseline"], idx, ref = cpu_baseline(args.grid)
+        # the timed FP64 map and the FP32 form against the oracle on the same columns
+        out["error_vs_fp64_oracle"] = map_error(res_t, idx, ref, stable)
+        out["error_vs_fp64_oracle"]["form"] = "fp64_tabulated (the timed map)"
+        out["fp32_form"]["error_vs_c_oracle_sample"] = map_error(res32, idx, ref, stable)
         log("reference CPU path ...")
         out["cpu_reference_path"] = reference_cpu_path()
     print(json.dumps(out), flush=True)
@@ -427,7 +500,7 @@ def bench_lzs(args):
     torch.cuda.synchronize()
     eps, amp = w["var_arrays"].values()
     m = models.lzs_two_level(eps, amp)
-    threads = min(len(os.sched_getaffinity(0)), 16)
+    threads = host_cpu()["threads"]
     t0 = time.perf_counter()
     ref = cbridge.rk4_affine(m, 0.1, 2000, threads=threads)
     cpu_s = time.perf_counter() - t0

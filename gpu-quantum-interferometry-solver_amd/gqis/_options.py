@@ -1,8 +1,9 @@
 """Grouped optional arguments of ``mesolve_2D`` / ``odesolve_2D``.
 
-Mirrors the reference's option groups (``gqis/_options.py:6-57``): a group is a dict that uses
-the flat keyword names; a name may be given flat or in its group, not both.  The groups here
-are a superset of the reference's (new names are marked), so every reference call still works.
+The group names, their members and the error messages are the reference's interface
+(``gqis/_options.py:6-57``): a group is a dict keyed by the flat keyword names, and a name may be
+given flat or in its group, not both.  The groups here are a superset of the reference's (the
+names added by this implementation are marked), so every reference call still works.
 """
 from collections.abc import Mapping
 import inspect
@@ -16,47 +17,66 @@ OPTION_GROUPS = {
     "codegen": ("RHSreuse", "kernel_template_file", "unroll", "pre_expand", "collect_rho",
                 "factor_terms", "cse_batch_size", "cse_simplify", "hoist_rho_independent",
                 "symbolic_workers", "max_symbolic_workers", "nvrtc_options",
-                # new in this implementation:
+                # added by this implementation:
                 "hiprtc_options", "time_table", "block_size", "liouville", "lanes_per_point"),
     "diagnostics": ("timings", "return_timing_info", "Actual_Kernel_Save", "beep_on_error",
                     "ignore_non_finite_output"),
 }
 
+# Every option belongs to exactly one group.
+_GROUP_OF = {name: group for group, names in OPTION_GROUPS.items() for name in names}
 
-def _unchanged(value, default):
-    """True when a flat argument still holds its signature default."""
+# Defaults of these types are compared by value: a flat ``fp64=False`` counts as not given.
+_VALUE_DEFAULTS = (bool, int, float, str, tuple)
+
+
+def _given(value, default):
+    """Whether a flat keyword was set to something other than its signature default."""
     if value is default:
-        return True
-    simple = (bool, int, float, str, tuple)
-    return isinstance(default, simple) and type(value) is type(default) and value == default
+        return False
+    if isinstance(default, _VALUE_DEFAULTS) and type(value) is type(default):
+        return value != default
+    return True
 
 
-def apply_option_groups(func, args, groups, extra=None):
-    """Fold ``groups`` (name -> dict or None) into the flat keyword dict ``args`` of ``func``.
-
-    ``extra`` is the ``**kwargs`` dict of ``func`` when it forwards further options.  Raises
-    ``TypeError`` for a group that is not a mapping and ``ValueError`` for an unknown name or an
-    option given both flat and grouped (``gqis/_options.py:30-57``).
-    """
-    params = inspect.signature(func).parameters
+def _grouped_options(groups):
+    """Yield ``(group, name, value)`` for every entry of every group dict, checking that each
+    group is a mapping and each name is one of that group's options."""
     for group, values in groups.items():
         if values is None:
             continue
         if not isinstance(values, Mapping):
             raise TypeError(f"{group} must be a dict of option names and values.")
-        allowed = OPTION_GROUPS[group]
         for name, value in values.items():
-            if name not in allowed:
+            if _GROUP_OF.get(name) != group:
+                members = OPTION_GROUPS[group]
                 raise ValueError(f"Unknown {group} option '{name}'. "
-                                 f"Valid options: {', '.join(allowed)}.")
-            if name in params:
-                if not _unchanged(args[name], params[name].default):
-                    raise ValueError(f"'{name}' is given both directly and in {group}; use one.")
-                args[name] = value
-            elif extra is not None:
-                if name in extra:
-                    raise ValueError(f"'{name}' is given both directly and in {group}; use one.")
-                extra[name] = value
-            else:
-                raise ValueError(f"{func.__name__} does not accept the {group} option '{name}'.")
-    return args, extra
+                                 f"Valid options: {', '.join(members)}.")
+            yield group, name, value
+
+
+def fold_option_groups(func, flat, groups, forwarded=None):
+    """Move the grouped options into the flat keyword dict ``flat`` of ``func``.
+
+    ``groups`` maps group name -> dict or None.  Names that are not parameters of ``func`` go
+    to ``forwarded`` (its ``**kwargs`` dict), when it has one.  Returns ``(flat, forwarded)``.
+    Raises ``TypeError`` for a group that is not a mapping and ``ValueError`` for an unknown
+    name or a name given both flat and grouped (``gqis/_options.py:30-57``).
+    """
+    defaults = {name: p.default for name, p in inspect.signature(func).parameters.items()}
+    for group, name, value in _grouped_options(groups):
+        if name in defaults:
+            target, clash = flat, _given(flat[name], defaults[name])
+        elif forwarded is not None:
+            target, clash = forwarded, name in forwarded
+        else:
+            raise ValueError(f"{func.__name__} does not accept the {group} option '{name}'.")
+        if clash:
+            raise ValueError(f"'{name}' is given both directly and in {group}; use one.")
+        target[name] = value
+    return flat, forwarded
+
+
+def split_groups(call):
+    """Remove the group arguments from a call's keyword dict and return them."""
+    return {group: call.pop(group) for group in OPTION_GROUPS}

@@ -1,21 +1,32 @@
 """``odesolve_2D`` / ``compile_ode_rhs``: general real ODE sweeps on the same kernels.
 
-Behaviour of the reference ODE front end (``gqis/gqis_ode_api.py:17-255``): states are renamed
-to ``rho[i]``, parameters to ``ode_param_i``, the time symbol to the kernel time; ``final`` maps
-to the full state, ``mean`` to the state mean; without an observable the result is the state.
+The public behaviour follows the reference ODE front end (``gqis/gqis_ode_api.py:17-255``):
+the same signatures, defaults, option names, error messages and result shapes.  Internally a
+user model becomes an ``OdeModel`` in kernel names -- states ``rho[i]``, the time symbol the
+kernel time, other parameters ``ode_param_i`` -- and runs through the Lindblad engine as an
+``ode_size``-state system whose "Hamiltonian" slot holds the RHS column.  ``final`` output
+maps to the full state and ``mean`` to the state mean when no observable is given.
 """
 import dataclasses
 
 import numpy as np
 import sympy as sp
 
-from ._options import OPTION_GROUPS, apply_option_groups
+from ._options import fold_option_groups, split_groups
 from .symbolic import TIME
 
+# ODE right-hand sides are not simplified on input, so CSE runs with simplification, and the
+# Lindblad-specific rewrites (expansion, collection by state) are off by default.
 _ODE_CODEGEN_DEFAULTS = {"pre_expand": False, "collect_rho": False, "cse_simplify": True}
 _ODE_CODEGEN_OPTIONS = ("pre_expand", "collect_rho", "factor_terms", "cse_batch_size",
                         "cse_simplify", "hoist_rho_independent", "RHSreuse")
-_DUMMY_SWEEP = sp.Symbol("ode_dummy", real=True)
+# A sweep axis of length one stands in for a model without sweep parameters.
+_NO_SWEEP = sp.Symbol("ode_dummy", real=True)
+# Keywords of mesolve_2D that have an ODE counterpart (or none) and are refused here.
+_LINDBLAD_ONLY = frozenset({"rho0", "rho0_values", "rho0_var_arrays", "compiled_rhs"})
+# Fixed when the RHS is compiled; a call with a CompiledRHS must leave them unset.
+_COMPILE_TIME_ARGS = ("state_symbols", "y0", "const_values", "keep_symbolic_consts",
+                      "time_symbol", "observable")
 
 
 def odesolve_2D(rhs, state_symbols=None, y0=None, tlist=None, *, var_arrays=None,
@@ -30,168 +41,229 @@ def odesolve_2D(rhs, state_symbols=None, y0=None, tlist=None, *, var_arrays=None
     traces add a sample axis before the state axis.  ``rhs`` may be a ``CompiledRHS`` from
     ``compile_ode_rhs``.  Other ``mesolve_2D`` options are forwarded through ``option_set``.
     """
-    args = dict(locals())
-    extra = args.pop("option_set")
-    groups = {name: args.pop(name) for name in OPTION_GROUPS}
-    args, extra = apply_option_groups(odesolve_2D, args, groups, extra)
-    return _solve_ode_system(**args, **extra)
+    call = dict(locals())
+    forwarded = call.pop("option_set")
+    call, forwarded = fold_option_groups(odesolve_2D, call, split_groups(call), forwarded)
+    return _run(call, forwarded)
 
 
 def compile_ode_rhs(rhs, state_symbols, y0=None, sweep_symbols=(), *, const_values=None,
                     runtime_consts=None, keep_symbolic_consts=None, time_symbol=None,
                     observable=None, **codegen_options):
     """Prepare and optimise an ODE RHS without the GPU (``gqis/gqis_ode_api.py:70-112``)."""
-    unknown = set(codegen_options) - set(_ODE_CODEGEN_OPTIONS)
-    if unknown:
-        raise TypeError(f"Unknown compile_ode_rhs option(s): {sorted(unknown)}")
+    stray = sorted(set(codegen_options).difference(_ODE_CODEGEN_OPTIONS))
+    if stray:
+        raise TypeError(f"Unknown compile_ode_rhs option(s): {stray}")
     sweep_symbols = tuple(sweep_symbols)
     if len(sweep_symbols) > 2:
         raise ValueError("At most two sweep symbols are supported.")
-    model = _prepare_ode_model(rhs, state_symbols, y0, dict.fromkeys(sweep_symbols), const_values,
-                               runtime_consts, keep_symbolic_consts, time_symbol, observable)
+    model = ode_model(rhs, state_symbols, y0, dict.fromkeys(sweep_symbols),
+                      const_values=const_values, runtime_consts=runtime_consts,
+                      keep_symbolic_consts=keep_symbolic_consts, time_symbol=time_symbol,
+                      observable=observable)
+    options = dict(_ODE_CODEGEN_DEFAULTS, **codegen_options)
     from .solver import _compile_rhs
-    options = {**_ODE_CODEGEN_DEFAULTS, **codegen_options}
-    reuse = options.pop("RHSreuse", True)
-    compiled, _ = _compile_rhs(
-        model["H"], {}, [], model["mean"], sweep_symbols=tuple(model["sweeps"]),
-        const_values=model["constants"], runtime_consts=model["runtime"],
-        keep_symbolic_consts=model["keep"], rho0=model["initial"], ode_size=model["size"],
-        reuse=reuse, **options)
-    return dataclasses.replace(compiled, ode={"symbol_map": model["replacements"],
+    compiled, _ = _compile_rhs(model.equations, {}, [], model.observable,
+                               sweep_symbols=tuple(model.sweeps), reuse=options.pop("RHSreuse", True),
+                               **model.symbol_kwargs(), **options)
+    # the name map belongs to this call: one cached internal model can serve several user
+    # spellings of the same equations
+    return dataclasses.replace(compiled, ode={"symbol_map": model.names,
                                               "sweep_symbols": sweep_symbols,
                                               "has_observable": observable is not None})
 
 
-def _prepare_ode_model(rhs, state_symbols, y0, var_arrays, const_values, runtime_consts,
-                       keep_symbolic_consts, time_symbol, observable):
-    """Validate an ODE model and rename it to kernel symbols
-    (``gqis/gqis_ode_api.py:115-181``)."""
+@dataclasses.dataclass
+class OdeModel:
+    """A validated ODE model in kernel names."""
+    equations: sp.Matrix          # RHS column, one row per state
+    observable: sp.Matrix         # 1 x 1 (zero without an observable)
+    initial: list                 # y0 expressions, or None
+    sweeps: dict                  # kernel symbol -> values (a dummy axis when none is swept)
+    constants: dict
+    runtime: dict
+    keep: object                  # keep_symbolic_consts in kernel names (or its string form)
+    names: dict                   # user symbol -> kernel symbol
+
+    @property
+    def size(self):
+        return self.equations.shape[0]
+
+    def symbol_kwargs(self):
+        """Keywords shared by the engine's compile and solve entry points."""
+        return {"const_values": self.constants, "runtime_consts": self.runtime,
+                "keep_symbolic_consts": self.keep, "rho0": self.initial, "ode_size": self.size}
+
+
+def _checked_states(state_symbols):
     states = tuple(state_symbols)
-    if (not states or any(not isinstance(s, sp.Symbol) for s in states)
-            or len(set(states)) != len(states)):
+    distinct = len(set(states)) == len(states)
+    if not states or not distinct or not all(isinstance(s, sp.Symbol) for s in states):
         raise ValueError("state_symbols must contain distinct SymPy Symbols.")
     if any(s.is_real is False for s in states):
         raise ValueError("State symbols must represent real variables.")
-    exprs = [sp.sympify(e) for e in rhs]
-    if len(exprs) != len(states):
-        raise ValueError("Provide one RHS expression per state symbol.")
-    init = None if y0 is None else [sp.sympify(e) for e in y0]
-    if init is not None and len(init) != len(states):
-        raise ValueError("y0 must contain one initial value per state symbol.")
-    obs = sp.sympify(observable) if observable is not None else sp.Integer(0)
-    all_syms = set().union(*(e.free_symbols for e in [*exprs, *(init or []), obs]))
-    candidates = [s for s in all_syms if s.name == "t" and s not in states]
+    return states
+
+
+def _choose_time_symbol(time_symbol, used, states):
+    """The given time symbol, else the one free symbol named ``t`` that is not a state (a fresh
+    real ``t`` when there is none)."""
     if time_symbol is None:
-        if len(candidates) > 1:
+        named_t = {s for s in used if s.name == "t"}.difference(states)
+        if len(named_t) > 1:
             raise ValueError("Specify time_symbol when multiple different symbols are named t.")
-        time_symbol = candidates[0] if candidates else sp.Symbol("t", real=True)
+        time_symbol = named_t.pop() if named_t else sp.Symbol("t", real=True)
     if not isinstance(time_symbol, sp.Symbol) or time_symbol in states:
         raise ValueError("time_symbol must be a Symbol distinct from state_symbols.")
-    for i, e in enumerate(exprs):
+    return time_symbol
+
+
+def _explicit_rhs(exprs, states, time_symbol):
+    """Right-hand sides of ``Eq(Derivative(state, t), rhs)`` entries; plain entries as given."""
+    out = []
+    for state, e in zip(states, exprs):
         if isinstance(e, sp.Equality):
-            if e.lhs != sp.Derivative(states[i], time_symbol, evaluate=False):
+            if e.lhs != sp.Derivative(state, time_symbol, evaluate=False):
                 raise ValueError("Each equation must explicitly define its state derivative.")
-            exprs[i] = e.rhs
-    if any(e.has(sp.Derivative) for e in exprs):
+            e = e.rhs
+        out.append(e)
+    if any(e.has(sp.Derivative) for e in out):
         raise ValueError("Supply explicit first-order RHS expressions, without derivatives.")
-    sweeps = {} if var_arrays is None else dict(var_arrays)
-    constants = {} if const_values is None else dict(const_values)
-    runtime = {} if runtime_consts is None else dict(runtime_consts)
-    params = set(sweeps) | set(constants) | set(runtime)
-    if (any(not isinstance(s, sp.Symbol) for s in params)
-            or params & (set(states) | {time_symbol})):
+    return out
+
+
+def _kernel_names(states, time_symbol, parameters):
+    names = {s: sp.Symbol(f"rho[{i}]", real=True) for i, s in enumerate(states)}
+    names[time_symbol] = TIME
+    ordered = sorted(parameters, key=lambda s: (s.name, sp.srepr(s)))
+    names.update((s, sp.Symbol(f"ode_param_{i}", real=True)) for i, s in enumerate(ordered))
+    return names
+
+
+def ode_model(rhs, state_symbols, y0=None, var_arrays=None, *, const_values=None,
+              runtime_consts=None, keep_symbolic_consts=None, time_symbol=None, observable=None):
+    """Validate a user ODE model and rename it to kernel symbols (the checks and messages of
+    ``gqis/gqis_ode_api.py:115-181``)."""
+    states = _checked_states(state_symbols)
+    exprs = list(map(sp.sympify, rhs))
+    if len(exprs) != len(states):
+        raise ValueError("Provide one RHS expression per state symbol.")
+    init = None if y0 is None else list(map(sp.sympify, y0))
+    if init is not None and len(init) != len(states):
+        raise ValueError("y0 must contain one initial value per state symbol.")
+    obs = sp.Integer(0) if observable is None else sp.sympify(observable)
+    used = set()
+    for e in (*exprs, *(init or ()), obs):
+        used |= e.free_symbols
+    time_symbol = _choose_time_symbol(time_symbol, used, states)
+    exprs = _explicit_rhs(exprs, states, time_symbol)
+
+    sweeps, constants, runtime = (dict(m or {}) for m in (var_arrays, const_values,
+                                                          runtime_consts))
+    parameters = set(sweeps) | set(constants) | set(runtime)
+    reserved = set(states) | {time_symbol}
+    if not all(isinstance(s, sp.Symbol) for s in parameters) or parameters & reserved:
         raise ValueError("Parameter keys must be Symbols distinct from the state and time symbols.")
     if set(sweeps) & (set(constants) | set(runtime)):
         raise ValueError("A swept parameter cannot also be a constant.")
-    unknown = all_syms - set(states) - {time_symbol} - params
-    if unknown:
-        raise ValueError(f"Unassigned ODE parameters: {sorted(map(str, unknown))}")
-    if init is not None and any(e.free_symbols & (set(states) | {time_symbol}) for e in init):
+    unassigned = used - reserved - parameters
+    if unassigned:
+        raise ValueError(f"Unassigned ODE parameters: {sorted(map(str, unassigned))}")
+    if init is not None and any(e.free_symbols & reserved for e in init):
         raise ValueError("Initial values may depend on parameters, not on state or time.")
-    repl = {s: sp.Symbol(f"rho[{i}]", real=True) for i, s in enumerate(states)}
-    repl[time_symbol] = TIME
-    repl.update({s: sp.Symbol(f"ode_param_{i}", real=True)
-                 for i, s in enumerate(sorted(params, key=sp.srepr))})
-    exprs = [e.xreplace(repl) for e in exprs]
-    init = None if init is None else [e.xreplace(repl) for e in init]
-    if any(sp.simplify(sp.im(e)) != 0 for e in [*exprs, *(init or [])]):
+
+    names = _kernel_names(states, time_symbol, parameters)
+    exprs = [e.xreplace(names) for e in exprs]
+    init = None if init is None else [e.xreplace(names) for e in init]
+    if not all(sp.simplify(sp.im(e)) == 0 for e in (*exprs, *(init or ()))):
         raise ValueError("ODE states and RHS must be real; split complex states into "
                          "real/imaginary parts.")
-    obs = obs.xreplace(repl)
-
-    def remap(mapping):
-        return {repl[s]: v for s, v in mapping.items()}
-    sweeps = remap(sweeps) or {_DUMMY_SWEEP: np.zeros(1)}
-    if keep_symbolic_consts is not None and not isinstance(keep_symbolic_consts, str):
-        keep_symbolic_consts = [repl[s] for s in keep_symbolic_consts]
-    return {"H": sp.Matrix(exprs), "mean": sp.Matrix([obs]), "initial": init, "sweeps": sweeps,
-            "constants": remap(constants), "runtime": remap(runtime), "keep": keep_symbolic_consts,
-            "size": len(states), "replacements": repl}
+    keep = keep_symbolic_consts
+    if keep is not None and not isinstance(keep, str):
+        keep = [names[s] for s in keep]
+    return OdeModel(equations=sp.Matrix(exprs), observable=sp.Matrix([obs.xreplace(names)]),
+                    initial=init,
+                    sweeps={names[s]: v for s, v in sweeps.items()} or {_NO_SWEEP: np.zeros(1)},
+                    constants={names[s]: v for s, v in constants.items()},
+                    runtime={names[s]: v for s, v in runtime.items()}, keep=keep, names=names)
 
 
-def _solve_ode_system(rhs, state_symbols, y0, tlist, *, var_arrays, const_values, time_symbol,
-                      observable, output_mode, y0_values, runtime_consts, keep_symbolic_consts,
-                      solver, fp64, return_time_trace, time_trace_every, t_in, return_device,
-                      calibrate_timesteps, **option_set):
-    from .solver import CompiledRHS, _solve_symbolic_system
-    if tlist is None:
+def _check_call(call, forwarded):
+    if call["tlist"] is None:
         raise TypeError("odesolve_2D needs tlist.")
-    if output_mode not in {"mean", "final"}:
+    if call["output_mode"] not in ("mean", "final"):
         raise ValueError("output_mode must be 'mean' or 'final'.")
-    if y0 is not None and y0_values is not None:
+    y0_values = call["y0_values"]
+    if call["y0"] is not None and y0_values is not None:
         raise ValueError("Use either y0 or y0_values, not both.")
-    if y0_values is not None and not hasattr(y0_values, "data_ptr") and np.iscomplexobj(y0_values):
+    # device tensors are taken as they are (their dtype is checked where they are uploaded)
+    if y0_values is not None and not hasattr(y0_values, "data_ptr") \
+            and np.iscomplexobj(y0_values):
         raise ValueError("y0_values must be real; split complex states into real/imaginary parts.")
-    if any(k.startswith("_") or k in {"rho0", "rho0_values", "rho0_var_arrays", "compiled_rhs"}
-           for k in option_set):
+    if any(k.startswith("_") or k in _LINDBLAD_ONLY for k in forwarded):
         raise ValueError("Use y0/y0_values and var_arrays for ODE initial conditions.")
-    if isinstance(rhs, CompiledRHS):
-        compiled = rhs
-        if compiled.kind != "ode" or compiled.ode is None:
-            raise ValueError("Pass a CompiledRHS from compile_ode_rhs (compile_lindblad_rhs "
-                             "results go to mesolve_2D).")
-        if any(v is not None for v in (state_symbols, y0, const_values, keep_symbolic_consts,
-                                       time_symbol, observable)):
-            raise ValueError("state_symbols, y0, const_values, keep_symbolic_consts, time_symbol "
-                             "and observable are fixed when the RHS is compiled.")
-        if y0_values is None and compiled.rho0_exprs is None:
-            raise ValueError("Provide y0_values: no y0 was given when the RHS was compiled.")
-        if tuple(var_arrays or {}) != compiled.ode["sweep_symbols"]:
-            raise ValueError(f"var_arrays must sweep {compiled.ode['sweep_symbols']}, in the "
-                             "order given when the RHS was compiled.")
-        smap = compiled.ode["symbol_map"]
 
-        def remap(mapping, name):
-            bad = [s for s in mapping if s not in smap]
-            if bad:
-                raise ValueError(f"{name} contains symbols unknown to the compiled RHS: {bad}")
-            return {smap[s]: v for s, v in mapping.items()}
-        sweeps = remap(var_arrays or {}, "var_arrays") or {_DUMMY_SWEEP: np.zeros(1)}
-        runtime = remap(runtime_consts or {}, "runtime_consts")
-        has_obs = compiled.ode["has_observable"]
-        mode = output_mode if has_obs else {"final": "final_rho", "mean": "mean_state"}[output_mode]
-        return _solve_symbolic_system(
-            None, None, None, None, tlist, var_arrays=sweeps, runtime_consts=runtime,
-            rho0_values=y0_values, output_mode=mode, solver=solver, fp64=fp64,
-            return_time_trace=return_time_trace, time_trace_every=time_trace_every, t_in=t_in,
-            return_device=return_device, calibrate_timesteps=calibrate_timesteps,
-            compiled_rhs=compiled, _ode_size=compiled.state_size, _state_trace=not has_obs,
-            **option_set)
-    if state_symbols is None:
+
+def _compiled_model_args(call):
+    """Engine keywords for a ``CompiledRHS`` from ``compile_ode_rhs``."""
+    compiled = call["rhs"]
+    if compiled.kind != "ode" or compiled.ode is None:
+        raise ValueError("Pass a CompiledRHS from compile_ode_rhs (compile_lindblad_rhs "
+                         "results go to mesolve_2D).")
+    if any(call[name] is not None for name in _COMPILE_TIME_ARGS):
+        raise ValueError("state_symbols, y0, const_values, keep_symbolic_consts, time_symbol "
+                         "and observable are fixed when the RHS is compiled.")
+    if call["y0_values"] is None and compiled.rho0_exprs is None:
+        raise ValueError("Provide y0_values: no y0 was given when the RHS was compiled.")
+    info = compiled.ode
+    if tuple(call["var_arrays"] or {}) != info["sweep_symbols"]:
+        raise ValueError(f"var_arrays must sweep {info['sweep_symbols']}, in the order given "
+                         "when the RHS was compiled.")
+
+    def to_kernel_names(mapping, what):
+        foreign = [s for s in mapping if s not in info["symbol_map"]]
+        if foreign:
+            raise ValueError(f"{what} contains symbols unknown to the compiled RHS: {foreign}")
+        return {info["symbol_map"][s]: v for s, v in mapping.items()}
+    sweeps = to_kernel_names(call["var_arrays"] or {}, "var_arrays")
+    engine = {"compiled_rhs": compiled, "var_arrays": sweeps or {_NO_SWEEP: np.zeros(1)},
+              "runtime_consts": to_kernel_names(call["runtime_consts"] or {}, "runtime_consts"),
+              "_ode_size": compiled.state_size}
+    return (None, None, None, None), engine, info["has_observable"], {}
+
+
+def _symbolic_model_args(call):
+    """Engine keywords for a model given as expressions."""
+    if call["state_symbols"] is None:
         raise TypeError("odesolve_2D needs state_symbols (or a CompiledRHS as rhs).")
-    if y0 is None and y0_values is None:
+    if call["y0"] is None and call["y0_values"] is None:
         raise ValueError("Provide y0 or y0_values.")
-    model = _prepare_ode_model(rhs, state_symbols, y0, var_arrays, const_values, runtime_consts,
-                               keep_symbolic_consts, time_symbol, observable)
-    for k, v in _ODE_CODEGEN_DEFAULTS.items():
-        option_set.setdefault(k, v)
-    mode = output_mode if observable is not None else {"final": "final_rho",
-                                                       "mean": "mean_state"}[output_mode]
-    return _solve_symbolic_system(
-        model["H"], {}, [], model["mean"], tlist, var_arrays=model["sweeps"],
-        const_values=model["constants"], runtime_consts=model["runtime"],
-        keep_symbolic_consts=model["keep"], rho0=model["initial"], rho0_values=y0_values,
-        output_mode=mode, solver=solver, fp64=fp64, return_time_trace=return_time_trace,
-        time_trace_every=time_trace_every, t_in=t_in, return_device=return_device,
-        calibrate_timesteps=calibrate_timesteps, _ode_size=model["size"],
-        _state_trace=observable is None, **option_set)
+    model = ode_model(call["rhs"], call["state_symbols"], call["y0"], call["var_arrays"],
+                      const_values=call["const_values"], runtime_consts=call["runtime_consts"],
+                      keep_symbolic_consts=call["keep_symbolic_consts"],
+                      time_symbol=call["time_symbol"], observable=call["observable"])
+    kw = model.symbol_kwargs()
+    engine = {"var_arrays": model.sweeps, "const_values": kw["const_values"],
+              "runtime_consts": kw["runtime_consts"], "keep_symbolic_consts": kw["keep_symbolic_consts"],
+              "rho0": kw["rho0"], "_ode_size": model.size}
+    return ((model.equations, {}, [], model.observable), engine, call["observable"] is not None,
+            _ODE_CODEGEN_DEFAULTS)
+
+
+def _run(call, forwarded):
+    """Validate an ``odesolve_2D`` call and run it on the shared sweep engine."""
+    from .solver import CompiledRHS, _solve_symbolic_system
+    _check_call(call, forwarded)
+    build = _compiled_model_args if isinstance(call["rhs"], CompiledRHS) else _symbolic_model_args
+    model_args, engine, has_observable, codegen_defaults = build(call)
+    # without an observable the result is the state: final -> final_rho, mean -> mean_state
+    mode = call["output_mode"]
+    if not has_observable:
+        mode = "final_rho" if mode == "final" else "mean_state"
+    passed = ("solver", "fp64", "return_time_trace", "time_trace_every", "t_in", "return_device",
+              "calibrate_timesteps")
+    engine.update({name: call[name] for name in passed})
+    engine.update(output_mode=mode, rho0_values=call["y0_values"], _state_trace=not has_observable)
+    options = {**codegen_defaults, **forwarded}
+    return _solve_symbolic_system(*model_args, call["tlist"], **engine, **options)
+

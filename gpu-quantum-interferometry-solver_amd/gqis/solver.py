@@ -6,6 +6,7 @@ kernel caches, the finite check of the outputs and the timing dictionary.  The d
 HIP: the kernel source is emitted from the symbolic plan, compiled with hiprtc and launched
 through ``libgqis_hip.so``; torch provides the device arrays and the stream.
 """
+from collections import ChainMap
 from dataclasses import dataclass, field
 import os
 import time
@@ -13,7 +14,7 @@ import time
 import numpy as np
 import sympy as sp
 
-from ._options import OPTION_GROUPS, apply_option_groups
+from ._options import fold_option_groups, split_groups
 from . import codegen, symbolic
 from .solvers import get_solver_spec, normalize_solver_name  # noqa: F401 (re-export)
 from .symbolic import (PARX, PARY, TIME, build_independent_rho,  # noqa: F401 (re-export)
@@ -202,17 +203,17 @@ def compile_lindblad_rhs(H, Drive, Col_Ops, mean_operator, sweep_symbols, *, con
 
 
 def _runtime_const_values(compiled, runtime_consts, const_values, use_defaults):
-    if not compiled.runtime_const_syms:
-        return []
-    values = dict(compiled.const_values) if use_defaults else {}
-    values.update(const_values or {})
-    values.update(runtime_consts or {})
-    out = []
-    for s in compiled.runtime_const_syms:
-        if s not in values:
-            raise ValueError(f"Missing value for runtime constant symbol: {s}")
-        out.append(float(values[s]))
-    return out
+    """Kernel-argument values of the compiled model's runtime constants, in argument order.
+    A call's ``runtime_consts`` take precedence over its ``const_values``, and both over the
+    values given when the RHS was compiled (consulted only with ``use_defaults``)."""
+    layers = [runtime_consts or {}, const_values or {}]
+    if use_defaults:
+        layers.append(compiled.const_values)
+    lookup = ChainMap(*layers)
+    absent = [s for s in compiled.runtime_const_syms if s not in lookup]
+    if absent:
+        raise ValueError(f"Missing value for runtime constant symbol: {absent[0]}")
+    return [float(lookup[s]) for s in compiled.runtime_const_syms]
 
 
 def mesolve_2D(H, Drive=None, Col_Ops=None, mean_operator=None, tlist=None,
@@ -253,16 +254,21 @@ def mesolve_2D(H, Drive=None, Col_Ops=None, mean_operator=None, tlist=None,
     real for ``"final_rho"``; with ``return_time_trace`` also the ``(num_X, num_Y, num_trace)``
     trace and its times; with ``return_timing_info`` the timing dictionary last.
     """
-    args = dict(locals())
-    groups = {name: args.pop(name) for name in OPTION_GROUPS}
-    args, _ = apply_option_groups(mesolve_2D, args, groups)
-    if args["output_mode"] not in {"mean", "final", "final_rho"}:
+    return _solve_symbolic_system(**_lindblad_call(dict(locals())))
+
+
+def _lindblad_call(call):
+    """Engine keywords of a ``mesolve_2D`` call: option groups folded into the flat names, the
+    output mode checked, a ``CompiledRHS`` in place of ``H`` moved to ``compiled_rhs``."""
+    call, _ = fold_option_groups(mesolve_2D, call, split_groups(call))
+    if call["output_mode"] not in {"mean", "final", "final_rho"}:
         raise ValueError("output_mode must be 'mean', 'final', or 'final_rho'.")
-    if isinstance(H, CompiledRHS):
-        if H.kind != "lindblad":
+    model = call["H"]
+    if isinstance(model, CompiledRHS):
+        if model.kind != "lindblad":
             raise ValueError("This CompiledRHS comes from compile_ode_rhs; pass it to odesolve_2D.")
-        args.update(H=None, compiled_rhs=H)
-    return _solve_symbolic_system(**args)
+        call["H"], call["compiled_rhs"] = None, model
+    return call
 
 
 def set_kernel_cache_dir(path, device=None):
@@ -282,19 +288,11 @@ def prepare_sweep(*args, **kwargs):
     import inspect
     bound = inspect.signature(mesolve_2D).bind(*args, **kwargs)
     bound.apply_defaults()
-    call = dict(bound.arguments)
-    groups = {name: call.pop(name) for name in OPTION_GROUPS}
-    call, _ = apply_option_groups(mesolve_2D, call, groups)
-    if call["output_mode"] not in {"mean", "final", "final_rho"}:
-        raise ValueError("output_mode must be 'mean', 'final', or 'final_rho'.")
+    call = _lindblad_call(dict(bound.arguments))
     for name in ("return_timing_info", "timings"):
         if call[name]:
             raise ValueError(f"{name} is not available for a prepared sweep (use run(timed=True) "
                              "and last_gpu_s).")
-    if isinstance(call["H"], CompiledRHS):
-        if call["H"].kind != "lindblad":
-            raise ValueError("This CompiledRHS comes from compile_ode_rhs; pass it to odesolve_2D.")
-        call.update(H=None, compiled_rhs=call["H"])
     return _solve_symbolic_system(**call, _prepare=True)
 
 
@@ -827,17 +825,19 @@ def _emit_source(compiled, spec, mode, uses_rho0_values, trace, state_trace, fp6
 
 
 def _print_timings(api_name, info, mode):
-    ode = {"miss": "new ODE", "hit": "reused ODE", "provided": "compiled ODE"}[info["cached_rhs"]]
-    kernel = "new kernel" if info["cached_kernel"] == "miss" else "reused kernel"
+    """Report a call's timing dictionary on stdout: one summary line, or with ``mode="full"``
+    one line per stage as well."""
+    rhs_origin = {"miss": "derived", "hit": "cached", "provided": "precompiled"}[info["cached_rhs"]]
+    kernel_origin = "built" if info["cached_kernel"] == "miss" else "cached"
 
-    def show(sec):
-        return f"{sec * 1e3:.1f} ms" if sec < 1.0 else f"{sec:.2f} s"
-    if mode != "full":
-        print(f"{api_name}: preparation {show(info['rhs_stage_s'])} ({ode}, {kernel}), "
-              f"GPU {show(info['gpu_kernel_s'])}, total {show(info['total_s'])}")
-        return
-    print(f"{api_name} timings ({info['solver']}):")
-    for name, key in (("symbolic ODE", "ode_prepare_s"), ("optimization", "ode_optimize_s"),
-                      ("HIP code", "code_emit_s"), ("kernel build", "kernel_build_s"),
-                      ("GPU calculation", "gpu_kernel_s"), ("total", "total_s")):
-        print(f"  {name:<18}{show(info[key]):>10}")
+    def fmt(seconds):
+        return f"{seconds:.2f} s" if seconds >= 1.0 else f"{1e3 * seconds:.1f} ms"
+    lines = [f"{api_name} [{info['solver']}]: host {fmt(info['rhs_stage_s'])} (RHS {rhs_origin}, "
+             f"kernel {kernel_origin}) | GPU {fmt(info['gpu_kernel_s'])} | "
+             f"total {fmt(info['total_s'])}"]
+    if mode == "full":
+        stages = {"symbolic model": "ode_prepare_s", "CSE and hoisting": "ode_optimize_s",
+                  "HIP source": "code_emit_s", "hiprtc build": "kernel_build_s",
+                  "GPU kernel": "gpu_kernel_s"}
+        lines += [f"    {label:.<20}{fmt(info[key]):>11}" for label, key in stages.items()]
+    print("\n".join(lines))

@@ -4,6 +4,7 @@ Same public names and aliases as the reference (``gqis/cuda_solvers.py:993-1080`
 new adaptive ``dop853a``.  Each spec names the hand-written HIP template that implements it.
 """
 from dataclasses import dataclass, field
+from fractions import Fraction
 import math
 import struct
 
@@ -91,10 +92,35 @@ def _float32(x):
     return struct.unpack("f", struct.pack("f", float(x)))[0]
 
 
-def anas5_coefficient(frequency, step_size):
-    """Fitted Anas5(w) coefficient from v = w*dt (``gqis/cuda_solvers.py:971-990``), evaluated
-    in high precision with mpmath."""
+def _exact(x):
+    """The rational with a denominator below 10^5 that the double ``x`` rounds (the tableau
+    entries are such rationals), as an mpmath number."""
     import mpmath as mp
+    q = Fraction(x).limit_denominator(100000)
+    return mp.mpf(q.numerator) / q.denominator
+
+
+def _stability_value(tab, z, p):
+    """R(z) = 1 + z b.k of an explicit tableau whose rows ``a1`` add p times their entries
+    (mpmath, exact rational coefficients)."""
+    import mpmath as mp
+    rows = [[_exact(a) for a in r] for r in tab["a"]]
+    for s, extra in tab["a1"].items():
+        rows[s] = [a + p * _exact(e) for a, e in zip(rows[s], extra)]
+    k = []
+    for row in rows:
+        k.append(1 + z * mp.fsum(a * kj for a, kj in zip(row, k)))
+    return 1 + z * mp.fsum(_exact(b) * kj for b, kj in zip(tab["b"], k))
+
+
+def anas5_coefficient(frequency, step_size):
+    """Fitted coefficient p of the Anas5(w) tableau for v = w*dt (the reference's
+    ``gqis/cuda_solvers.py:971-990``): the value that makes the method's phase lag vanish on
+    y' = i w y, i.e. arg R(iv) = v.  R is affine in p (p enters only the last stage), so
+    Im(R(iv) e^(-iv)) = 0 is one linear equation in p, solved here in 60-digit arithmetic.
+    At v = 0 the unfitted value -800/1071 is returned."""
+    import mpmath as mp
+    from .tableaux import ANAS5
     if step_size is None:
         return ()
     if not math.isfinite(step_size) or step_size <= 0.0:
@@ -106,12 +132,10 @@ def anas5_coefficient(frequency, step_size):
         v = mp.mpf(str(frequency)) * mp.mpf(str(step_size))
         if v == 0:
             return (float(mp.mpf(-800) / 1071),)
-        tn = mp.tan(v)
-        a43 = mp.mpf(2)
-        num = (-a43 * v ** 5 + 6 * tn * v ** 4 + 24 * v ** 3 - 72 * tn * v ** 2 - 144 * v
-               + 144 * tn)
-        den = v ** 5 * (a43 * tn * v + 12 - 10 * a43)
-        return (float((-mp.mpf(8000) / 1071) * num / den),)
+        z, turn = mp.mpc(0, v), mp.expj(-v)
+        r0 = _stability_value(ANAS5, z, 0)
+        r1 = _stability_value(ANAS5, z, 1) - r0
+        return (float(-mp.im(r0 * turn) / mp.im(r1 * turn)),)
 
 
 def get_solver_spec(solver, *, frequency=None, step_size=None, adaptive_options=None,

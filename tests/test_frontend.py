@@ -99,14 +99,13 @@ def test_plan_reproduces_the_equations():
 
 def test_nonlinear_ode_plan_hoists_time_terms():
     w = workloads.duffing(n_gamma=2, n_x0=2)
-    from gqis.gqis_ode_api import _prepare_ode_model
+    from gqis.gqis_ode_api import ode_model
     from gqis.solver import _compile_rhs
-    model = _prepare_ode_model(w["rhs"], w["state_symbols"], w["y0"], w["var_arrays"], None,
-                               None, None, None, None)
-    c, _ = _compile_rhs(model["H"], {}, [], model["mean"], sweep_symbols=tuple(model["sweeps"]),
-                        const_values=model["constants"], runtime_consts=model["runtime"],
-                        keep_symbolic_consts=model["keep"], rho0=model["initial"],
-                        ode_size=model["size"])
+    model = ode_model(w["rhs"], w["state_symbols"], w["y0"], w["var_arrays"])
+    c, _ = _compile_rhs(model.equations, {}, [], model.observable, sweep_symbols=tuple(model.sweeps),
+                        const_values=model.constants, runtime_consts=model.runtime,
+                        keep_symbolic_consts=model.keep, rho0=model.initial,
+                        ode_size=model.size)
     assert not c.plan.affine
     assert [str(e) for _, e in c.plan.time_atoms] == ["cos(1.2*t)"]
     got, _ = _eval_plan(c.plan, 0.4, 1.5, 3.1, [0.7, -0.2])
@@ -117,15 +116,12 @@ def test_nonlinear_ode_plan_hoists_time_terms():
 
 def test_generated_source_prints_literals_in_kernel_precision():
     x, y, t, a = sp.symbols("x y t a", real=True)
-    from gqis.gqis_ode_api import _prepare_ode_model
+    from gqis.gqis_ode_api import ode_model
     from gqis.solver import _compile_rhs, _emit_source
-    model = _prepare_ode_model([y, -sp.sqrt(2) * x - sp.pi * a * y + sp.E * sp.cos(2 * sp.pi * t),
-                                ], [x, y][:1] + [y], [1, 0], {a: np.ones(1)}, None, None, None,
-                               None, None) if False else None
-    model = _prepare_ode_model([y, -sp.sqrt(2) * x - sp.pi * a * y + sp.E * sp.cos(2 * sp.pi * t)],
-                               [x, y], [1, 0], {a: np.ones(1)}, None, None, None, None, None)
-    c, _ = _compile_rhs(model["H"], {}, [], model["mean"], sweep_symbols=tuple(model["sweeps"]),
-                        rho0=model["initial"], ode_size=2)
+    model = ode_model([y, -sp.sqrt(2) * x - sp.pi * a * y + sp.E * sp.cos(2 * sp.pi * t)],
+                      [x, y], [1, 0], {a: np.ones(1)})
+    c, _ = _compile_rhs(model.equations, {}, [], model.observable,
+                        sweep_symbols=tuple(model.sweeps), rho0=model.initial, ode_size=2)
     src = _emit_source(c, get_solver_spec("rk4"), "final_rho", False, False, False, False, True,
                        256, False)
     assert not re.search(r"\bM_\w+", src)            # no C macros such as M_SQRT2
@@ -180,11 +176,11 @@ def test_hiprtc_compiles_generated_kernels_for_gfx950():
 
 def test_bare_time_in_an_equation_becomes_a_time_atom():
     x, t = sp.symbols("x t", real=True)
-    from gqis.gqis_ode_api import _prepare_ode_model
+    from gqis.gqis_ode_api import ode_model
     from gqis.solver import _compile_rhs
-    model = _prepare_ode_model([t], [x], [0], None, None, None, None, None, None)
-    c, _ = _compile_rhs(model["H"], {}, [], model["mean"], sweep_symbols=tuple(model["sweeps"]),
-                        rho0=model["initial"], ode_size=1)
+    model = ode_model([t], [x], [0], None)
+    c, _ = _compile_rhs(model.equations, {}, [], model.observable, sweep_symbols=tuple(model.sweeps),
+                        rho0=model.initial, ode_size=1)
     assert [str(e) for _, e in c.plan.time_atoms] == ["t"]
     got, _ = _eval_plan(c.plan, 0.0, 0.0, 2.25, [0.3])
     assert got[0] == 2.25

@@ -12,7 +12,7 @@ import pytest
 import sympy as sp
 
 from gqis import codegen, runtime, workloads
-from gqis.gqis_ode_api import _prepare_ode_model
+from gqis.gqis_ode_api import ode_model
 from gqis.solver import _compile_rhs, _emit_source
 from gqis.solvers import get_solver_spec
 
@@ -88,10 +88,9 @@ def _models():
     four, _ = _compile_rhs(w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"],
                            sweep_symbols=tuple(w["var_arrays"]))
     w = workloads.duffing(n_gamma=2, n_x0=2)
-    m = _prepare_ode_model(w["rhs"], w["state_symbols"], w["y0"], w["var_arrays"], None, None,
-                           None, None, None)
-    duff, _ = _compile_rhs(m["H"], {}, [], m["mean"], sweep_symbols=tuple(m["sweeps"]),
-                           rho0=m["initial"], ode_size=m["size"], cse_simplify=True)
+    m = ode_model(w["rhs"], w["state_symbols"], w["y0"], w["var_arrays"])
+    duff, _ = _compile_rhs(m.equations, {}, [], m.observable, sweep_symbols=tuple(m.sweeps),
+                           rho0=m.initial, ode_size=m.size, cse_simplify=True)
     return {"chain8": (chain8, (2, 4, 8)), "four_level": (four, (2, 4)), "duffing": (duff, (2,))}
 
 

@@ -248,3 +248,37 @@ def test_python_cpu_path_matches_c_oracle():
     out, compute_s, workers = python_cpu.sweep("four_level", eps, amp, 0.05, 200, workers=2)
     assert workers == 2 and compute_s >= 0.0
     np.testing.assert_allclose(out, ref.reshape(2, 3), rtol=0, atol=1e-13)
+
+
+def _anas5_rotation_phase(omega, dt, steps, monkeypatch=None, unfitted=False):
+    """Phase error after ``steps`` Anas5 steps on the oscillator x' = -w y, y' = w x (that is,
+    z' = i w z for z = x + i y) started at z = 1."""
+    if unfitted:
+        monkeypatch.setattr(integrate, "anas5_parameter", lambda w, h: -800.0 / 1071.0)
+    rot = np.array([[0.0, -omega], [omega, 0.0]])
+    y, _ = integrate.fixed_step("anas5", lambda t, y: rot @ y, np.array([[1.0], [0.0]]), dt,
+                                steps, mode="final_rho", frequency=omega)
+    z = complex(y[0, 0], y[0, 1])
+    return np.angle(z * np.exp(-1j * omega * dt * steps)), abs(z)
+
+
+@pytest.mark.parametrize("omega,dt", [(1.0, 0.3), (2.0, 0.3), (1.0, 1.0), (5.0, 0.1)])
+def test_anas5_fitted_coefficient_has_zero_phase_lag(omega, dt, monkeypatch):
+    """The fitted Anas5(w) coefficient (reference gqis/cuda_solvers.py:971-990) is the one that
+    removes the method's phase lag at v = w*dt: 50 steps on z' = i w z end at phase 50 v to
+    rounding, while the unfitted -800/1071 lags by a v^7-sized amount per step."""
+    lag, amp = _anas5_rotation_phase(omega, dt, 50)
+    assert abs(lag) < 1e-12 and abs(amp - 1) < 0.05   # dissipation error is not fitted
+    lag0, _ = _anas5_rotation_phase(omega, dt, 50, monkeypatch, unfitted=True)
+    v = omega * dt
+    assert abs(lag0) > 1e-5 * v ** 7 * 50 and abs(lag0) > 1e3 * abs(lag)
+
+
+def test_product_anas5_coefficient_equals_the_closed_form():
+    """The product derives the coefficient from the tableau's stability function; the oracle
+    restates the published closed form.  They agree over v = 1e-5 .. 1.4."""
+    from gqis.solvers import anas5_coefficient
+    for omega, dt in [(1.0, 1e-5), (1.0, 0.05), (0.1, 0.01), (1.0, 0.3), (2.0, 0.5), (3.0, 0.4),
+                      (7.0, 0.2)]:
+        p = anas5_coefficient(omega, dt)[0]
+        assert p == pytest.approx(integrate.anas5_parameter(omega, dt), rel=1e-14, abs=0)
