@@ -5,7 +5,7 @@ the oracle's FP64 answers on exact subsets of the benchmark grids, written once 
 the GPU tests compare the HIP kernels with them without recomputing the oracle, and a CPU test
 checks that the oracle still reproduces them.  ``configs.json`` records the workload decisions
 the fixtures and benchmarks use.  Run from the repository root:
-``python tests/golden/make_golden.py``.
+``python tests/golden/make_golden.py [fixture names]``.
 """
 import json
 import os
@@ -35,13 +35,14 @@ def c2_four_level_subset():
     return dict(eps=eps, amp=amp, t_end=100.0, n_t=2001, stride=128, result=res.reshape(16, 16))
 
 
-def c3_random_n_level(n):
-    eps = np.linspace(-5.0, 5.0, 256)[::32]
-    amp = np.linspace(0.0, 5.0, 256)[::32]
+def c3_random_n_level(n, stride=32):
+    eps = np.linspace(-5.0, 5.0, 256)[::stride]
+    amp = np.linspace(0.0, 5.0, 256)[::stride]
     m = models.random_n_level(n, 2024, eps, amp)
     res, _ = integrate.rk4(m.f, m.ground_state(), 0.01, 2000, obs=m.obs)
-    return dict(eps=eps, amp=amp, t_end=20.0, n_t=2001, seed=2024, stride=32,
-                result=res.reshape(8, 8))
+    side = 256 // stride
+    return dict(eps=eps, amp=amp, t_end=20.0, n_t=2001, seed=2024, stride=stride,
+                result=res.reshape(side, side))
 
 
 def c4_duffing_short():
@@ -60,16 +61,21 @@ FIXTURES = {
     "c2_four_level_16x16_subset_rk4_mean_fp64.npz": c2_four_level_subset,
     "c3_random_4_level_8x8_subset_rk4_mean_fp64.npz": lambda: c3_random_n_level(4),
     "c3_random_8_level_8x8_subset_rk4_mean_fp64.npz": lambda: c3_random_n_level(8),
+    "c3_random_16_level_8x8_subset_rk4_mean_fp64.npz": lambda: c3_random_n_level(16),
+    "c3_random_32_level_4x4_subset_rk4_mean_fp64.npz": lambda: c3_random_n_level(32, stride=64),
     "c4_duffing_8x8_subset_t20_tsit5a_fp64.npz": c4_duffing_short,
 }
 
 
-def main():
+def main(names=()):
+    """Write every fixture, or only those named on the command line."""
     for name, make in FIXTURES.items():
+        if names and name not in names:
+            continue
         data = make()
         np.savez_compressed(os.path.join(HERE, name), **data)
         print(name, {k: np.shape(v) for k, v in data.items()})
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -28,12 +28,22 @@ def test_oracle_reproduces_lzs_fixture():
     _close(res.reshape(64, 64), g["result"])
 
 
-@pytest.mark.parametrize("n", [4, 8])
+RANDOM_FIXTURES = {4: "c3_random_4_level_8x8_subset_rk4_mean_fp64.npz",
+                   8: "c3_random_8_level_8x8_subset_rk4_mean_fp64.npz",
+                   16: "c3_random_16_level_8x8_subset_rk4_mean_fp64.npz",
+                   32: "c3_random_32_level_4x4_subset_rk4_mean_fp64.npz"}
+
+
+@pytest.mark.parametrize("n", [4, 8, 16, 32])
 def test_oracle_reproduces_random_n_level_fixture(n):
-    g = _load(f"c3_random_{n}_level_8x8_subset_rk4_mean_fp64.npz")
+    g = _load(RANDOM_FIXTURES[n])
     m = models.random_n_level(n, int(g["seed"]), g["eps"], g["amp"])
     res, _ = integrate.rk4(m.f, m.ground_state(), 0.01, 2000, obs=m.obs)
-    _close(res.reshape(8, 8), g["result"])
+    assert g["result"].shape == (256 // int(g["stride"]),) * 2
+    _close(res.reshape(g["result"].shape), g["result"])
+    # the fixture's axes are exact subsets of the 256 x 256 benchmark grid
+    np.testing.assert_array_equal(np.linspace(-5.0, 5.0, 256)[::int(g["stride"])], g["eps"])
+    np.testing.assert_array_equal(np.linspace(0.0, 5.0, 256)[::int(g["stride"])], g["amp"])
 
 
 def test_oracle_reproduces_duffing_fixture():

@@ -40,12 +40,44 @@ def test_four_level_benchmark_subset_fixture(gpu):
         assert _rel(got, g["result"]) <= 1e-10
 
 
-@pytest.mark.parametrize("n,liouville", [(4, "lanes"), (8, "lanes"), (8, "mfma")])
+RANDOM_FIXTURES = {4: "c3_random_4_level_8x8_subset_rk4_mean_fp64.npz",
+                   8: "c3_random_8_level_8x8_subset_rk4_mean_fp64.npz",
+                   16: "c3_random_16_level_8x8_subset_rk4_mean_fp64.npz",
+                   32: "c3_random_32_level_4x4_subset_rk4_mean_fp64.npz"}
+
+
+def _random_fixture(n):
+    g = _load(RANDOM_FIXTURES[n])
+    side = g["result"].shape[0]
+    return g, workloads.random_n_level(n, seed=int(g["seed"]), n_eps=side, n_amp=side)
+
+
+@pytest.mark.parametrize("n,liouville", [(4, "lanes"), (8, "lanes"), (8, "mfma"),
+                                         (16, "mfma"), (16, "lanes"), (32, "mfma")])
 def test_random_n_level_fixture_fp64(gpu, n, liouville):
-    g = _load(f"c3_random_{n}_level_8x8_subset_rk4_mean_fp64.npz")
-    w = workloads.random_n_level(n, seed=int(g["seed"]), n_eps=8, n_amp=8)
+    # FP64, full 2000 steps: within 1e-10 of the map's scale, per-point and matrix-core kernels
+    g, w = _random_fixture(n)
     got = _sweep(w, g["eps"], g["amp"], fp64=True, liouville=liouville)
+    assert got.shape == g["result"].shape
     assert _rel(got, g["result"]) <= 1e-10
+
+
+# Reduced-precision matrix-core forms over the full 2000 steps, as maximum absolute error
+# against the FP64 fixture: FP32 products and FP16 hi/lo split products within the 1e-6 gate
+# (measured 0.8e-7 to 3e-7 on 16 x 16 subsets), BF16 hi/lo (8 + 8 significant bits) within
+# 1e-5 (measured 1.8e-6 to 2.1e-6).
+REDUCED_TOL = {"mfma": 1e-6, "mfma_f16": 1e-6, "mfma_bf16": 1e-5}
+
+
+@pytest.mark.parametrize("n,liouville", [(16, "mfma"), (16, "mfma_f16"), (16, "mfma_bf16"),
+                                         (32, "mfma"), (32, "mfma_f16"), (32, "mfma_bf16")])
+def test_random_n_level_fixture_reduced_precision(gpu, n, liouville):
+    g, w = _random_fixture(n)
+    got = _sweep(w, g["eps"], g["amp"], fp64=False, liouville=liouville)
+    err = float(np.max(np.abs(got - g["result"])))
+    print(f"N={n} {liouville}: max abs error {err:.3e} (map scale "
+          f"{float(np.max(np.abs(g['result']))):.3f})")
+    assert err <= REDUCED_TOL[liouville]
 
 
 def test_random_8_level_fixture_fp32_mfma(gpu):
