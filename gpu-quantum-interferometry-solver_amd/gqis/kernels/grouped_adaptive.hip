@@ -9,9 +9,11 @@
 // size, counters).  Stage inputs go through LDS as in the grouped RK4 kernel; the error norm
 // is a sum over rows, so each wave writes its partial sum and every wave adds the GQ_G
 // partials in the same order: the norm, and with it every accept / reject / step-size
-// decision, is bitwise the same in all waves, so all waves take the same branches and reach
-// the same barriers.  Points are assigned one per lane for the whole integration (no work
-// stealing across groups).
+// decision, is bitwise the same in all waves.  That sameness is needed for correct results,
+// not for progress: the branches that contain barriers (loop exit, trace sample, point
+// completion) are taken on a work-group vote (gq_group_vote), which every wave reads
+// identically, so all waves reach the same barriers even if their copies disagreed.  Points
+// are assigned one per lane for the whole integration (no work stealing across groups).
 //
 // The time atoms of a step's stage times (gq_atoms_stage: about 95 instructions per FP64
 // sin / cos; an FP64 reduction and an FP32 function in FP32 kernels) are the same in every
@@ -63,6 +65,29 @@ __device__ __forceinline__ void gq_share(real* __restrict__ Y, const st_t* __res
     for (int r = 0; r < P::NR; ++r) Y[(P::R0 + r) * 64 + lane] = (real)v[r];
 }
 
+// Work-group OR of each lane's vote bits.  Every wave ORs the bits over its lanes, writes that
+// to its slot of vote[parity], and after one barrier ORs the GQ_G slots: all waves get the same
+// word, so a branch taken on it is uniform over the work-group.  Callers alternate the parity:
+// a slot row is rewritten two votes later, after every wave has passed the barrier of the vote
+// in between (and with it has read this row).
+#define GQ_VOTE_BITS 3
+__device__ __forceinline__ unsigned gq_group_vote(unsigned (*vote)[GQ_G], int parity, int g,
+                                                  unsigned bits) {
+    unsigned wave = 0;
+#pragma unroll
+    for (int b = 0; b < GQ_VOTE_BITS; ++b)
+        if (__any((bits >> b) & 1u)) wave |= 1u << b;
+    vote[parity][g] = wave;   // every lane stores the same wave-uniform word
+    __syncthreads();
+    unsigned all = 0;
+#pragma unroll
+    for (int w = 0; w < GQ_G; ++w) all |= vote[parity][w];
+    return all;
+}
+#define GQ_VOTE_SAMPLE 1u
+#define GQ_VOTE_DONE 2u
+#define GQ_VOTE_LIVE 4u
+
 // Sum of the waves' partial sums for this lane, in wave order (identical in every wave).
 __device__ __forceinline__ double gq_group_sum(double* __restrict__ red, double part, int g,
                                                int lane) {
@@ -78,7 +103,7 @@ __device__ __forceinline__ double gq_group_sum(double* __restrict__ red, double 
 template <int GI>
 __device__ __forceinline__ void gq_adaptive_body(const gq_sweep_args& a,
                                                  real (*s_y)[GQ_D * 64], double* s_red,
-                                                 double* s_tt,
+                                                 double* s_tt, unsigned (*s_vote)[GQ_G],
                                                  const int lane, const long long q,
                                                  const bool valid, const real px, const real py,
                                                  const gq_point& pt) {
@@ -131,8 +156,11 @@ __device__ __forceinline__ void gq_adaptive_body(const gq_sweep_args& a,
 #endif
 
     int attempt = 0;   // the same in every wave: selects the s_tt half of this attempt
+    int vote_parity = 0;
     // lanes with nothing to do (past the end, or zero steps) take part in the barriers only
-    while (__any(live)) {
+    unsigned group = gq_group_vote(s_vote, vote_parity, GI, live ? GQ_VOTE_LIVE : 0u);
+    vote_parity ^= 1;
+    while (group & GQ_VOTE_LIVE) {
         const double remaining = dt_out - tau;
         const bool last = h * 1.01 >= remaining;
         const double hs = last ? remaining : h;
@@ -273,14 +301,29 @@ __device__ __forceinline__ void gq_adaptive_body(const gq_sweep_args& a,
             }
             interval_end = failed || tau >= dt_out;
         }
+        // this attempt's output interval; the trace sample at its end (or at a failure)
+        const int interval = step;
 #if GQ_TRACE
-        // trace sample at the end of an output interval (or at a failure inside it)
-        const bool sample = interval_end && step >= a.trace_offset &&
-                            (step - a.trace_offset) % a.trace_stride == 0 &&
-                            (step - a.trace_offset) / a.trace_stride < a.num_trace;
-        if (__any(sample)) {
+        const bool sample = interval_end && interval >= a.trace_offset &&
+                            (interval - a.trace_offset) % a.trace_stride == 0 &&
+                            (interval - a.trace_offset) / a.trace_stride < a.num_trace;
+#else
+        const bool sample = false;
+#endif
+        if (interval_end) {
+            ++step;
+            tau = 0.0;
+            sub = 0;
+        }
+        const bool done = live && (failed || step >= num_steps);
+        group = gq_group_vote(s_vote, vote_parity, GI,
+                              (sample ? GQ_VOTE_SAMPLE : 0u) | (done ? GQ_VOTE_DONE : 0u) |
+                                  (live && !done ? GQ_VOTE_LIVE : 0u));
+        vote_parity ^= 1;
+#if GQ_TRACE
+        if (group & GQ_VOTE_SAMPLE) {
             const long long ld = a.trace_point_stride;
-            const int kk = (step - a.trace_offset) / a.trace_stride;
+            const int kk = (interval - a.trace_offset) / a.trace_stride;
 #if GQ_TRACE == 1
             gq_share<GI>(s_y[buf], y, lane);
             __syncthreads();
@@ -304,13 +347,7 @@ __device__ __forceinline__ void gq_adaptive_body(const gq_sweep_args& a,
 #endif
         }
 #endif
-        if (interval_end) {
-            ++step;
-            tau = 0.0;
-            sub = 0;
-        }
-        const bool done = live && (failed || step >= num_steps);
-        if (__any(done)) {
+        if (group & GQ_VOTE_DONE) {
             // ---- points finished: outputs -----------------------------------------------
 #if GQ_OUT_FINAL
             gq_share<GI>(s_y[buf], y, lane);
@@ -369,6 +406,7 @@ extern "C" __global__ __launch_bounds__(GQ_GW) void GQ_KERNEL(gq_adaptive_groupe
     __shared__ real s_y[GQ_NBUF][GQ_D * 64];
     __shared__ double s_red[GQ_G * 64];
     __shared__ double s_tt[2 * GQ_NS * GQ_TT_STRIDE];
+    __shared__ unsigned s_vote[2][GQ_G];
     const int g = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int lane = (int)(threadIdx.x & 63u);
     const long long q_raw = (long long)blockIdx.x * 64 + lane;
@@ -377,5 +415,5 @@ extern "C" __global__ __launch_bounds__(GQ_GW) void GQ_KERNEL(gq_adaptive_groupe
     const gq_point pt = gq_locate(a, q);
     const real px = ((const real*)a.par_x)[pt.ix];
     const real py = ((const real*)a.par_y)[pt.iy];
-    GQ_GROUP_DISPATCH(g, gq_adaptive_body, (a, s_y, s_red, s_tt, lane, q, valid, px, py, pt))
+    GQ_GROUP_DISPATCH(g, gq_adaptive_body, (a, s_y, s_red, s_tt, s_vote, lane, q, valid, px, py, pt))
 }

@@ -758,15 +758,16 @@ def grouped_buffers(spec, d, fp64, groups, natoms=0):
     when both fit one compute unit, 1 (a barrier before each write as well) when one does, 0
     when no grouped kernel applies -- the solver has no grouped template, the state does not
     split into ``groups`` non-empty row blocks, or one copy (d x 64 reals) does not fit.  The
-    adaptive kernels also keep their error-norm partials and the FP64 time atoms (``natoms``)
-    of up to 12 stages, double buffered."""
+    adaptive kernels also keep their error-norm partials, the FP64 time atoms (``natoms``) of
+    up to 12 stages, double buffered, and two rows of one vote word per wavefront."""
     if spec.name not in GROUPED_TEMPLATES or groups < 2:
         return 0
     rw = -(-d // groups)
     if (groups - 1) * rw >= d:
         return 0
     copy = d * 64 * (8 if fp64 else 4)
-    extra = (groups * 64 * 8 + 2 * 12 * max(1, natoms) * 64 * 8) if spec.adaptive else 0
+    extra = ((groups * 64 * 8 + 2 * 12 * max(1, natoms) * 64 * 8 + 2 * groups * 4)
+             if spec.adaptive else 0)
     for n in (2, 1):
         if n * copy + extra <= LDS_BYTES_PER_CU:
             return n

@@ -83,3 +83,39 @@ def test_substep_overflow_is_reported(gpu):
                       solver="tsit5a", fp64=True, ignore_non_finite_output=True,
                       adaptive_options={"step_rtol": 1e-14, "step_atol": 1e-14, "max_substeps": 2})
     assert np.all(np.isnan(res))
+
+
+# FP32 right-hand side (the reference's default precision, fp64=False) on the 8 x 8 Duffing
+# subset of the 1024 x 1024 benchmark grid, against the FP64 oracle at the same tolerances.
+# FP32 rounding of the RHS (6e-8 relative) exceeds rtol = 1e-8, so step counts differ from the
+# oracle's and the error grows with each point's sensitivity: a CPU emulation (oracle with the
+# RHS rounded to FP32) gives max 3.5e-7 / 6.8e-7 (Tsit5 / DOP853) at t = 2 and, at t = 20,
+# median 1.1e-6 / 1.9e-6, 90th percentile 2.9e-5 / 6.2e-5, max 3.4e-3 / 2.1e-3 (the FP64
+# oracle's own change from rtol 1e-8 to 1e-12 reaches 1.7e-5 at the most sensitive point).
+# Tolerances below: t = 2 max 5e-6 (FP64 state) / 2e-5 (FP32 state); t = 20 median 2e-5,
+# 90th percentile 5e-4, max 2e-2; every point finite.
+@pytest.mark.parametrize("state_fp64", [True, False])
+@pytest.mark.parametrize("solver", ["tsit5a", "dop853a"])
+def test_duffing_fp32_rhs_against_oracle(gpu, solver, state_fp64):
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden",
+                             "c4_duffing_8x8_subset_t20_tsit5a_fp64.npz"))
+    d = models.Duffing(g["gamma"], g["x0"])
+    fn = integrate.tsit5a if solver == "tsit5a" else integrate.dop853a
+    for n_out, bound in ((20, 5e-6 if state_fp64 else 2e-5), (200, None)):
+        w = workloads.duffing(t_end=0.1 * n_out, n_out=n_out + 1)
+        gs, xs = list(w["var_arrays"])
+        res = odesolve_2D(w["rhs"], w["state_symbols"], w["y0"], w["tlist"],
+                          var_arrays={gs: g["gamma"], xs: g["x0"]}, solver=solver, fp64=False,
+                          adaptive_options={"step_rtol": 1e-8, "step_atol": 1e-10,
+                                            "max_substeps": 4096, "state_fp64": state_fp64})
+        ref, _, _, _ = fn(d.f, d.y0(), 0.1, n_out, rtol=1e-8, atol=1e-10, max_substeps=1000)
+        assert np.all(np.isfinite(res))
+        err = np.abs(res.reshape(-1, 2) - ref).max(axis=1)
+        print(f"{solver} state_fp64={state_fp64} t={0.1 * n_out:g}: median {np.median(err):.2e}"
+              f" p90 {np.percentile(err, 90):.2e} max {err.max():.2e}")
+        if bound is not None:
+            assert err.max() <= bound
+        else:
+            assert np.median(err) <= 2e-5 and np.percentile(err, 90) <= 5e-4
+            assert err.max() <= 2e-2

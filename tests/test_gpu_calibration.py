@@ -137,10 +137,6 @@ def test_calibrated_trace_keeps_the_supplied_sample_times(gpu, solver):
 @pytest.mark.parametrize("solver", ["rk4", "tsit5a"])
 @pytest.mark.parametrize("output", ["final", "mean", "observable", "trace"])
 def test_calibration_meets_limits_for_ode_outputs(gpu, solver, output):
-    if solver == "tsit5a" and output == "mean":
-        pytest.skip("the tsit5a mean is a trapezoid rule over its own steps: with a smooth "
-                    "solution and a coarse output grid that quadrature error is not set by the "
-                    "step tolerance")
     x, v, t, damp, amp = sp.symbols("x v t damping amplitude", real=True)
     axes = {damp: np.linspace(0.1, 0.5, 16), amp: np.linspace(0.1, 1.0, 16)}
 
@@ -151,6 +147,24 @@ def test_calibration_meets_limits_for_ode_outputs(gpu, solver, output):
     extra = {"observable": dict(observable=x ** 2 + v ** 2), "mean": dict(output_mode="mean"),
              "trace": dict(return_time_trace=True, time_trace_every=8), "final": {}}[output]
     core._TIMESTEP_CALIBRATION_CACHE.clear()
+    if solver == "tsit5a" and output == "mean":
+        # The tsit5a mean is the trapezoid rule over the solver's own steps (reference
+        # gqis/cuda_solvers.py:731-908): for a smooth solution its quadrature error is not set
+        # by the step tolerance, so the exact time average is not the target.  What calibration
+        # promises is its Richardson test (order 1 in the tolerance): tightening the chosen
+        # tolerance tenfold moves the mean by at most 0.9 x the limits (MARGIN: the search
+        # checks every 2nd point only), and the final run is an ordinary run at that tolerance.
+        got, info = run(64, solver=solver, calibrate_timesteps=lim, return_timing_info=True,
+                        **extra)
+        rep = info["timestep_calibration"]
+        assert rep["details"]["status"] == "converged"
+        chosen = {"step_rtol": rep["step_rtol"], "step_atol": rep["step_atol"]}
+        plain = run(64, solver=solver, adaptive_options=chosen, **extra)
+        np.testing.assert_array_equal(got, plain)
+        tight = run(64, solver=solver, adaptive_options={"max_substeps": 100000, **{
+            k: val / 10.0 for k, val in chosen.items()}}, **extra)
+        _within(got, tight, 0.9 * MARGIN * lim["rms_atol"], 0.9 * MARGIN * lim["max_atol"])
+        return
     got = run(64, solver=solver, calibrate_timesteps=lim, **extra)
     if output == "trace":
         _, tr, times = got
