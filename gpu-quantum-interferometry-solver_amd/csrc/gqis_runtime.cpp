@@ -30,6 +30,19 @@ int hip_fail(hipError_t err, const char* what) {
     return fail(GQ_E_HIP, std::string(what) + ": " + hipGetErrorString(err));
 }
 
+// Scratch of gq_launch_sweep_noalloc: one array per GPU for the life of the process.  Host
+// code never uses this name as an address (on the host it is only a stand-in); every use goes
+// through the per-device pointer from hipGetSymbolAddress kept in gq_ctx::scratch.
+static __device__ __align__(256) unsigned char scratch[GQ_SCRATCH_BYTES];
+
+// Work cursor (and the caller's status word, when given) back to zero before a chain.
+__global__ void gq_scratch_reset(unsigned int* cursor, unsigned int* status) {
+    if (threadIdx.x == 0) {
+        *cursor = 0u;
+        if (status) *status = 0u;
+    }
+}
+
 // FNV-1a 64-bit over the source, options and architecture: the module cache key.
 unsigned long long fnv1a(const std::string& text, unsigned long long h = 1469598103934665603ull) {
     for (unsigned char c : text) {
@@ -131,6 +144,7 @@ struct gq_ctx {
     std::string arch;
     int compute_units = 0;
     std::string cache_dir;   // on-disk code-object cache; empty = disabled
+    unsigned char* scratch = nullptr;   // this device's copy of `scratch`
     std::mutex mu;
     std::map<unsigned long long, gq_module*> modules;
 };
@@ -153,8 +167,13 @@ int gq_ctx_create(int device_ordinal, gq_ctx** out) {
     hipDeviceProp_t prop;
     e = hipGetDeviceProperties(&prop, device_ordinal);
     if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+    // the device address of this GPU's scratch array (the device is current)
+    void* scratch_ptr = nullptr;
+    e = hipGetSymbolAddress(&scratch_ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
     gq_ctx* ctx = new gq_ctx();
     ctx->device = device_ordinal;
+    ctx->scratch = (unsigned char*)scratch_ptr;
     std::string arch = prop.gcnArchName;
     ctx->arch = arch.substr(0, arch.find(':'));   // "gfx950:sramecc+:xnack-" -> "gfx950"
     ctx->compute_units = prop.multiProcessorCount;
@@ -343,6 +362,48 @@ int gq_graph_launch(gq_graph* graph, void* stream) {
     if (!graph || !graph->exec) return fail(GQ_E_INVALID, "gq_graph_launch: null graph");
     hipError_t e = hipGraphLaunch(graph->exec, (hipStream_t)stream);
     return e == hipSuccess ? GQ_OK : hip_fail(e, "hipGraphLaunch");
+}
+
+int gq_launch_sweep_noalloc(gq_module* module, const gq_launch_desc* launches, int count,
+                            const gq_sweep_args* args, unsigned long long table_bytes,
+                            void* stream) {
+    if (!module || !launches || count <= 0 || !args)
+        return fail(GQ_E_INVALID, "gq_launch_sweep_noalloc: null argument or empty launch list");
+    const unsigned long long table_max = GQ_SCRATCH_BYTES - GQ_SCRATCH_TABLE_OFFSET;
+    if (table_bytes > table_max)
+        return fail(GQ_E_INVALID, "gq_launch_sweep_noalloc: a time table of " +
+                                      std::to_string(table_bytes) + " bytes exceeds the " +
+                                      std::to_string(table_max) + "-byte scratch table");
+    unsigned char* base = module->ctx->scratch;
+    if (!base) return fail(GQ_E_HIP, "gq_launch_sweep_noalloc: no scratch address for the device");
+    std::vector<hipFunction_t> fns((size_t)count);
+    for (int i = 0; i < count; ++i) {
+        if (!launches[i].name || launches[i].grid == 0 || launches[i].block == 0 ||
+            launches[i].block > 1024)
+            return fail(GQ_E_INVALID, "gq_launch_sweep_noalloc: bad launch description");
+        int rc = get_function(module, launches[i].name, &fns[(size_t)i]);
+        if (rc != GQ_OK) return rc;
+    }
+    gq_sweep_args local = *args;
+    local.work_cursor = (unsigned int*)base;
+    if (table_bytes) local.time_table = base + GQ_SCRATCH_TABLE_OFFSET;
+    // the reset kernel runs on the current device: make it the module's for this call
+    int previous = 0;
+    hipError_t e = hipGetDevice(&previous);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (previous != module->ctx->device) {
+        e = hipSetDevice(module->ctx->device);
+        if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    }
+    gq_scratch_reset<<<1, 64, 0, (hipStream_t)stream>>>(local.work_cursor, local.status);
+    e = hipGetLastError();
+    void* params[] = {&local};
+    for (int i = 0; i < count && e == hipSuccess; ++i) {
+        e = hipModuleLaunchKernel(fns[(size_t)i], launches[i].grid, 1, 1, launches[i].block, 1,
+                                  1, 0, (hipStream_t)stream, params, nullptr);
+    }
+    if (previous != module->ctx->device) (void)hipSetDevice(previous);
+    return e == hipSuccess ? GQ_OK : hip_fail(e, "gq_launch_sweep_noalloc");
 }
 
 void gq_graph_destroy(gq_graph* graph) {

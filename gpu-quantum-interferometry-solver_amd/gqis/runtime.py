@@ -15,6 +15,8 @@ MAX_SOLVER_PARAMS = 8
 STATUS_NONFINITE_RESULT = 1
 STATUS_NONFINITE_TRACE = 2
 STATUS_SUBSTEP_OVERFLOW = 4
+SCRATCH_BYTES = 4 << 20          # GQ_SCRATCH_BYTES: the no-allocation entry point's array
+SCRATCH_TABLE_OFFSET = 256
 
 
 class SweepArgs(ctypes.Structure):
@@ -91,6 +93,9 @@ def load_library(path=LIB_PATH):
             "gq_graph_create": (c_int, [c_void_p, ctypes.POINTER(LaunchDesc), c_int,
                                         ctypes.POINTER(SweepArgs), ctypes.POINTER(c_void_p)]),
             "gq_graph_launch": (c_int, [c_void_p, c_void_p]),
+            "gq_launch_sweep_noalloc": (c_int, [c_void_p, ctypes.POINTER(LaunchDesc), c_int,
+                                                ctypes.POINTER(SweepArgs), ctypes.c_ulonglong,
+                                                c_void_p]),
             "gq_graph_destroy": (None, [c_void_p]),
             "gq_event_create": (c_int, [ctypes.POINTER(c_void_p)]),
             "gq_event_record": (c_int, [c_void_p, c_void_p]),
@@ -118,7 +123,7 @@ def exported_symbols():
     return ("gq_ctx_create", "gq_ctx_destroy", "gq_ctx_info", "gq_ctx_set_cache_dir",
             "gq_module_compile",
             "gq_compile_check", "gq_launch_sweep", "gq_resident_groups", "gq_module_release",
-            "gq_graph_create", "gq_graph_launch", "gq_graph_destroy",
+            "gq_graph_create", "gq_graph_launch", "gq_graph_destroy", "gq_launch_sweep_noalloc",
             "gq_event_create", "gq_event_record", "gq_event_elapsed_ms", "gq_event_destroy",
             "gq_stream_synchronize", "gq_status_read", "gq_sweep_args_size", "gq_last_error")
 
@@ -226,6 +231,19 @@ class Context:
         _check(self.lib.gq_graph_create(module, descs, len(launches), ctypes.byref(args),
                                         ctypes.byref(h)), "gq_graph_create")
         return h
+
+    def launch_noalloc(self, module, launches, args, table_bytes, stream=None):
+        """Run ``launches`` ([(kernel name, grid, block)]) once with ``args``, the work cursor
+        and (``table_bytes`` > 0) the time table in the library's static scratch array: no
+        device allocation (``gq_launch_sweep_noalloc``)."""
+        if stream is None:
+            stream = self.torch.cuda.current_stream(self.device).cuda_stream
+        names = [n.encode() for n, _, _ in launches]
+        descs = (LaunchDesc * len(launches))(*[LaunchDesc(nm, int(g), int(b))
+                                                for nm, (_, g, b) in zip(names, launches)])
+        _check(self.lib.gq_launch_sweep_noalloc(module, descs, len(launches), ctypes.byref(args),
+                                                int(table_bytes), ctypes.c_void_p(stream)),
+               "gq_launch_sweep_noalloc")
 
     def graph_launch(self, graph, stream=None):
         if stream is None:

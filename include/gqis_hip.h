@@ -128,6 +128,27 @@ int gq_graph_create(gq_module* module, const gq_launch_desc* launches, int count
 int gq_graph_launch(gq_graph* graph, void* stream);
 void gq_graph_destroy(gq_graph* graph);
 
+/* Size of the library's per-device scratch array used by gq_launch_sweep_noalloc: 256 bytes
+ * for the work cursor, the rest for a time table (4 MiB in all: 524 256 FP64 table entries). */
+#define GQ_SCRATCH_BYTES (4u << 20)
+#define GQ_SCRATCH_TABLE_OFFSET 256u
+
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the launch chain of
+ * gq_graph_create, run once on `stream` like gq_launch_sweep, with the sweep's scratch -- the
+ * work cursor of the persistent adaptive kernels and, when table_bytes > 0, the time table of
+ * the fixed-step kernels (args->time_table and args->work_cursor are replaced) -- in a
+ * library-owned static device array of GQ_SCRATCH_BYTES per GPU instead of caller buffers.
+ * The caller still passes the inputs and outputs; the status word args->status (an output)
+ * and the cursor are cleared by a kernel before the chain, so the chain should not start with
+ * gq_clear_flags.  A time table above GQ_SCRATCH_BYTES - GQ_SCRATCH_TABLE_OFFSET bytes is
+ * rejected (GQ_E_INVALID).  Nothing is allocated.  All calls on one GPU share its array, which
+ * keeps the last call's contents: calls that overlap on one GPU (other streams or threads)
+ * race and must be serialised by the caller. */
+int gq_launch_sweep_noalloc(gq_module* module, const gq_launch_desc* launches, int count,
+                            const gq_sweep_args* args, unsigned long long table_bytes,
+                            void* stream);
+
 /* Event pair timing on a stream (cp.cuda.Event, gqis/solver.py:2316-2345). */
 int gq_event_create(void** event);
 int gq_event_record(void* event, void* stream);
