@@ -131,7 +131,8 @@ struct gq_module {
     hipModule_t module = nullptr;
     std::map<std::string, hipFunction_t> functions;
     std::mutex mu;
-    int refs = 1;
+    int refs = 1;                    // the context's cache holds one; callers hold the others
+    unsigned long long last_use = 0;  // context clock of the last compile call that returned it
 };
 
 struct gq_graph {
@@ -147,7 +148,32 @@ struct gq_ctx {
     unsigned char* scratch = nullptr;   // this device's copy of `scratch`
     std::mutex mu;
     std::map<unsigned long long, gq_module*> modules;
+    unsigned long long clock = 0;
 };
+
+namespace {
+
+// Loaded modules a context keeps for reuse.  Beyond this many, modules that no caller holds
+// any more (only the cache's reference left) are unloaded, least recently used first.
+constexpr size_t kModuleCacheMax = 128;
+
+// With ctx->mu held.
+void evict_idle_modules(gq_ctx* ctx) {
+    while (ctx->modules.size() > kModuleCacheMax) {
+        auto victim = ctx->modules.end();
+        for (auto it = ctx->modules.begin(); it != ctx->modules.end(); ++it) {
+            if (it->second->refs == 1 &&
+                (victim == ctx->modules.end() || it->second->last_use < victim->second->last_use))
+                victim = it;
+        }
+        if (victim == ctx->modules.end()) return;   // every module is still held by a caller
+        if (victim->second->module) (void)hipModuleUnload(victim->second->module);
+        delete victim->second;
+        ctx->modules.erase(victim);
+    }
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -236,6 +262,7 @@ int gq_module_compile(gq_ctx* ctx, const char* source, const char* const* option
         auto it = ctx->modules.find(key);
         if (it != ctx->modules.end()) {
             it->second->refs++;
+            it->second->last_use = ++ctx->clock;
             *out = it->second;
             if (cached_out) *cached_out = 1;
             if (build_seconds) *build_seconds = 0.0;
@@ -269,13 +296,24 @@ int gq_module_compile(gq_ctx* ctx, const char* source, const char* const* option
         if (e != hipSuccess) return hip_fail(e, "hipModuleLoadData");
         if (!path.empty()) write_file_atomic(path, c.code);
     }
-    gq_module* m = new gq_module();
-    m->ctx = ctx;
-    m->module = mod;
-    m->refs = 2;   // one for the cache, one for the caller
+    gq_module* m = nullptr;
     {
         std::lock_guard<std::mutex> lock(ctx->mu);
-        ctx->modules[key] = m;
+        auto it = ctx->modules.find(key);
+        if (it != ctx->modules.end()) {
+            // another thread compiled the same key meanwhile: keep its module, drop this one
+            m = it->second;
+            m->refs++;
+            (void)hipModuleUnload(mod);
+        } else {
+            m = new gq_module();
+            m->ctx = ctx;
+            m->module = mod;
+            m->refs = 2;   // one for the cache, one for the caller
+            ctx->modules[key] = m;
+        }
+        m->last_use = ++ctx->clock;
+        evict_idle_modules(ctx);
     }
     *out = m;
     if (cached_out) *cached_out = cached;
@@ -428,7 +466,9 @@ int gq_resident_groups(gq_module* module, const char* name, unsigned int block, 
 void gq_module_release(gq_module* module) {
     if (!module) return;
     std::lock_guard<std::mutex> lock(module->ctx->mu);
-    if (module->refs > 1) module->refs--;   // the cache keeps its reference until ctx_destroy
+    // the cache keeps its own reference: an unheld module stays loaded for reuse until the
+    // cache outgrows kModuleCacheMax (evict_idle_modules) or the context is destroyed
+    if (module->refs > 1) module->refs--;
 }
 
 int gq_event_create(void** event) {

@@ -152,6 +152,25 @@ def compile_check(source, options, arch="gfx950"):
     return size.value
 
 
+class ModuleRef:
+    """A caller's reference to a loaded module; passed to the library as its handle, and
+    released (``gq_module_release``) when the last Python reference goes away, so the library
+    may unload modules nobody uses once its cache is full."""
+
+    def __init__(self, lib, handle):
+        self._lib = lib
+        self._as_parameter_ = handle
+
+    def __del__(self):
+        lib, handle = getattr(self, "_lib", None), getattr(self, "_as_parameter_", None)
+        if lib is not None and handle:
+            try:
+                lib.gq_module_release(handle)
+            except Exception:       # interpreter shutdown
+                pass
+        self._as_parameter_ = None
+
+
 class Context:
     """Device context: module cache and launch helpers (one per process and device)."""
 
@@ -212,7 +231,7 @@ class Context:
         _check(self.lib.gq_module_compile(self.handle, source.encode(), arr, n, ctypes.byref(mod),
                                           ctypes.byref(cached), ctypes.byref(secs)),
                "hiprtc compile")
-        return mod, int(cached.value), secs.value
+        return ModuleRef(self.lib, mod), int(cached.value), secs.value
 
     def launch(self, module, name, grid, block, args, stream=None):
         if stream is None:

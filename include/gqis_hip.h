@@ -110,6 +110,8 @@ int gq_launch_sweep(gq_module* module, const char* name, unsigned int grid, unsi
 /* Number of work-groups of `block` threads of kernel `name` that fit on the device at once
  * (occupancy x compute units): the grid of a persistent launch. */
 int gq_resident_groups(gq_module* module, const char* name, unsigned int block, int* groups);
+/* Drop the caller's reference from gq_module_compile.  A module no caller holds stays cached
+ * for reuse; beyond 128 cached modules the least recently used unheld ones are unloaded. */
 void gq_module_release(gq_module* module);
 
 /* A fixed chain of launches of one module with one argument struct, recorded once as a HIP

@@ -50,3 +50,63 @@ def test_new_process_loads_code_object_from_disk(gpu, tmp_path):
     assert proc.returncode == 0, proc.stderr[-2000:]
     assert "cached_kernel disk" in proc.stdout
     np.testing.assert_array_equal(np.load(out), first)
+
+
+def _probe_source(value):
+    from gqis import codegen
+    with open(codegen._header_path()) as fh:
+        header = fh.read()
+    return (header + '\nextern "C" __global__ void gq_cache_probe(const gq_sweep_args a) '
+            f"{{ if (threadIdx.x == 0) *a.status = {value}u; }}\n")
+
+
+def _probe(ctx, module, stream):
+    import torch
+    from gqis import runtime
+    status = torch.zeros(1, dtype=torch.int32, device="cuda")
+    a = runtime.SweepArgs()
+    a.status = status.data_ptr()
+    ctx.launch(module, "gq_cache_probe", 1, 64, a, stream)
+    return ctx.read_status(status, stream)
+
+
+def test_module_cache_keeps_held_modules_and_unloads_unheld_ones(gpu):
+    """The context keeps at most 128 modules that no caller holds: compiling 140 throw-away
+    modules unloads the oldest unheld ones (a new compile of one of them is not a cache hit),
+    while a module the caller still holds stays loaded and launchable.  Two threads compiling
+    the same new source get the same module."""
+    import threading
+    import torch
+    ctx = Context.get()
+    stream = torch.cuda.current_stream().cuda_stream
+    tag = uuid.uuid4().hex[:8]
+    opts = ["-O3", f"-DGQ_CACHE_TAG_{tag}=1"]
+    held, cached, _ = ctx.compile(_probe_source(7), opts)
+    assert cached == 0
+    first = None
+    for i in range(140):
+        m, c, _ = ctx.compile(_probe_source(1000 + i), opts)
+        assert c == 0
+        if i == 0:
+            first = m._as_parameter_.value
+        del m
+    assert _probe(ctx, held, stream) == 7
+    again, c, _ = ctx.compile(_probe_source(7), opts)
+    assert c == 1 and again._as_parameter_.value == held._as_parameter_.value
+    m, c, _ = ctx.compile(_probe_source(1000), opts)
+    assert c == 0                     # it was unloaded and is compiled anew
+    assert _probe(ctx, m, stream) == 1000
+    del first
+
+    results = []
+
+    def worker():
+        mod, _, _ = ctx.compile(_probe_source(4242), opts)
+        results.append(mod)
+    threads = [threading.Thread(target=worker) for _ in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert len({r._as_parameter_.value for r in results}) == 1
+    assert _probe(ctx, results[0], stream) == 4242
