@@ -271,6 +271,45 @@ def emit_model(plan, rho0_exprs=None):
     return "\n".join(lines)
 
 
+def emit_model_f64_stage(plan):
+    """FP64 statics and stage values for the mixed-precision RK4 form (FP64 parameters, time
+    terms and state; FP32 right-hand side): ``gq_statics_d`` / ``gq_stage_d`` on double
+    parameters and an FP64 time-atom row, and ``gq_stage_narrow`` rounding each stage value to
+    the kernel's FP32 ``gq_stage_t`` once per stage time (structs ``gq_static64_t``, ``gq_stage64_t``)."""
+    names = _names_for(plan)
+    for i, s in enumerate(plan.const_syms):
+        names[s.name] = f"(double)C[{i}]"
+    pr = HipPrinter(names, literal="DBL")
+    lines = ["struct gq_static64_t {"]
+    lines += [f"    double {s.name};" for s, _ in plan.statics] or ["    double unused_;"]
+    lines.append("};")
+    lines.append("struct gq_stage64_t {")
+    lines += [f"    double {s.name};" for s, _ in plan.stage] or ["    double unused_;"]
+    lines.append("};")
+    lines.append("__device__ __forceinline__ void gq_statics_d(const double px, const double py, "
+                 "const real* __restrict__ C, gq_static64_t& S) {")
+    lines.append("    (void)px; (void)py; (void)C;")
+    lines += [f"    S.{s.name} = {pr.doprint(e)};" for s, e in plan.statics]
+    lines.append("}")
+    lines.append("__device__ __forceinline__ void gq_stage_d(const double px, const double py, "
+                 "const real* __restrict__ C, const gq_static64_t& S, "
+                 "const double* __restrict__ TT, gq_stage64_t& G) {")
+    lines.append("    (void)px; (void)py; (void)C; (void)S; (void)TT;")
+    lines += [f"    G.{s.name} = {pr.doprint(e)};" for s, e in plan.stage]
+    lines.append("}")
+    lines.append("__device__ __forceinline__ void gq_stage_narrow(const gq_stage64_t& Gd, "
+                 "gq_stage_t& G) {")
+    lines.append("    (void)Gd; (void)G;")
+    lines += [f"    G.{s.name} = (real)Gd.{s.name};" for s, _ in plan.stage]
+    lines.append("}")
+    lines.append("__device__ __forceinline__ void gq_statics_narrow(const gq_static64_t& Sd, "
+                 "gq_static_t& S) {")
+    lines.append("    (void)Sd; (void)S;")
+    lines += [f"    S.{s.name} = (real)Sd.{s.name};" for s, _ in plan.statics]
+    lines.append("}")
+    return "\n".join(lines)
+
+
 def _closure(ordered, seeds):
     """The entries of an ordered [(symbol, expr)] list that ``seeds`` reach (in list order);
     an entry's expression may use earlier entries."""

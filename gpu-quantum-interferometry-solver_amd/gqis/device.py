@@ -69,9 +69,12 @@ def run_sweep(*, source, variant, spec, compiled, mode, var_arrays, rho0_values,
     dev = torch.device("cuda", ctx.device)
     rdt = torch.float64 if fp64 else torch.float32
     cdt = torch.complex128 if fp64 else torch.complex64
+    # the mixed-precision RK4 form keeps its sweep values and time table in FP64
+    mixed = spec.template == "fixed_rk4_mixed.hip"
+    adt = torch.float64 if mixed else rdt
     axes = list(var_arrays.values())
-    px = _axis(axes[0], rdt, dev)
-    py = _axis(axes[1], rdt, dev) if len(axes) > 1 else torch.zeros(1, dtype=rdt, device=dev)
+    px = _axis(axes[0], adt, dev)
+    py = _axis(axes[1], adt, dev) if len(axes) > 1 else torch.zeros(1, dtype=adt, device=dev)
     nx, ny = int(px.numel()), int(py.numel())
     npts = nx * ny
     size = compiled.state_size
@@ -106,7 +109,7 @@ def run_sweep(*, source, variant, spec, compiled, mode, var_arrays, rho0_values,
     from .tableaux import rows_per_step
     rows = rows_per_step(spec.name) * num_steps + 1
     if use_table:
-        table = torch.empty(rows * natoms, dtype=rdt, device=dev)
+        table = torch.empty(rows * natoms, dtype=adt, device=dev)
     steps_out = (torch.zeros((npts, 2), dtype=torch.int32, device=dev)
                  if spec.adaptive and spec.options.get("count_steps") else None)
 
@@ -246,4 +249,5 @@ def _timed_launches(ctx, module, kname, grid, block, args, stream, cursor, repea
 
 
 def table_kernel(spec):
-    return "gq_rk4_time_table" if spec.template == "fixed_rk4.hip" else "gq_fixed_time_table"
+    return ("gq_rk4_time_table" if spec.template in ("fixed_rk4.hip", "fixed_rk4_mixed.hip")
+            else "gq_fixed_time_table")

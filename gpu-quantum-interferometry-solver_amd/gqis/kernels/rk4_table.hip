@@ -18,8 +18,14 @@ extern "C" __global__ void GQ_KERNEL(gq_rk4_time_table)(const gq_sweep_args a) {
     }
     double TT[GQ_NATOMS_ALLOC];
     gq_atoms_f64(t_local + a.t_in, C, TT);
+#if GQ_TABLE_F64
+    double* __restrict__ tab = (double*)a.time_table;   // the mixed-precision form's FP64 table
+#pragma unroll
+    for (int k = 0; k < GQ_NATOMS; ++k) tab[s * GQ_NATOMS + k] = TT[k];
+#else
     real* __restrict__ tab = (real*)a.time_table;
 #pragma unroll
     for (int k = 0; k < GQ_NATOMS; ++k) tab[s * GQ_NATOMS + k] = (real)TT[k];
+#endif
 }
 

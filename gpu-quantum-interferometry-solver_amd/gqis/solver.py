@@ -7,6 +7,7 @@ HIP: the kernel source is emitted from the symbolic plan, compiled with hiprtc a
 through ``libgqis_hip.so``; torch provides the device arrays and the stream.
 """
 from collections import ChainMap
+import dataclasses
 from dataclasses import dataclass, field
 import os
 import time
@@ -350,6 +351,18 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
     if kernel_template_file is not None:
         raise ValueError("kernel_template_file: the HIP kernels are built from the package's "
                          "solver templates; custom CUDA templates are not supported.")
+    mixed = isinstance(fp64, str)
+    if mixed:
+        if fp64 != "mixed":
+            raise ValueError("fp64 must be True, False or 'mixed'.")
+        if (normalize_solver_name(solver) != "rk4" or not time_table
+                or lanes_per_point not in ("auto", 1) or liouville not in ("auto", "lanes")
+                or return_time_trace):
+            raise ValueError("fp64='mixed' (FP64 state, parameters and time terms with an FP32 "
+                             "right-hand side) is a form of the one-lane RK4 kernel with a time "
+                             "table: solver 'rk4', time_table=True, lanes_per_point 1, no "
+                             "matrix-core path and no time trace.")
+        fp64, liouville, lanes_per_point = False, "lanes", 1
     t_in = float(t_in)
     if not np.isfinite(t_in):
         raise ValueError("t_in must be finite.")
@@ -394,6 +407,8 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
     num_steps, dt, t_host, diffs, tol = _check_tlist(tlist)
     spec = get_solver_spec(solver, frequency=solver_frequency, step_size=dt,
                            adaptive_options=adaptive_options, fp64=fp64)
+    if mixed:
+        spec = dataclasses.replace(spec, template="fixed_rk4_mixed.hip", kernel="gq_rk4_mixed")
     if not np.isclose(t_host[0], 0.0, rtol=0.0, atol=tol):
         raise ValueError("tlist must begin at zero; the kernel derives stage times from the "
                          "step index.")
@@ -538,7 +553,7 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
     # 3) device work
     from . import device
     outputs = device.run_sweep(
-        source=source, emit=emit_for, lanes_per_point=lanes_per_point,
+        source=source, emit=None if mixed else emit_for, lanes_per_point=lanes_per_point,
         variant=variant, spec=spec, compiled=compiled, mode=mode,
         var_arrays=var_arrays, rho0_values=rho0_values, const_vals=const_vals, fp64=fp64,
         dt=dt, t_in=t_in, num_steps=num_steps, warmup_steps=warmup_steps,
@@ -696,7 +711,8 @@ def _kernel_variant(compiled, spec, mode, uses_rho0_values, trace, state_trace, 
                     block_size, unroll, options):
     adaptive_key = tuple(sorted((k, v) for k, v in spec.options.items()
                                 if k in ("count_steps", "state_fp64", "persistent")))
-    return (compiled.cache_key, spec.name, adaptive_key, mode, bool(uses_rho0_values),
+    return (compiled.cache_key, spec.name, spec.template, adaptive_key, mode,
+            bool(uses_rho0_values),
             bool(trace), bool(state_trace), bool(fp64), bool(time_table),
             block_size if block_size == "auto" else int(block_size),
             bool(unroll), options)
@@ -790,8 +806,8 @@ def _emit_source(compiled, spec, mode, uses_rho0_values, trace, state_trace, fp6
         "GQ_RHO0_VALUES": int(bool(uses_rho0_values)),
         "GQ_TIME_TABLE": int(bool(time_table) and not spec.adaptive),
     }
-    if spec.template == "fixed_rk4.hip":
-        templates = ["rk4_table.hip", "fixed_rk4.hip"]
+    if spec.template in ("fixed_rk4.hip", "fixed_rk4_mixed.hip"):
+        templates = ["rk4_table.hip", spec.template]
     elif spec.template == "fixed_tableau.hip":
         templates = ["tableau_table.hip", "fixed_tableau.hip"]
     else:
@@ -811,6 +827,9 @@ def _emit_source(compiled, spec, mode, uses_rho0_values, trace, state_trace, fp6
         defines["GQ_PERSISTENT"] = int(bool(spec.options.get("persistent", True)))
     elif spec.template == "fixed_tableau.hip":
         extra = tableaux.emit_fixed(spec.name)
+    elif spec.template == "fixed_rk4_mixed.hip":
+        extra = codegen.emit_model_f64_stage(compiled.plan)
+        defines["GQ_TABLE_F64"] = 1
     if groups > 1:
         block_size = 64 * groups
         nbuf = grouped_buffers(spec, compiled.state_size, fp64, groups,

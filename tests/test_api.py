@@ -163,6 +163,10 @@ def _minimal():
     ([0.0, 1.0], {"unroll": "False"}, "unroll must be True or False"),
     ([0.0], {}, "at least two"),
     ([0.0, 1.0], {"warmup_time": 1.5}, "warmup_time"),
+    ([0.0, 1.0], {"fp64": "half"}, "fp64 must be"),
+    ([0.0, 1.0], {"fp64": "mixed", "solver": "tsit5a"}, "one-lane RK4"),
+    ([0.0, 1.0], {"fp64": "mixed", "time_table": False}, "one-lane RK4"),
+    ([0.0, 1.0], {"fp64": "mixed", "lanes_per_point": 2}, "one-lane RK4"),
 ])
 def test_validation_before_launch(fake, tlist, kw, match):
     # reference tests/test_package.py:88-116
@@ -170,6 +174,14 @@ def test_validation_before_launch(fake, tlist, kw, match):
     with pytest.raises(ValueError, match=match):
         mesolve_2D(h, d, c, o, tlist, var_arrays=sweep, **kw)
     assert fake == []
+
+
+def test_mixed_form_emits_the_mixed_kernel(fake):
+    h, d, c, o, sweep = _minimal()
+    mesolve_2D(h, d, c, o, [0.0, 0.5, 1.0], var_arrays=sweep, fp64="mixed")
+    src = fake[-1]
+    assert "gq_rk4_mixed" in src and "#define GQ_TABLE_F64 1" in src
+    assert "gq_stage_narrow" in src and "typedef float real;" in src
 
 
 def test_invalid_ode_input(fake):

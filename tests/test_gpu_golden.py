@@ -101,3 +101,20 @@ def test_duffing_tsit5a_fixture(gpu):
     np.testing.assert_array_equal(info["adaptive_steps"]["accepted"], g["accepted"])
     np.testing.assert_array_equal(info["adaptive_steps"]["rejected"], g["rejected"])
     assert _rel(res, g["result"]) <= 1e-10
+
+
+@pytest.mark.parametrize("acc64", [False, True])
+def test_four_level_fixture_mixed_precision(gpu, acc64):
+    # fp64="mixed": FP64 state, parameters and time terms with an FP32 right-hand side.  The
+    # FP32 rounding of the right-hand side's values stays: on this 16 x 16 subset of the 2048 x
+    # 2048 grid the error is of the order of 1e-6 (measured against the FP64 fixture), checked
+    # here at 5e-6 absolute; the full-grid figures are in the four-level benchmark line.
+    g = _load("c2_four_level_16x16_subset_rk4_mean_fp64.npz")
+    w = workloads.four_level(n_eps=16, n_amp=16)
+    opts = ("-DGQ_MIXED_ACC64=1",) if acc64 else ()
+    got = _sweep(w, g["eps"], g["amp"], fp64="mixed", hiprtc_options=opts)
+    plain32 = _sweep(w, g["eps"], g["amp"], fp64=False)
+    err = float(np.max(np.abs(got - g["result"])))
+    err32 = float(np.max(np.abs(plain32 - g["result"])))
+    print(f"mixed acc64={acc64}: max abs error {err:.3e} (FP32 form {err32:.3e})")
+    assert got.dtype == np.complex64 and err <= 5e-6
