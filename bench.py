@@ -48,6 +48,46 @@ def algorithmic_flops_per_point_step(compiled):
     return 4 * rhs + 13 * d + obs, rhs
 
 
+def _expr_flops(e):
+    """Arithmetic operations of an expression (integer powers as repeated products; calls of
+    transcendental functions are counted apart, not here)."""
+    import sympy as sp
+    ops = sp.count_ops(e, visual=True)
+    total = 0
+    for term in sp.Add.make_args(ops):
+        coeff, sym = term.as_coeff_Mul()
+        name = str(sym)
+        if name == "POW":
+            continue
+        if name in ("ADD", "SUB", "MUL", "NEG", "DIV"):
+            total += int(coeff)
+    for p in e.atoms(sp.Pow):
+        if p.exp.is_Integer and p.exp > 1:
+            total += int(p.exp) - 1
+    return total
+
+
+def adaptive_flops_per_attempt(plan, tableau, d):
+    """Algorithmic FP64 FLOP of one attempted embedded-RK step of a d-state ODE: the stage
+    right-hand sides and stage values (every stage after the first, plus the FSAL derivative),
+    the stage inputs y + h sum a_sj k_j, the solution, the error estimate(s) and the RMS norm
+    (2 per multiply-add; the step controller counted as 10).  Returns (FLOP, transcendental
+    calls): one call per time atom and stage time."""
+    ns = len(tableau["c"])
+    f_rhs = sum(_expr_flops(e) for e in plan.rhs_out) + sum(_expr_flops(e)
+                                                           for _, e in plan.rhs_temps)
+    f_stage = sum(_expr_flops(e) for _, e in plan.stage)
+    flops = ns * (f_rhs + f_stage)
+    for row in tableau["a"][1:]:
+        flops += d * (2 * sum(1 for v in row if v != 0.0) + 2)
+    flops += d * (2 * sum(1 for v in tableau["b"] if v != 0.0) + 2)
+    for key in ("e", "e5", "e3"):
+        if key in tableau:
+            flops += d * 2 * sum(1 for v in tableau[key] if v != 0.0)
+    flops += d * 6 + 10
+    return flops, (ns - 1) * len(plan.time_atoms)
+
+
 def run_sweeps(w, fp64, k, time_table=True, **kw):
     import torch
     from gqis import mesolve_2D
@@ -245,7 +285,8 @@ def main():
     ap.add_argument("--grid", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--profile", choices=["fp64", "fp32", "plain", "mixed", "mixed64"],
+    ap.add_argument("--profile", choices=["fp64", "fp32", "plain", "mixed", "mixed64",
+                                          "tsit5a", "dop853a"],
                     default=None,
                     help="run only `--steps` sweeps of one form (for rocprofv3), no JSON line")
     ap.add_argument("--requests", type=int, default=1000, help="serving: number of requests")
@@ -444,6 +485,23 @@ def bench_duffing(args):
            "config": {"workload": f"forced Duffing {side}x{side}, t 0..500, dt_out 0.1, "
                                   "rtol 1e-8 atol 1e-10", "points": side * side}}
     finals = {}
+    from gqis import tableaux
+    from gqis.gqis_ode_api import ode_model
+    from gqis.solver import _compile_rhs
+    model = ode_model(w["rhs"], w["state_symbols"], w["y0"], w["var_arrays"])
+    compiled, _ = _compile_rhs(model.equations, {}, [], model.observable,
+                               sweep_symbols=tuple(model.sweeps), cse_simplify=True,
+                               pre_expand=False, **model.symbol_kwargs())
+    tabs = {"tsit5a": tableaux.TSIT5, "dop853a": tableaux.DOP853}
+    if args.profile in tabs:
+        ao = {"step_rtol": 1e-8, "step_atol": 1e-10, "max_substeps": 4096}
+        kw = dict(var_arrays=w["var_arrays"], solver=args.profile, fp64=True, adaptive_options=ao,
+                  return_device=True, return_timing_info=True, ignore_non_finite_output=True)
+        odesolve_2D(w["rhs"], w["state_symbols"], w["y0"], w["tlist"][:11], **kw)
+        for _ in range(max(1, args.steps)):
+            _, info = odesolve_2D(w["rhs"], w["state_symbols"], w["y0"], w["tlist"], **kw)
+            log(f"profile duffing {args.profile}: kernel {info['gpu_kernel_s']:.6f} s")
+        return
     for solver in ("tsit5a", "dop853a"):
         for label, fp64, opts in (("fp64", True, {}), ("fp32_rhs", False, {"state_fp64": True})):
             ao = {"step_rtol": 1e-8, "step_atol": 1e-10, "max_substeps": 4096,
@@ -468,6 +526,16 @@ def bench_duffing(args):
                                        "min": int(acc.min())},
                 "value": float(acc.sum() / k),
                 "nonfinite_points": int(np.sum(~np.isfinite(finals[(solver, label)]).any(-1)))}
+            if fp64:
+                f_att, n_trans = adaptive_flops_per_attempt(compiled.plan, tabs[solver], 2)
+                attempts = int(acc.sum() + rej.sum())
+                ach = f_att * attempts / k / 1e12
+                out[f"{solver}_{label}"]["roofline"] = {
+                    "bound": "valu", "achieved": ach, "peak": PEAK_TFLOPS["fp64"],
+                    "unit": "TFLOP/s", "frac": ach / PEAK_TFLOPS["fp64"],
+                    "flops_per_attempted_step": f_att, "attempted_steps": attempts,
+                    "transcendentals_per_attempted_step": n_trans,
+                    "kernel": "gq_adaptive_f64"}
         a, b = finals[(solver, "fp64")], finals[(solver, "fp32_rhs")]
         d = np.abs(a - b).max(-1).ravel()
         out[f"{solver}_fp32_rhs_vs_fp64"] = {"median": float(np.nanmedian(d)),
@@ -493,6 +561,7 @@ def bench_duffing(args):
                                fin_s, x.reshape(side, side, 2)[np.ix_(gi, gi)]))}
     t_best = out["tsit5a_fp64"]
     out["value"] = t_best["value"]
+    out["roofline"] = t_best["roofline"]
     out["dtype"] = "f64"
     print(json.dumps(out), flush=True)
 

@@ -158,14 +158,15 @@ def _names_for(plan):
 
 def _atoms_stage_lines(plan, const_double, const_real):
     """``gq_atoms_stage``: the time atoms at an FP64 time, in the kernel's real type (the
-    adaptive kernels' stage times).  FP64 kernels evaluate them in FP64.  FP32 kernels reduce
-    each sin / cos of an affine function of t to [-pi, pi] in FP64 (two-part 2 pi, Cody and
-    Waite) and take the FP32 function of the reduced angle: FP32 accuracy, as the FP32 RHS,
-    for a fraction of the FP64 evaluation.  Atoms with t outside such a function stay FP64."""
+    adaptive kernels' stage times).  Each sin / cos of an affine function of t is evaluated
+    without the library function: FP64 kernels by ``gq_sincos64`` (Cody-Waite reduction to
+    |r| <= pi/4 and Taylor polynomials, within 2 ulp of the FP64 function), FP32 kernels by
+    reducing the FP64 angle to [-pi, pi] (two-part 2 pi) and taking the FP32 function of it:
+    FP32 accuracy, as the FP32 RHS, for a fraction of the FP64 evaluation.  Other atoms (t
+    outside such a function) are evaluated in FP64."""
     lines = ["__device__ __forceinline__ void gq_atoms_stage(const double t_now, "
              "const real* __restrict__ C, real* __restrict__ TT) {",
-             "    (void)t_now; (void)C; (void)TT;", "#if GQ_FP64",
-             "    gq_atoms_f64(t_now, C, TT);", "#else"]
+             "    (void)t_now; (void)C; (void)TT;"]
     dnames = {"t": "t_now", **const_double}
     dpr = HipPrinter(dnames, literal="DBL")
     rnames = dict(const_real)
@@ -182,15 +183,19 @@ def _atoms_stage_lines(plan, const_double, const_real):
                     reduced[f] = sp.Symbol(name, real=True)
                     rnames[name] = name
                     fn = "cos" if isinstance(f, sp.cos) else "sin"
-                    lines.append(f"    const real {name} = {fn}((real)gq_reduce_2pi("
-                                 f"{dpr.doprint(arg)}));")
+                    a = dpr.doprint(arg)
+                    lines += ["#if GQ_FP64",
+                              f"    const real {name} = gq_{fn}64({a});",
+                              "#else",
+                              f"    const real {name} = {fn}((real)gq_reduce_2pi({a}));",
+                              "#endif"]
                 subs[f] = reduced[f]
         rest = e.xreplace(subs)
         if subs and not any(x.name == "t" for x in rest.free_symbols):
             lines.append(f"    TT[{i}] = {HipPrinter(rnames).doprint(rest)};")
         else:
             lines.append(f"    TT[{i}] = (real)({dpr.doprint(e)});")
-    lines += ["#endif", "}"]
+    lines += ["}"]
     return lines
 
 
@@ -472,6 +477,48 @@ def kernel_name(base, fp64):
     return f"{base}_{kernel_suffix(fp64)}"
 
 
+def _sincos64_lines():
+    """FP64 sin / cos without a library call, for the adaptive kernels' stage-time atoms:
+    x = k pi/2 + r with |r| <= pi/4 by Cody and Waite's reduction (pi/2 in two FP64 parts; with
+    FMA the first difference x - k * (pi/2)_hi is exact for |k| < 2^31), the Taylor polynomials
+    of sin and cos in r to degree 17 / 16 by Horner's rule (the first omitted terms are below
+    1e-19 at |r| = pi/4), and the quadrant k mod 4.  About 30 FP64 instructions for both."""
+    from fractions import Fraction
+    from math import factorial
+    pio2 = sp.pi.evalf(40) / 2
+    hi = float(pio2)
+    lo = float(pio2 - sp.Float(hi, 40))
+    inv = float(2 / sp.pi.evalf(40))
+
+    def coef(n):
+        return repr(float(Fraction((-1) ** (n // 2), factorial(n))))
+    # sin r = r + r^3 (c3 + r^2 (c5 + ...)), cos r = 1 + r^2 (c2 + r^2 (c4 + ...))
+    sin_terms = [coef(n) for n in range(17, 2, -2)]
+    cos_terms = [coef(n) for n in range(16, 1, -2)]
+    lines = ["__device__ __forceinline__ void gq_sincos64(double x, double* s, double* c) {",
+             f"    const double k = rint(x * {inv!r});",
+             f"    double r = fma(-k, {hi!r}, x);",
+             f"    r = fma(-k, {lo!r}, r);",
+             "    const double r2 = r * r;",
+             f"    double ps = {sin_terms[0]};"]
+    lines += [f"    ps = fma(ps, r2, {c});" for c in sin_terms[1:]]
+    lines += [f"    double pc = {cos_terms[0]};"]
+    lines += [f"    pc = fma(pc, r2, {c});" for c in cos_terms[1:]]
+    lines += ["    const double sr = fma(r * r2, ps, r);",
+              "    const double cr = fma(r2, pc, 1.0);",
+              "    const int q = (int)k & 3;",
+              "    const double sa = (q & 1) ? cr : sr;",
+              "    const double ca = (q & 1) ? sr : cr;",
+              "    *s = (q & 2) ? -sa : sa;",
+              "    *c = ((q + 1) & 2) ? -ca : ca;",
+              "}",
+              "__device__ __forceinline__ double gq_sin64(double x) {",
+              "    double s, c;", "    gq_sincos64(x, &s, &c);", "    return s;", "}",
+              "__device__ __forceinline__ double gq_cos64(double x) {",
+              "    double s, c;", "    gq_sincos64(x, &s, &c);", "    return c;", "}"]
+    return lines
+
+
 def prelude(fp64, acc64, block_size, defines):
     real = "double" if fp64 else "float"
     acc = "double" if (fp64 or acc64) else "float"
@@ -491,7 +538,7 @@ def prelude(fp64, acc64, block_size, defines):
            "__device__ __forceinline__ double gq_reduce_2pi(double x) {",
            "    const double k = rint(x * 0.15915494309189535);",
            "    return fma(-k, 2.4492935982947064e-16, fma(-k, 6.283185307179586, x));",
-           "}"]
+           "}"] + _sincos64_lines()
     for k, v in defines.items():
         out.append(f"#define {k} {v}")
     out += ["#ifndef GQ_HILBERT", "#define GQ_HILBERT 0", "#endif"]
