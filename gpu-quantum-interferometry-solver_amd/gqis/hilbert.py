@@ -203,6 +203,9 @@ def pack_a_dmfma(m):
 
 
 SPLIT_KINDS = {None: 0, "f16": 1, "bf16": 2}
+# the sparse form uses compile-time XOR masks when the model has the qubit-register structure
+# (``xor_form``); False forces its table form (kept for tests of that form)
+SPARSE_XOR = True
 
 
 def _round_half(x, kind):
@@ -214,9 +217,12 @@ def _round_half(x, kind):
     return rounded.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def kernel_base(split=None, fp64=False, valu=False):
+def kernel_base(split=None, fp64=False, valu=False, sparse=False):
     """Kernel name (before the precision suffix): FP32 MFMA / split-precision forms, the FP64
-    matrix-core form (gq_hilbert_rk4_dmfma_f64) or, with ``valu``, the FP64 VALU form."""
+    matrix-core form (gq_hilbert_rk4_dmfma_f64), with ``valu`` the FP64 VALU form, with
+    ``sparse`` the wavefront-per-point form for a sparse H_eff."""
+    if sparse:
+        return "gq_hilbert_sparse"
     if fp64 and not valu:
         return "gq_hilbert_rk4_dmfma"
     return {None: "gq_hilbert_rk4", "f16": "gq_hilbert_rk4_f16x3",
@@ -249,6 +255,58 @@ def pack_a_split(m, kind):
     return hi, lo
 
 
+def ell_table(m):
+    """Off-diagonal nonzeros of H_eff as an ELL table: K = the largest count in a row (at
+    least 1); row i's entries (column, value), padded with (i, 0)."""
+    n = m.n
+    off = [[k for k in range(n) if k != i and m.h_eff[i, k] != 0] for i in range(n)]
+    width = max(1, max(len(r) for r in off))
+    cols = np.tile(np.arange(n)[:, None], (1, width))
+    vals = np.zeros((n, width), complex)
+    for i, r in enumerate(off):
+        cols[i, :len(r)] = r
+        vals[i, :len(r)] = m.h_eff[i, r]
+    return width, cols, vals
+
+
+def xor_form(m):
+    """The qubit-register structure of H_eff and the jump operators, or None: every row i has
+    its off-diagonal nonzeros at columns i ^ mask_k with values v_k that do not depend on i
+    (sums of sx / sy on single qubits), and every general jump operator maps row i to i ^ b for
+    the rows with i & b == cond, all with one value l (a qubit's lowering or raising operator).
+    Returns (masks, values, [(b, cond, |l|^2)])."""
+    n = m.n
+    _, cols, vals = ell_table(m)
+    ref = None
+    for i in range(n):
+        row = {int(c) ^ i: complex(v) for c, v in zip(cols[i], vals[i]) if v != 0}
+        if ref is None:
+            ref = row
+        elif row.keys() != ref.keys() or not all(
+                np.isclose(row[k], ref[k], rtol=1e-14, atol=0.0) for k in row):
+            return None
+    masks = sorted(ref) or [0]
+    values = [ref.get(k, 0.0) for k in masks]
+    mono = []
+    for kk in range(n_mono(m)):
+        col, lv = m.mcol[kk], m.ml[kk]
+        rows = [i for i in range(n) if col[i] >= 0]
+        flips = {int(col[i]) ^ i for i in rows}
+        if len(flips) != 1:
+            return None
+        b = flips.pop()
+        conds = {i & b for i in rows}
+        if len(conds) != 1:
+            return None
+        cond = conds.pop()
+        if any(((i & b) == cond) != (col[i] >= 0) for i in range(n)):
+            return None
+        if not np.allclose(lv[rows], lv[rows[0]], rtol=1e-14, atol=0.0):
+            return None
+        mono.append((b, cond, abs(complex(lv[rows[0]])) ** 2))
+    return masks, values, mono
+
+
 def n_mono(m):
     """Number of collapse operators handled by the general gather (one nonzero per row)."""
     return 0 if m.mcol is None else int(m.mcol.shape[0])
@@ -276,7 +334,11 @@ def constants(m, runtime_values, split=None):
     mono = n_mono(m)
     mcol = m.mcol if mono else np.full((1, n), -1)
     ml = m.ml if mono else np.zeros((1, n), complex)
+    _, ecol, evals = ell_table(m)
     parts = {"RC": rc, "APACK": pack_a(m).ravel(), "HBIG": big.ravel(),
+             "ECOL": ecol.ravel().astype(np.float64), "ERE": evals.real.ravel(),
+             "EIM": evals.imag.ravel(), "HDR": np.diag(m.h_eff).real.copy(),
+             "HDI": np.diag(m.h_eff).imag.copy(),
              "DPACK": pack_a_dmfma(m).ravel(), "DEP": m.dep.ravel(),
              "JW": jw.ravel(), "JB": jb.ravel(), "VCOEF": m.vcoef.ravel(), "OBS": m.obs,
              "MCOL": mcol.ravel().astype(np.float64), "MRE": ml.real.ravel(),
@@ -295,10 +357,17 @@ def constants(m, runtime_values, split=None):
 LDS_BYTES_PER_CU = 160 * 1024      # gfx950
 
 
-def lds_bytes(n, fp64, block, jmax, split=None, valu=False, mono=0):
+def lds_bytes(n, fp64, block, jmax, split=None, valu=False, mono=0, sparse_k=0):
     """Static LDS of the kernel (the ``__shared__`` arrays of ``kernels/hilbert_rk4.hip`` and,
     for FP64 without ``valu``, ``kernels/hilbert_rk4_dmfma.hip``); ``mono`` general jump
-    operators add a per-wavefront copy of the state and their row tables."""
+    operators add a per-wavefront copy of the state and their row tables.  ``sparse_k`` > 0:
+    ``kernels/hilbert_sparse.hip`` with that ELL width (a state copy per wavefront)."""
+    if sparse_k:
+        # state copies, the static diagonal coefficients, drive rows (up to 8 basis terms),
+        # jump lists and, for the table form, the ELL and jump-operator tables
+        r = 8 if fp64 else 4
+        return ((block // 64 + 1) * n * n * 2 * r + 8 * n * r + n * jmax * (r + 4)
+                + n * sparse_k * (4 + 2 * r) + mono * n * (4 + 2 * r))
     if fp64 and not valu:
         waves, pts = block // 64, (2 if n == 8 else 1)
         total = (n // 8) ** 2 * 4 * 64 * 8 + waves * (2 * pts * n * (n + 1) + pts * n + 64) * 8 \
@@ -323,23 +392,26 @@ def lds_bytes(n, fp64, block, jmax, split=None, valu=False, mono=0):
     return total
 
 
-def block_for(m, fp64, block_size=256, valu=False, split=None):
+def block_for(m, fp64, block_size=256, valu=False, split=None, sparse=False):
     """Work-group size: the requested one, halved until the kernel's LDS fits a compute unit
     (the FP64 VALU kernel at N = 32, and the FP64 matrix-core kernel at N = 32 with general
     jump operators, run two waves per work-group)."""
     if block_size == "auto":
         return block_size
     b = int(block_size)
+    kw = ell_table(m)[0] if sparse else 0
     while b > 64 and lds_bytes(m.n, fp64, b, jump_width(m), split, valu,
-                               n_mono(m)) > LDS_BYTES_PER_CU:
+                               n_mono(m), kw) > LDS_BYTES_PER_CU:
         b //= 2
     return b
 
 
-def emit(m, offs, mode, block_size=256, fp64=False, split=None, valu=False, trace=False):
+def emit(m, offs, mode, block_size=256, fp64=False, split=None, valu=False, trace=False,
+         sparse=False):
     """hiprtc source of the Hilbert-space RK4 kernel for model m (``split``: the commutator
     GEMM as three FP16 or BF16 products of hi/lo parts with FP32 accumulation; FP64 runs on the
-    FP64 matrix cores, or with ``valu`` as VALU dot products)."""
+    FP64 matrix cores, or with ``valu`` as VALU dot products; ``sparse``: the wavefront-per-
+    point kernel over an ELL table of H_eff, any precision)."""
     split_kind = SPLIT_KINDS[split]
     atoms = {}
     names = {"ParX": "px", "ParY": "py"}
@@ -353,7 +425,25 @@ def emit(m, offs, mode, block_size=256, fp64=False, split=None, valu=False, trac
     ftype = "double" if fp64 else "float"
     lines = [f"#define GQ_HN {m.n}", f"#define GQ_NBASIS {len(m.basis)}",
              f"#define GQ_JMAX {jump_width(m)}", f"#define GQ_NMONO {n_mono(m)}",
-             f"#define GQ_NATOMS {len(atom_list)}", f"#define GQ_NATOMS_ALLOC {max(1, len(atom_list))}"]
+             f"#define GQ_NATOMS {len(atom_list)}", f"#define GQ_NATOMS_ALLOC {max(1, len(atom_list))}",
+             f"#define GQ_HK {ell_table(m)[0]}"]
+    xf = xor_form(m) if sparse and SPARSE_XOR else None
+    lines.append(f"#define GQ_XOR {int(xf is not None)}")
+    if xf is not None:
+        masks, values, mono = xf
+        lines.append(f"static constexpr int gq_xmask[{len(masks)}] = "
+                     f"{{{', '.join(str(k) for k in masks)}}};")
+        lines.append(f"static constexpr real gq_xre[{len(masks)}] = "
+                     f"{{{', '.join(repr(float(v.real)) for v in values)}}};")
+        lines.append(f"static constexpr real gq_xim[{len(masks)}] = "
+                     f"{{{', '.join(repr(float(v.imag)) for v in values)}}};")
+        if mono:
+            lines.append(f"static constexpr int gq_mmask[{len(mono)}] = "
+                         f"{{{', '.join(str(b) for b, _, _ in mono)}}};")
+            lines.append(f"static constexpr int gq_mcond[{len(mono)}] = "
+                         f"{{{', '.join(str(c) for _, c, _ in mono)}}};")
+            lines.append(f"static constexpr real gq_mw[{len(mono)}] = "
+                         f"{{{', '.join(repr(float(w)) for _, _, w in mono)}}};")
     for k, v in offs.items():
         lines.append(f"#define GQ_OFF_{k} {v}")
     dnames = dict(names)
@@ -380,16 +470,20 @@ def emit(m, offs, mode, block_size=256, fp64=False, split=None, valu=False, trac
                "GQ_OUT_MEAN_STATE": 0, "GQ_TRACE": int(bool(trace)),
                "GQ_RHO0_VALUES": "GQ_RHO0_FROM_ARGS",
                "GQ_MFMA_SPLIT": split_kind}
+    template = ("hilbert_sparse.hip" if sparse else
+                "hilbert_rk4_dmfma.hip" if fp64 and not valu else "hilbert_rk4.hip")
     parts = [header, codegen.prelude(fp64, True, block_size, defines), model_src,
              codegen.read_kernel_template("common.hip"), codegen.read_kernel_template("rk4_table.hip"),
-             codegen.read_kernel_template("hilbert_rk4_dmfma.hip" if fp64 and not valu
-                                          else "hilbert_rk4.hip")]
+             codegen.read_kernel_template(template)]
     return "\n".join(parts)
 
 
-def points_per_wave(n, fp64, valu=False):
+def points_per_wave(n, fp64, valu=False, sparse=False):
     """Points per wavefront: 32 / N columns of one 32-column MFMA tile (FP32 forms, FP64 VALU),
-    16 / N for the FP64 matrix-core form (one point in two column tiles at N = 32)."""
+    16 / N for the FP64 matrix-core form (one point in two column tiles at N = 32), one for the
+    sparse form."""
+    if sparse:
+        return 1
     if fp64 and not valu:
         return 2 if n == 8 else 1
     return 32 // n
@@ -400,7 +494,7 @@ _SOURCES = {}
 
 def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps, warmup_steps,
         block_size=256, collect=False, options=(), fp64=False, split=None, valu=False,
-        prepare=False, num_trace=0, trace_stride=1, trace_offset=0):
+        prepare=False, num_trace=0, trace_stride=1, trace_offset=0, sparse=False):
     """Launch the Hilbert-space kernel; returns the same dict as ``device.run_sweep``
     (``valu``: the FP64 VALU form instead of the FP64 matrix-core form).  ``prepare``: build
     buffers and kernels but launch nothing; returns the launch plan of ``device.run_sweep``
@@ -416,16 +510,18 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
     uses_rho0 = rho0_values is not None
     if valu and n_mono(m):
         raise ValueError("the FP64 VALU form has no general jump operators")
-    block_size = block_for(m, fp64, block_size, valu, split)
+    if sparse and (split or valu):
+        raise ValueError("the sparse form has no split-precision or VALU variant")
+    block_size = block_for(m, fp64, block_size, valu, split, sparse)
     tune = block_size == "auto"
     if tune:
-        block_size = block_for(m, fp64, 256, valu, split)
+        block_size = block_for(m, fp64, 256, valu, split, sparse)
     trace = num_trace > 0
     skey = (m.key, mode, block_size, uses_rho0, tuple(options), bool(fp64), split, bool(valu),
-            trace)
+            trace, bool(sparse), bool(sparse and SPARSE_XOR))
     src = _SOURCES.get(skey)
     if src is None:
-        src = emit(m, offs, mode, block_size, fp64, split, valu, trace).replace(
+        src = emit(m, offs, mode, block_size, fp64, split, valu, trace, sparse).replace(
             "GQ_RHO0_FROM_ARGS", "1" if uses_rho0 else "0")
         _SOURCES[skey] = src
     options = codegen.HIPRTC_BASE_OPTIONS + tuple(options)
@@ -464,7 +560,7 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
     a.time_table = table.data_ptr()
     a.result, a.status = result.data_ptr(), status.data_ptr()
     stream = torch.cuda.current_stream(dev).cuda_stream
-    kname = codegen.kernel_name(kernel_base(split, fp64, valu), fp64)
+    kname = codegen.kernel_name(kernel_base(split, fp64, valu, sparse), fp64)
     tune_s, tune_times = 0.0, {}
     if tune and npts > 0 and num_steps > 0:
         from . import autotune
@@ -475,15 +571,16 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
                        256, a, stream)
         trial_steps = min(int(num_steps), 64)
         jmax = jump_width(m)
+        kw = ell_table(m)[0] if sparse else 0
         fits = tuple(b for b in autotune.CANDIDATES
-                     if lds_bytes(m.n, fp64, b, jmax, split, valu, n_mono(m))
+                     if lds_bytes(m.n, fp64, b, jmax, split, valu, n_mono(m), kw)
                      <= LDS_BYTES_PER_CU)
         spare = torch.zeros(1, dtype=torch.int32, device=dev)
 
         def trial(mod, block):
             t = runtime.SweepArgs.from_buffer_copy(a)
             t.num_steps, t.warmup_steps, t.num_trace, t.trace = trial_steps, 0, 0, None
-            per = (block // 64) * points_per_wave(m.n, fp64, valu)
+            per = (block // 64) * points_per_wave(m.n, fp64, valu, sparse)
             return _timed_launches(ctx, mod, kname, (npts + per - 1) // per, block, t, stream,
                                    spare)
 
@@ -495,7 +592,7 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
         module, _, _ = ctx.compile(autotune.with_block(src, block_size), options)
         ctx.read_status(status, stream, clear=True)
         tune_s = time.time() - t0
-    per_block = (block_size // 64) * points_per_wave(m.n, fp64, valu)
+    per_block = (block_size // 64) * points_per_wave(m.n, fp64, valu, sparse)
     if prepare:
         launches = []
         if npts > 0:

@@ -1,0 +1,359 @@
+// RK4 on the full density matrix for models whose H_eff is sparse (multi-qubit chains: a few
+// nonzeros per row), one wavefront per parameter point, on the vector ALUs.
+//
+// Same model structure and equation as hilbert_rk4.hip: H(t) = H0 + diag(v(ParX, ParY, t)),
+// H_eff = H0 - (i/2) sum L^+L, and per element (i, j)
+//     d rho_ij = c_ij rho_ij - i sum_k h_ik rho_kj + i sum_k conj(h_jk) rho_ik + jumps,
+//     c_ij = -i (h_ii - conj(h_jj)) - i (v_i - v_j) + sum_k d_k(i) d_k(j),
+// where the sums run over the off-diagonal nonzeros of H_eff: -i H_eff rho + i rho H_eff^+
+// without a product of dense matrices, 2 K complex multiply-adds per element instead of 2 N.
+// Jumps: single-entry operators l |a><b| add |l|^2 rho_bb to rho_aa; operators with one
+// nonzero per row add l_i conj(l_j) rho[col_i][col_j].  RK4 stage times, k1 pipelining and
+// output semantics are the reference's (gqis/cuda_solvers.py:470-517,
+// gqis/solver.py:2126-2193).
+//
+// Off-diagonal structure, two forms:
+//   GQ_XOR 1: every row i has its nonzeros at columns i ^ mask_k with values v_k that do not
+//     depend on i, and each general jump operator maps i to i ^ b where i & b == cond, with
+//     one weight (qubit operators sx_k, s-_k on a register of qubits): masks, values and
+//     weights are compile-time constants, the neighbours' addresses one XOR away;
+//   GQ_XOR 0: an ELL table of GQ_HK (column, value) entries per row, padded with zero values,
+//     and the jump operators' row tables, read from LDS.
+//
+// Layout: rho is Hermitian, so only the N (N + 1) / 2 elements i <= j are integrated: element u
+// of the upper triangle in row-major order is slot m = u / 64 of lane u % 64.  The wavefront
+// keeps the stage input in an LDS copy of the full rho (row-major, complex): the caller writes
+// each element and, below the diagonal, its conjugate; the RHS reads each element's neighbours
+// rho[k][j] and rho[i][k] from there (consecutive lanes: consecutive columns of one row).  The
+// static part of c_ij is a per-element LDS table, the drive part v_i - v_j comes from the drive
+// row table.
+
+#define HN GQ_HN
+#define HK GQ_HK                  // off-diagonal nonzeros per row (ELL width)
+#define HT (HN * (HN + 1) / 2)    // elements i <= j
+#define HE ((HT + 63) / 64)       // complex elements (slots) per lane
+#define HWAVES (GQ_BLOCK / 64)
+
+extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_sparse)(const gq_sweep_args a) {
+    __shared__ real2 s_rho[HWAVES][HN * HN];
+    __shared__ real2 s_cf[HN * HN];                 // static c_ij
+    __shared__ real s_vrow[GQ_NBASIS * HN];         // drive row coefficients v_i per basis term
+    __shared__ real s_jw[HN * GQ_JMAX];
+    __shared__ int s_jb[HN * GQ_JMAX];
+#if !GQ_XOR
+    __shared__ int s_ecol[HN * HK];
+    __shared__ real2 s_eval[HN * HK];
+#if GQ_NMONO
+    __shared__ int s_mcol[GQ_NMONO * HN];
+    __shared__ real2 s_ml[GQ_NMONO * HN];
+#endif
+#endif
+
+    const real* __restrict__ cst = (const real*)a.consts;   // packed operator constants
+    for (int e = threadIdx.x; e < HN * HN; e += GQ_BLOCK) {
+        const int i = e / HN, jj = e % HN;
+        s_cf[e] = real2{cst[GQ_OFF_HDI + i] + cst[GQ_OFF_HDI + jj] + cst[GQ_OFF_DEP + e],
+                        cst[GQ_OFF_HDR + jj] - cst[GQ_OFF_HDR + i]};
+    }
+    for (int e = threadIdx.x; e < GQ_NBASIS * HN; e += GQ_BLOCK) s_vrow[e] = cst[GQ_OFF_VCOEF + e];
+    for (int e = threadIdx.x; e < HN * GQ_JMAX; e += GQ_BLOCK) {
+        s_jw[e] = cst[GQ_OFF_JW + e];
+        s_jb[e] = (int)cst[GQ_OFF_JB + e];
+    }
+#if !GQ_XOR
+    for (int e = threadIdx.x; e < HN * HK; e += GQ_BLOCK) {
+        s_ecol[e] = (int)cst[GQ_OFF_ECOL + e];
+        s_eval[e] = real2{cst[GQ_OFF_ERE + e], cst[GQ_OFF_EIM + e]};
+    }
+#if GQ_NMONO
+    for (int e = threadIdx.x; e < GQ_NMONO * HN; e += GQ_BLOCK) {
+        s_mcol[e] = (int)cst[GQ_OFF_MCOL + e];
+        s_ml[e] = real2{cst[GQ_OFF_MRE + e], cst[GQ_OFF_MIM + e]};
+    }
+#endif
+#endif
+    __syncthreads();
+
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    // the lane's slots: element u = lane + 64 m of the upper triangle, (row, column) (ri, rj)
+    int ri[HE], rj[HE];
+    bool used[HE];
+#pragma unroll
+    for (int m = 0; m < HE; ++m) {
+        const int u = lane + 64 * m;
+        used[m] = u < HT;
+        int i = 0, start = 0;
+        while (i < HN - 1 && start + (HN - i) <= u) {
+            start += HN - i;
+            ++i;
+        }
+        ri[m] = used[m] ? i : 0;
+        rj[m] = used[m] ? i + (u - start) : 0;
+    }
+    const long long q = (long long)blockIdx.x * HWAVES + wave;
+    const bool valid = q < a.point_count;
+    const long long qc = valid ? q : 0;
+    const gq_point pt = gq_locate(a, qc);
+    const real px = ((const real*)a.par_x)[pt.ix];
+    const real py = ((const real*)a.par_y)[pt.iy];
+    real2* __restrict__ rho = s_rho[wave];
+
+    // ---- initial state: rho0 values (reduced layout) or |0><0| ------------------------------
+    real y[2 * HE];
+#pragma unroll
+    for (int m = 0; m < HE; ++m) {
+        const int i = ri[m], j = rj[m];
+        real re = R(0.0), im = R(0.0);
+#if GQ_RHO0_VALUES
+        const real* __restrict__ r0 = (const real*)a.rho0 + qc * (HN * HN - 1);
+        if (i == j) {
+            if (i < HN - 1) {
+                re = r0[i];
+            } else {
+                real s = R(0.0);
+                for (int k = 0; k < HN - 1; ++k) s += r0[k];
+                re = R(1.0) - s;
+            }
+        } else {
+            const int lo = i < j ? i : j, hi = i < j ? j : i;
+            const int u = lo * HN - (lo * (lo + 1)) / 2 + (hi - lo - 1);
+            re = r0[HN - 1 + 2 * u];
+            im = (i < j ? R(1.0) : R(-1.0)) * r0[HN - 1 + 2 * u + 1];
+        }
+#else
+        re = (i == 0 && j == 0) ? R(1.0) : R(0.0);
+#endif
+        y[m] = re;
+        y[m + HE] = im;
+    }
+
+    // ---- per-lane constants ------------------------------------------------------------------
+    // observable weight of each diagonal slot (zero elsewhere)
+    real od[HE];
+#pragma unroll
+    for (int m = 0; m < HE; ++m)
+        od[m] = (used[m] && ri[m] == rj[m]) ? cst[GQ_OFF_OBS + ri[m]] : R(0.0);
+
+    const real dt = (real)a.dt;
+    const real dt2 = dt * R(0.5);
+    const real dt6 = dt / R(6.0);
+    const int num_steps = a.num_steps;
+    real g[GQ_NBASIS];         // drive basis terms at the current stage time
+
+    auto stage_values = [&](int row) {
+        const real* __restrict__ tab = (const real*)a.time_table;
+        real TT[GQ_NATOMS_ALLOC];
+#pragma unroll
+        for (int k = 0; k < GQ_NATOMS; ++k) TT[k] = tab[row * GQ_NATOMS + k];
+        gq_basis(px, py, TT, cst, g);
+    };
+
+    // the stage input x + c k into the wavefront's LDS copy (and the conjugates below the
+    // diagonal)
+    auto put = [&](const real* __restrict__ x, real c, const real* __restrict__ kk) {
+#pragma unroll
+        for (int m = 0; m < HE; ++m) {
+            if (used[m]) {
+                const real vr = fma(c, kk[m], x[m]), vi = fma(c, kk[m + HE], x[m + HE]);
+                rho[ri[m] * HN + rj[m]] = real2{vr, vi};
+                if (ri[m] != rj[m]) rho[rj[m] * HN + ri[m]] = real2{vr, -vi};
+            }
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    };
+
+    // right-hand side at the state in the LDS copy
+    auto rhs = [&](real* __restrict__ kout) {
+#pragma unroll
+        for (int m = 0; m < HE; ++m) {
+            const int i = ri[m], j = rj[m];
+            const real2 x = rho[i * HN + j];
+            const real2 cf = s_cf[i * HN + j];
+            real dvi = R(0.0), dvj = R(0.0);
+#pragma unroll
+            for (int k = 0; k < GQ_NBASIS; ++k) {
+                dvi = fma(s_vrow[k * HN + i], g[k], dvi);
+                dvj = fma(s_vrow[k * HN + j], g[k], dvj);
+            }
+            const real ci = cf.y - (dvi - dvj);
+            real dr = fma(cf.x, x.x, -ci * x.y);
+            real di = fma(cf.x, x.y, ci * x.x);
+            real lr = R(0.0), li = R(0.0), ur = R(0.0), ui = R(0.0);
+#if GQ_XOR
+            // -i sum_k v_k rho[i ^ mask_k][j] and +i sum_k conj(v_k) rho[i][j ^ mask_k]
+#pragma unroll
+            for (int k = 0; k < HK; ++k) {
+                const real2 p = rho[(i ^ gq_xmask[k]) * HN + j];
+                lr = fma(gq_xre[k], p.x, fma(-gq_xim[k], p.y, lr));
+                li = fma(gq_xre[k], p.y, fma(gq_xim[k], p.x, li));
+                const real2 u = rho[i * HN + (j ^ gq_xmask[k])];
+                ur = fma(gq_xre[k], u.x, fma(gq_xim[k], u.y, ur));
+                ui = fma(gq_xre[k], u.y, fma(-gq_xim[k], u.x, ui));
+            }
+#else
+#pragma unroll
+            for (int k = 0; k < HK; ++k) {
+                const int c = s_ecol[i * HK + k];
+                const real2 hv = s_eval[i * HK + k];
+                const real2 p = rho[c * HN + j];
+                lr = fma(hv.x, p.x, fma(-hv.y, p.y, lr));
+                li = fma(hv.x, p.y, fma(hv.y, p.x, li));
+                const int cj = s_ecol[j * HK + k];
+                const real2 hj = s_eval[j * HK + k];
+                const real2 u = rho[i * HN + cj];
+                ur = fma(hj.x, u.x, fma(hj.y, u.y, ur));
+                ui = fma(hj.x, u.y, fma(-hj.y, u.x, ui));
+            }
+#endif
+            dr += li - ui;
+            di += ur - lr;
+#if GQ_NMONO
+#if GQ_XOR
+            // l_i conj(l_j) rho[i ^ b][j ^ b] where both i and j map (i & b == cond)
+#pragma unroll
+            for (int kk = 0; kk < GQ_NMONO; ++kk) {
+                if ((i & gq_mmask[kk]) == gq_mcond[kk] && (j & gq_mmask[kk]) == gq_mcond[kk]) {
+                    const real2 p = rho[(i ^ gq_mmask[kk]) * HN + (j ^ gq_mmask[kk])];
+                    dr = fma(gq_mw[kk], p.x, dr);
+                    di = fma(gq_mw[kk], p.y, di);
+                }
+            }
+#else
+#pragma unroll
+            for (int kk = 0; kk < GQ_NMONO; ++kk) {
+                const int ci2 = s_mcol[kk * HN + i], cj2 = s_mcol[kk * HN + j];
+                if (ci2 >= 0 && cj2 >= 0) {
+                    // w = l_i conj(l_j), times rho[col_i][col_j]
+                    const real2 l = s_ml[kk * HN + i], lj = s_ml[kk * HN + j];
+                    const real wr = fma(l.x, lj.x, l.y * lj.y);
+                    const real wi = fma(l.y, lj.x, -l.x * lj.y);
+                    const real2 p = rho[ci2 * HN + cj2];
+                    dr = fma(wr, p.x, fma(-wi, p.y, dr));
+                    di = fma(wr, p.y, fma(wi, p.x, di));
+                }
+            }
+#endif
+#endif
+            if (i == j) {
+                // single-entry jumps into the population i: sum_b |l|^2 rho_bb
+#pragma unroll
+                for (int e = 0; e < GQ_JMAX; ++e)
+                    dr = fma(s_jw[i * GQ_JMAX + e], rho[s_jb[i * GQ_JMAX + e] * (HN + 1)].x, dr);
+            }
+            kout[m] = dr;
+            kout[m + HE] = di;
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    };
+
+    // sum over the wavefront (every lane takes part; the result is in every lane)
+    auto wave_sum = [&](double part) -> double {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off);
+        return part;
+    };
+    bool bad_trace = false;
+    (void)bad_trace;
+
+    stage_values(0);
+    real k[2 * HE];
+    put(y, R(0.0), y);
+    rhs(k);
+    double avg = 0.0;
+    for (int step = 0; step < num_steps; ++step) {
+        stage_values(2 * step + 1);
+        real acc[2 * HE];
+#pragma unroll
+        for (int e = 0; e < 2 * HE; ++e) acc[e] = k[e];
+        put(y, dt2, k);
+        rhs(k);
+#pragma unroll
+        for (int e = 0; e < 2 * HE; ++e) acc[e] = fma(R(2.0), k[e], acc[e]);
+        put(y, dt2, k);
+        rhs(k);
+#pragma unroll
+        for (int e = 0; e < 2 * HE; ++e) acc[e] = fma(R(2.0), k[e], acc[e]);
+        put(y, dt, k);
+        stage_values(2 * step + 2);
+        rhs(k);
+#pragma unroll
+        for (int e = 0; e < 2 * HE; ++e) {
+            acc[e] += k[e];
+            y[e] = fma(dt6, acc[e], y[e]);
+        }
+        if (step + 1 < num_steps) {
+            put(y, R(0.0), y);
+            rhs(k);
+        }
+#if GQ_OUT_MEAN
+        if (step >= a.warmup_steps) {
+            real yd = R(0.0);
+#pragma unroll
+            for (int m = 0; m < HE; ++m) yd = fma(od[m], y[m], yd);
+            avg += (double)yd;
+        }
+#endif
+#if GQ_TRACE
+        // observable trace after steps offset + k * stride (gqis/solver.py:2177-2193)
+        if (step >= a.trace_offset && (step - a.trace_offset) % a.trace_stride == 0) {
+            const int kk = (step - a.trace_offset) / a.trace_stride;
+            if (kk < a.num_trace) {
+                real yd = R(0.0);
+#pragma unroll
+                for (int m = 0; m < HE; ++m) yd = fma(od[m], y[m], yd);
+                const double s = wave_sum((double)yd);
+                if (lane == 0 && valid) {
+                    const real r = (real)s;
+                    bad_trace |= !gq_finite(r);
+                    gq_store_complex(a.trace, (long long)kk * a.trace_point_stride + q, r, R(0.0));
+                }
+            }
+        }
+#endif
+    }
+
+    bool bad = false;
+#if GQ_OUT_MEAN || GQ_OUT_FINAL
+#if GQ_OUT_FINAL
+    {
+        real yd = R(0.0);
+#pragma unroll
+        for (int m = 0; m < HE; ++m) yd = fma(od[m], y[m], yd);
+        avg = (double)yd;
+    }
+#endif
+    double s = wave_sum(avg);
+    if (lane == 0) {
+#if GQ_OUT_MEAN
+        s /= (double)(num_steps - a.warmup_steps);
+#endif
+        if (valid) {
+            const real r = (real)s;
+            bad = !gq_finite(r);
+            gq_store_complex(a.result, q, r, R(0.0));
+        }
+    }
+#elif GQ_OUT_FINAL_RHO
+    if (valid) {
+#pragma unroll
+        for (int m = 0; m < HE; ++m) {
+            const int i = ri[m], j = rj[m];
+            if (!used[m]) continue;
+            if (i == j && i < HN - 1) {
+                ((real*)a.result)[(long long)i * a.point_count + q] = y[m];
+                bad |= !gq_finite(y[m]);
+            } else if (i < j) {
+                const int idx = HN - 1 + 2 * (i * HN - (i * (i + 1)) / 2 + (j - i - 1));
+                ((real*)a.result)[(long long)idx * a.point_count + q] = y[m];
+                ((real*)a.result)[(long long)(idx + 1) * a.point_count + q] = y[m + HE];
+                bad |= !(gq_finite(y[m]) && gq_finite(y[m + HE]));
+            }
+        }
+    }
+#endif
+    gq_flag(a.status, bad, GQ_STATUS_NONFINITE_RESULT);
+#if GQ_TRACE
+    gq_flag(a.status, bad_trace, GQ_STATUS_NONFINITE_TRACE);
+#endif
+}

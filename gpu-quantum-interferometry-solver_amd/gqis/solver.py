@@ -451,8 +451,9 @@ def _solve_symbolic_system(H, Drive, Col_Ops, mean_operator, tlist, var_arrays=N
     collect = bool(timings or return_timing_info)
     total_start = time.time()
 
-    if liouville not in ("auto", "lanes", "mfma", "mfma_f16", "mfma_bf16"):
-        raise ValueError("liouville must be 'auto', 'lanes', 'mfma', 'mfma_f16' or 'mfma_bf16'.")
+    if liouville not in ("auto", "lanes", "mfma", "mfma_f16", "mfma_bf16", "sparse"):
+        raise ValueError("liouville must be 'auto', 'lanes', 'mfma', 'mfma_f16', 'mfma_bf16' or "
+                         "'sparse'.")
     if liouville in ("mfma_f16", "mfma_bf16") and fp64:
         raise ValueError(f"liouville='{liouville}' (split-precision matrix-core products) is an "
                          "FP32-mode option.")
@@ -643,14 +644,16 @@ def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, run
     return its launch plan instead of running it)."""
     from . import hilbert
     n = H.shape[0] if hasattr(H, "shape") else 0
-    forced = liouville.startswith("mfma")
+    forced = liouville.startswith("mfma") or liouville == "sparse"
+    sparse = liouville == "sparse"
     split = {"mfma_f16": "f16", "mfma_bf16": "bf16"}.get(liouville)
     # measured on gfx950 (256^2, 2000 steps): for a dense H_eff (hilbert.dense_enough) the
     # matrix-core forms are fastest from 16 levels in both precisions and at 8 levels in FP64
     # (52 ms against 77 ms with 8 lanes per point); a sparse one (the 8-level chain, the
     # multi-qubit chain: 3-5 nonzeros per row) runs 3-6x faster on the per-point kernels below
     # 32 levels.  At 32 levels the per-point form needs 1023 symbolic equations and a very large
-    # kernel, so the matrix-core form is taken whenever the model fits it.
+    # kernel, so the Hilbert-space form is taken whenever the model fits it: the dense
+    # matrix-core product, or the sparse form for a sparse H_eff.
     wanted = forced or (liouville == "auto" and (n >= 16 or (fp64 and n >= 8)))
     reasons = []
     if n not in hilbert.SUPPORTED_N:
@@ -677,12 +680,18 @@ def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, run
                              "terms, constant diagonal or single-entry collapse operators and a "
                              "diagonal observable.")
         return None
-    if liouville == "auto" and n < 32 and not hilbert.dense_enough(hm):
-        return None          # a sparse H_eff runs faster on the per-point kernels
+    if liouville == "auto" and not hilbert.dense_enough(hm):
+        if n < 32:
+            return None      # a sparse H_eff runs faster on the per-point kernels
+        # at 32 levels the wavefront-per-point sparse form: the 5-qubit chain (5 nonzeros per
+        # row) takes 29.7 ns per point-step in FP32 against 31.2 with FP16 hi/lo products and
+        # 51 against 117 ns in FP64 (128^2, 200 RK4 steps)
+        sparse = True
     values = dict(model["defaults"])
     values.update(runtime_consts or {})
     rt = [float(values[s]) for s in model["runtime_const_syms"]]
-    if liouville == "auto" and not fp64 and np.max(np.abs(hm.h_eff)) <= SPLIT_F16_MAX_ENTRY:
+    if liouville == "auto" and not fp64 and np.max(np.abs(hm.h_eff)) <= SPLIT_F16_MAX_ENTRY \
+            and not sparse:
         # FP32 mode: FP16 hi/lo products reach the FP32 product's error at 1.4x its speed;
         # the constant operator must stay well inside the FP16 range
         split = "f16"
@@ -692,7 +701,7 @@ def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, run
                           warmup_steps=warmup_steps, block_size=block_size, collect=collect,
                           options=options, fp64=fp64, split=split, prepare=prepare,
                           num_trace=num_trace if trace else 0, trace_stride=trace_stride,
-                          trace_offset=trace_offset)
+                          trace_offset=trace_offset, sparse=sparse)
     if prepare:
         outputs["runtime_defaults"] = dict(model["defaults"])
     stage_s = {"ode_prepare_s": prep, "ode_optimize_s": 0.0, "emit_s": 0.0}

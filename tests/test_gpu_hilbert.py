@@ -216,3 +216,10 @@ def test_automatic_choice_follows_the_density_of_h(gpu):
         kernels.append(info["kernel"])
     assert kernels[0] == "gq_hilbert_rk4_f16x3_f32"
     assert not kernels[1].startswith("gq_hilbert")
+    # 32 levels: the sparse 5-qubit chain takes the wavefront-per-point sparse form
+    five = workloads.multi_qubit(5, n_eps=2, n_amp=2, n_t=11, t_end=0.1)
+    for fp64 in (False, True):
+        _, info = mesolve_2D(five["H"], five["Drive"], five["Col_Ops"], five["mean_operator"],
+                             five["tlist"], var_arrays=five["var_arrays"], fp64=fp64,
+                             return_timing_info=True)
+        assert info["kernel"] == "gq_hilbert_sparse_" + ("f64" if fp64 else "f32")

@@ -1,0 +1,93 @@
+"""The wavefront-per-point sparse form (liouville="sparse", kernels/hilbert_sparse.hip) on the
+reference's multi-qubit chain (Benchmark 04): FP64 against the FP64 oracle to 1e-10 of the
+map's scale, FP32 to 2e-5, every output mode, a time trace, initial states, ragged grids."""
+import numpy as np
+import pytest
+
+from gqis import mesolve_2D, workloads
+from oracle import integrate, models
+
+pytestmark = pytest.mark.gpu
+
+
+def _model(nq, steps, n_eps=2, n_amp=2):
+    w = workloads.multi_qubit(nq, n_eps=n_eps, n_amp=n_amp, n_t=steps + 1, t_end=0.01 * steps)
+    eps_v, amp_v = w["var_arrays"].values()
+    return w, models.multi_qubit(nq, eps_v, amp_v)
+
+
+def _args(w):
+    return (w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], w["tlist"])
+
+
+@pytest.mark.parametrize("nq,steps", [(3, 200), (4, 100), (5, 40)])
+def test_sparse_form_matches_the_oracle(gpu, nq, steps):
+    n = 2 ** nq
+    w, m = _model(nq, steps)
+    print(f"sparse form: {nq} qubits, {steps} steps", flush=True)
+    ref, _ = integrate.rk4(m.f, m.ground_state(), 0.01, steps, obs=m.obs)
+    fin, _ = integrate.rk4(m.f, m.ground_state(), 0.01, steps, mode="final_rho")
+    scale = max(1.0, float(np.max(np.abs(ref))))
+    for fp64, tol in ((True, 1e-10), (False, 2e-5)):
+        got, info = mesolve_2D(*_args(w), var_arrays=w["var_arrays"], liouville="sparse",
+                               fp64=fp64, return_timing_info=True)
+        assert info["kernel"].startswith("gq_hilbert_sparse")
+        err = float(np.max(np.abs(got.ravel() - ref)))
+        print(f"  fp64={fp64}: mean max abs error {err:.2e}", flush=True)
+        assert err <= tol * scale
+        rho = mesolve_2D(*_args(w), var_arrays=w["var_arrays"], liouville="sparse", fp64=fp64,
+                         output_mode="final_rho")
+        assert rho.shape == (2, 2, n * n - 1)
+        assert np.max(np.abs(rho.reshape(4, -1) - fin)) <= tol
+        last = mesolve_2D(*_args(w), var_arrays=w["var_arrays"], liouville="sparse", fp64=fp64,
+                          output_mode="final")
+        ref_last, _ = integrate.rk4(m.f, m.ground_state(), 0.01, steps, obs=m.obs, mode="final")
+        assert np.max(np.abs(last.ravel() - ref_last)) <= tol * scale
+
+
+def test_sparse_form_trace_warmup_and_initial_states(gpu):
+    # 4 qubits on a ragged 3 x 5 grid: observable trace every 7th step, warmup, and a run
+    # continued from the final states of a first one equals one run over both intervals
+    nq, steps = 4, 60
+    w, m = _model(nq, steps, n_eps=3, n_amp=5)
+    res, tr, times = mesolve_2D(*_args(w), var_arrays=w["var_arrays"], liouville="sparse",
+                                fp64=True, return_time_trace=True, time_trace_every=7,
+                                warmup_time=0.25)
+    ref, ref_tr = integrate.rk4(m.f, m.ground_state(), 0.01, steps, obs=m.obs, warmup_steps=15,
+                                trace_stride=7, n_trace=9)
+    assert res.shape == (3, 5) and tr.shape == (3, 5, 9)
+    assert np.max(np.abs(res.ravel() - ref)) <= 1e-10
+    assert np.max(np.abs(tr.reshape(15, 9) - ref_tr)) <= 1e-10
+    half = workloads.multi_qubit(nq, n_eps=3, n_amp=5, n_t=31, t_end=0.3)
+    mid = mesolve_2D(*_args(half), var_arrays=half["var_arrays"], liouville="sparse", fp64=True,
+                     output_mode="final_rho")
+    end = mesolve_2D(*_args(half), var_arrays=half["var_arrays"], liouville="sparse", fp64=True,
+                     output_mode="final_rho", rho0_values=mid, t_in=0.3)
+    whole = mesolve_2D(*_args(w), var_arrays=w["var_arrays"], liouville="sparse", fp64=True,
+                       output_mode="final_rho")
+    assert np.max(np.abs(end - whole)) <= 1e-12
+
+
+def test_sparse_and_matrix_core_forms_agree(gpu):
+    # 5 qubits, FP64: the ELL form and the dense FP64 matrix-core form of the same sweep
+    w, _ = _model(5, 30, n_eps=3, n_amp=2)
+    a = mesolve_2D(*_args(w), var_arrays=w["var_arrays"], liouville="sparse", fp64=True)
+    b = mesolve_2D(*_args(w), var_arrays=w["var_arrays"], liouville="mfma", fp64=True)
+    assert np.max(np.abs(a - b)) <= 1e-12
+
+
+def test_sparse_table_form_and_a_dense_model(gpu, monkeypatch):
+    # the ELL / jump-table form (no XOR structure): the multi-qubit chain with the XOR form
+    # switched off, and a random dense 8-level model with single-entry jumps, against the oracle
+    from gqis import hilbert
+    monkeypatch.setattr(hilbert, "SPARSE_XOR", False)
+    w, m = _model(4, 50)
+    ref, _ = integrate.rk4(m.f, m.ground_state(), 0.01, 50, obs=m.obs)
+    got = mesolve_2D(*_args(w), var_arrays=w["var_arrays"], liouville="sparse", fp64=True)
+    assert np.max(np.abs(got.ravel() - ref)) <= 1e-10
+    w = workloads.random_n_level(8, seed=2024, n_eps=3, n_amp=2, n_t=101, t_end=1.0)
+    eps_v, amp_v = w["var_arrays"].values()
+    rm = models.random_n_level(8, 2024, eps_v, amp_v)
+    ref, _ = integrate.rk4(rm.f, rm.ground_state(), 0.01, 100, obs=rm.obs)
+    got = mesolve_2D(*_args(w), var_arrays=w["var_arrays"], liouville="sparse", fp64=True)
+    assert np.max(np.abs(got.ravel() - ref)) <= 1e-10 * max(1.0, float(np.max(np.abs(ref))))
