@@ -29,6 +29,8 @@ typedef double real;
 #define GQ_FP64 1
 inline real gq_sign(real x) { return (x > R(0)) - (x < R(0)); }
 inline double gq_reduce_2pi(double x) { return x; }
+inline double gq_cos64(double x) { return std::cos(x); }
+inline double gq_sin64(double x) { return std::sin(x); }
 namespace lane {
 %(lane)s
 }
