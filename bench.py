@@ -292,7 +292,7 @@ def main():
     ap.add_argument("--requests", type=int, default=1000, help="serving: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="serving: Poisson rate (req/s)")
     ap.add_argument("--workload", choices=["four_level", "lzs", "duffing", "serving", "scaling",
-                                           "resonator"],
+                                           "resonator", "multiqubit"],
                     default="four_level",
                     help="four_level: the headline sweep; lzs: the 64x64 two-level map with the "
                          "CPU oracle beside it; duffing: adaptive Tsit5/DOP853 sweep; serving: "
@@ -301,6 +301,8 @@ def main():
     ap.add_argument("--photons", type=int, nargs="*", default=[2, 3, 4, 5, 6, 7, 8, 9, 10],
                     help="resonator: Fock states of the cavity (2 to 10)")
     ap.add_argument("--no-adaptive", action="store_true", help="resonator: RK4 rows only")
+    ap.add_argument("--qubits", type=int, nargs="*", default=[3, 4, 5],
+                    help="multiqubit: chain lengths (3 to 5)")
     ap.add_argument("--levels", type=int, nargs="*", default=None,
                     help="scaling: only these N (4, 8, 16, 32)")
     ap.add_argument("--forms", default="",
@@ -327,6 +329,8 @@ def main():
         return bench_lzs(args)
     if args.workload == "resonator":
         return bench_resonator(args)
+    if args.workload == "multiqubit":
+        return bench_multiqubit(args)
 
     w = workloads.four_level(n_eps=args.grid, n_amp=args.grid)
     npts = args.grid * args.grid
@@ -773,6 +777,111 @@ def bench_resonator(args):
                       "data": "synthetic", "vs_baseline": None,
                       "config": {"workload": "qubit-resonator (reference Benchmark 02 / 04), "
                                              "wd500 regime, 40 drive periods, mean of a"},
+                      "results": rows}), flush=True)
+
+
+# The reference's published multi-qubit chain rows (RTX 3080, FP32, 1024 x 1024 grid, 40 drive
+# periods, calc_s = GPU time of the full-grid solve): Benchmarks/results/
+# Benchmark_04_hilbert_space_scaling_calibrated.csv:94-96 (dop853), 99-101 (lserk4), 104-106
+# (lserk4f), 109-111 (rk4; the 5-qubit run stopped at the 900 s limit), 114-116 (tsit5),
+# 119-121 (tsit5a at accuracy 3.16228e-5: seconds and mean accepted steps per point).
+MULTI_QUBIT_STEPS = {"rk4": 10240, "tsit5": 4305, "dop853": 1810, "lserk4": 7241, "lserk4f": 7241}
+MULTI_QUBIT_PUBLISHED_S = {
+    "dop853": {3: 17.4067402, 4: 296.216663, 5: 1657.27025},
+    "lserk4": {3: 2.20887549, 4: 255.716538, 5: 2544.99575},
+    "lserk4f": {3: 2.00129834, 4: 209.329688, 5: 2290.759},
+    "rk4": {3: 2.17273145, 4: 390.043406, 5: None},
+    "tsit5": {3: 3.36533301, 4: 357.227966, 5: 1907.40088},
+    "tsit5a": {3: (8.92895996, 1269.62088), 4: (146.657688, 1251.5546),
+               5: (880.714938, 1339.77004)}}
+
+
+def bench_multiqubit(args):
+    """The reference's multi-qubit chain rows of Benchmark 04 (n qubits, Hilbert dimension 2^n,
+    sx gaps, ZZ neighbours, drive on qubit 0, relaxation and dephasing on every qubit) on the
+    published grid (1024 x 1024), 40 drive periods, published step counts and FP32, beside the
+    published RTX 3080 kernel times.  Up to 4 qubits every solver runs on the per-point
+    kernels; at 5 qubits (1023 reduced states) RK4 runs on the sparse Hilbert-space form, in 8
+    row blocks of the grid (kernel times summed; a block is 128 x 1024 points)."""
+    import numpy as np
+    import torch
+    from gqis import mesolve_2D, workloads
+    side = 1024 if args.grid == 2048 else args.grid
+    w_drive = 1.14
+    duration = 40 * 2.0 * np.pi / w_drive
+    rows = {}
+
+    def call(w, tl, var_arrays, **kw):
+        torch.cuda.synchronize()
+        out, info = mesolve_2D(w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"], tl,
+                               var_arrays=var_arrays, return_timing_info=True,
+                               ignore_non_finite_output=True, return_device=True, **kw)
+        torch.cuda.synchronize()
+        return out, info
+
+    for nq in args.qubits:
+        w = workloads.multi_qubit(nq, n_eps=side, n_amp=side)
+        va = {k: v.astype(np.float32) for k, v in w["var_arrays"].items()}
+        eps_s, amp_s = list(va)
+        for solver in ("rk4", "tsit5", "dop853", "lserk4", "lserk4f", "tsit5a"):
+            label = f"{nq} qubits {solver}"
+            pub = MULTI_QUBIT_PUBLISHED_S[solver][nq]
+            if nq >= 5 and solver != "rk4":
+                rows[label] = {"skipped": "no kernel for a 1023-state non-RK4 sweep: the per-point "
+                                          "kernels' LDS copies do not fit, the Hilbert-space "
+                                          "forms are RK4", "published_rtx3080_s": pub}
+                continue
+            kw = {}
+            if solver == "tsit5a":
+                tl = np.linspace(0.0, duration, 161)
+                kw = {"solver": "tsit5a",
+                      "adaptive_options": {"step_rtol": 3.16228e-5, "step_atol": 3.16228e-5,
+                                           "max_substeps": 128, "count_steps": True}}
+            else:
+                tl = np.linspace(0.0, duration, MULTI_QUBIT_STEPS[solver] + 1)
+                kw = {"solver": solver}
+            log(f"multi-qubit {nq} qubits ({4 ** nq - 1} states), {solver}, {side}x{side} ...")
+            small = {eps_s: va[eps_s][:2], amp_s: va[amp_s][:2]}
+            t0 = time.perf_counter()
+            call(w, tl[:3], small, **kw)                     # symbolic model and kernel build
+            prep = time.perf_counter() - t0
+            blocks = 8 if nq >= 5 else 1
+            kernel_s, kernels, finite, acc_sum, rej_sum = 0.0, set(), True, 0, 0
+            step = side // blocks
+            for b in range(blocks):
+                part = {eps_s: va[eps_s][b * step:(b + 1) * step], amp_s: va[amp_s]}
+                out, info = call(w, tl, part, **kw)
+                kernel_s += info["gpu_kernel_s"]
+                kernels.add(info.get("kernel"))
+                finite &= bool(torch.isfinite(out).all().item())
+                if solver == "tsit5a":
+                    acc_sum += int(info["adaptive_steps"]["accepted"].sum())
+                    rej_sum += int(info["adaptive_steps"]["rejected"].sum())
+                if blocks > 1:
+                    log(f"  block {b + 1}/{blocks}: {info['gpu_kernel_s']:.2f} s")
+            row = {"states": 4 ** nq - 1, "kernel_s": kernel_s, "kernel": sorted(kernels),
+                   "lanes_per_point": info.get("lanes_per_point"), "prepare_and_build_s": prep,
+                   "finite": finite, "row_blocks": blocks}
+            if solver == "tsit5a":
+                row.update({"mean_accepted": acc_sum / side / side,
+                            "rejected_frac": rej_sum / max(acc_sum, 1),
+                            "published_rtx3080_s": pub[0],
+                            "published_mean_accepted": pub[1], "speedup": pub[0] / kernel_s})
+            else:
+                row.update({"steps": MULTI_QUBIT_STEPS[solver],
+                            "point_steps_per_s": side * side * MULTI_QUBIT_STEPS[solver] / kernel_s,
+                            "published_rtx3080_s": pub,
+                            "speedup": (pub / kernel_s) if pub else None})
+                if pub is None:
+                    row["published_note"] = "the reference's run stopped at its 900 s limit"
+            rows[label] = row
+            log(json.dumps({label: row}))
+    print(json.dumps({"metric": "kernel time on the reference's multi-qubit chain benchmarks",
+                      "unit": "s", "higher_is_better": False, "n_gpus": 1, "dtype": "f32",
+                      "data": "synthetic", "vs_baseline": None,
+                      "config": {"workload": f"multi-qubit chain (reference Benchmark 04), "
+                                             f"{side}x{side}, 40 drive periods, mean of qubit 0 "
+                                             f"excited"},
                       "results": rows}), flush=True)
 
 
