@@ -407,11 +407,14 @@ def block_for(m, fp64, block_size=256, valu=False, split=None, sparse=False):
 
 
 def emit(m, offs, mode, block_size=256, fp64=False, split=None, valu=False, trace=False,
-         sparse=False):
+         sparse=False, solver="rk4"):
     """hiprtc source of the Hilbert-space RK4 kernel for model m (``split``: the commutator
     GEMM as three FP16 or BF16 products of hi/lo parts with FP32 accumulation; FP64 runs on the
     FP64 matrix cores, or with ``valu`` as VALU dot products; ``sparse``: the wavefront-per-
-    point kernel over an ELL table of H_eff, any precision)."""
+    point kernel over an ELL table of H_eff, any precision, which also integrates with the
+    other fixed-step methods, ``solver``)."""
+    if solver != "rk4" and not sparse:
+        raise ValueError("the matrix-core forms integrate with RK4 only")
     split_kind = SPLIT_KINDS[split]
     atoms = {}
     names = {"ParX": "px", "ParY": "py"}
@@ -472,8 +475,14 @@ def emit(m, offs, mode, block_size=256, fp64=False, split=None, valu=False, trac
                "GQ_MFMA_SPLIT": split_kind}
     template = ("hilbert_sparse.hip" if sparse else
                 "hilbert_rk4_dmfma.hip" if fp64 and not valu else "hilbert_rk4.hip")
+    table = "rk4_table.hip"
+    if solver != "rk4":
+        from . import tableaux
+        model_src += "\n" + tableaux.emit_fixed(solver)
+        defines["GQ_TIME_TABLE"] = 1
+        table = "tableau_table.hip"
     parts = [header, codegen.prelude(fp64, True, block_size, defines), model_src,
-             codegen.read_kernel_template("common.hip"), codegen.read_kernel_template("rk4_table.hip"),
+             codegen.read_kernel_template("common.hip"), codegen.read_kernel_template(table),
              codegen.read_kernel_template(template)]
     return "\n".join(parts)
 
@@ -490,13 +499,15 @@ def points_per_wave(n, fp64, valu=False, sparse=False):
 
 
 _SOURCES = {}
+SPARSE_UNROLL_OPTIONS = ("-mllvm", "-pragma-unroll-threshold=200000")
 
 
 def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps, warmup_steps,
         block_size=256, collect=False, options=(), fp64=False, split=None, valu=False,
-        prepare=False, num_trace=0, trace_stride=1, trace_offset=0, sparse=False):
+        prepare=False, num_trace=0, trace_stride=1, trace_offset=0, sparse=False, spec=None):
     """Launch the Hilbert-space kernel; returns the same dict as ``device.run_sweep``
-    (``valu``: the FP64 VALU form instead of the FP64 matrix-core form).  ``prepare``: build
+    (``valu``: the FP64 VALU form instead of the FP64 matrix-core form; ``spec``: the fixed-step
+    solver, RK4 when None, other methods on the sparse form only).  ``prepare``: build
     buffers and kernels but launch nothing; returns the launch plan of ``device.run_sweep``
     (for ``prepare_sweep``), whose ``set_consts`` rewrites the runtime constants in place."""
     import torch
@@ -512,19 +523,24 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
         raise ValueError("the FP64 VALU form has no general jump operators")
     if sparse and (split or valu):
         raise ValueError("the sparse form has no split-precision or VALU variant")
+    solver = spec.name if spec is not None else "rk4"
     block_size = block_for(m, fp64, block_size, valu, split, sparse)
     tune = block_size == "auto"
     if tune:
         block_size = block_for(m, fp64, 256, valu, split, sparse)
     trace = num_trace > 0
     skey = (m.key, mode, block_size, uses_rho0, tuple(options), bool(fp64), split, bool(valu),
-            trace, bool(sparse), bool(sparse and SPARSE_XOR))
+            trace, bool(sparse), bool(sparse and SPARSE_XOR), solver)
     src = _SOURCES.get(skey)
     if src is None:
-        src = emit(m, offs, mode, block_size, fp64, split, valu, trace, sparse).replace(
+        src = emit(m, offs, mode, block_size, fp64, split, valu, trace, sparse, solver).replace(
             "GQ_RHO0_FROM_ARGS", "1" if uses_rho0 else "0")
         _SOURCES[skey] = src
     options = codegen.HIPRTC_BASE_OPTIONS + tuple(options)
+    if solver != "rk4":
+        # unroll every stage of the step, whatever its size, so the stage vectors stay in
+        # registers (DOP853 at 32 levels: 8.3 against 20 ns per RHS and point in FP64)
+        options += SPARSE_UNROLL_OPTIONS
     module, cached, build_s = ctx.compile(src, options)
     rdt = torch.float64 if fp64 else torch.float32
     axes = list(var_arrays.values())
@@ -544,7 +560,10 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
     trace_buf = (torch.zeros((num_trace, max(npts, 1)),
                              dtype=torch.complex128 if fp64 else torch.complex64, device=dev)
                  if trace else None)
-    rows = 2 * num_steps + 1
+    from . import tableaux
+    rows = tableaux.rows_per_step(solver) * num_steps + 1
+    table_kernel = codegen.kernel_name("gq_rk4_time_table" if solver == "rk4"
+                                       else "gq_fixed_time_table", fp64)
     n_atoms = int(src.split("#define GQ_NATOMS ")[1].split()[0])
     table = torch.empty(max(1, rows * max(1, n_atoms)), dtype=rdt, device=dev)
     a = runtime.SweepArgs()
@@ -559,6 +578,8 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
     a.rho0 = rho0.data_ptr() if rho0 is not None else None
     a.time_table = table.data_ptr()
     a.result, a.status = result.data_ptr(), status.data_ptr()
+    for i, v in enumerate(spec.parameters if spec is not None else ()):
+        a.solver_param[i] = float(v)
     stream = torch.cuda.current_stream(dev).cuda_stream
     kname = codegen.kernel_name(kernel_base(split, fp64, valu, sparse), fp64)
     tune_s, tune_times = 0.0, {}
@@ -567,7 +588,7 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
         from .device import _timed_launches
         t0 = time.time()
         if n_atoms > 0:
-            ctx.launch(module, codegen.kernel_name("gq_rk4_time_table", fp64), (rows + 255) // 256,
+            ctx.launch(module, table_kernel, (rows + 255) // 256,
                        256, a, stream)
         trial_steps = min(int(num_steps), 64)
         jmax = jump_width(m)
@@ -597,7 +618,7 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
         launches = []
         if npts > 0:
             if n_atoms > 0:
-                launches.append((codegen.kernel_name("gq_rk4_time_table", fp64),
+                launches.append((table_kernel,
                                  (rows + 255) // 256, 256))
             launches.append((kname, (npts + per_block - 1) // per_block, block_size))
         n_rc = len(m.runtime_syms)
@@ -623,7 +644,7 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
         if timer:
             timer.record_start()
         if n_atoms > 0:
-            ctx.launch(module, codegen.kernel_name("gq_rk4_time_table", fp64), (rows + 255) // 256,
+            ctx.launch(module, table_kernel, (rows + 255) // 256,
                        256, a, stream)
         ctx.launch(module, kname, (npts + per_block - 1) // per_block, block_size, a, stream)
         if timer:

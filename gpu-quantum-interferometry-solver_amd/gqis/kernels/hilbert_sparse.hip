@@ -1,5 +1,6 @@
-// RK4 on the full density matrix for models whose H_eff is sparse (multi-qubit chains: a few
-// nonzeros per row), one wavefront per parameter point, on the vector ALUs.
+// Fixed-step integration of the full density matrix for models whose H_eff is sparse
+// (multi-qubit chains: a few nonzeros per row), one wavefront per parameter point, on the
+// vector ALUs.
 //
 // Same model structure and equation as hilbert_rk4.hip: H(t) = H0 + diag(v(ParX, ParY, t)),
 // H_eff = H0 - (i/2) sum L^+L, and per element (i, j)
@@ -8,9 +9,10 @@
 // where the sums run over the off-diagonal nonzeros of H_eff: -i H_eff rho + i rho H_eff^+
 // without a product of dense matrices, 2 K complex multiply-adds per element instead of 2 N.
 // Jumps: single-entry operators l |a><b| add |l|^2 rho_bb to rho_aa; operators with one
-// nonzero per row add l_i conj(l_j) rho[col_i][col_j].  RK4 stage times, k1 pipelining and
-// output semantics are the reference's (gqis/cuda_solvers.py:470-517,
-// gqis/solver.py:2126-2193).
+// nonzero per row add l_i conj(l_j) rho[col_i][col_j].  Methods: classical RK4 (GQ_SCHEME
+// undefined; stage times, k1 pipelining: gqis/cuda_solvers.py:470-517) or, with GQ_SCHEME and
+// the tableau constants of fixed_tableau.hip, an explicit Butcher tableau (0), the 2N-storage
+// LSRK4 (1) or AB5 (2).  Output semantics are the reference's (gqis/solver.py:2126-2193).
 //
 // Off-diagonal structure, two forms:
 //   GQ_XOR 1: every row i has its nonzeros at columns i ^ mask_k with values v_k that do not
@@ -33,6 +35,9 @@
 #define HT (HN * (HN + 1) / 2)    // elements i <= j
 #define HE ((HT + 63) / 64)       // complex elements (slots) per lane
 #define HWAVES (GQ_BLOCK / 64)
+#if GQ_XOR
+static_assert((HN & (HN - 1)) == 0, "the XOR form addresses a register of qubits");
+#endif
 
 extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_sparse)(const gq_sweep_args a) {
     __shared__ real2 s_rho[HWAVES][HN * HN];
@@ -149,17 +154,18 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_spar
         gq_basis(px, py, TT, cst, g);
     };
 
-    // the stage input x + c k into the wavefront's LDS copy (and the conjugates below the
+    // element m of the stage input into the wavefront's LDS copy (and its conjugate below the
     // diagonal)
+    auto store = [&](int m, real vr, real vi) {
+        if (used[m]) {
+            rho[ri[m] * HN + rj[m]] = real2{vr, vi};
+            if (ri[m] != rj[m]) rho[rj[m] * HN + ri[m]] = real2{vr, -vi};
+        }
+    };
+    // the stage input x + c k
     auto put = [&](const real* __restrict__ x, real c, const real* __restrict__ kk) {
 #pragma unroll
-        for (int m = 0; m < HE; ++m) {
-            if (used[m]) {
-                const real vr = fma(c, kk[m], x[m]), vi = fma(c, kk[m + HE], x[m + HE]);
-                rho[ri[m] * HN + rj[m]] = real2{vr, vi};
-                if (ri[m] != rj[m]) rho[rj[m] * HN + ri[m]] = real2{vr, -vi};
-            }
-        }
+        for (int m = 0; m < HE; ++m) store(m, fma(c, kk[m], x[m]), fma(c, kk[m + HE], x[m + HE]));
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     };
 
@@ -167,9 +173,14 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_spar
     auto rhs = [&](real* __restrict__ kout) {
 #pragma unroll
         for (int m = 0; m < HE; ++m) {
-            const int i = ri[m], j = rj[m];
-            const real2 x = rho[i * HN + j];
-            const real2 cf = s_cf[i * HN + j];
+            // the element's address, opaque to the compiler: every address below derives from
+            // it with an XOR or a multiply-add per call instead of living in a register
+            // across the step loop (which took about 200 registers at 32 levels)
+            int b = ri[m] * HN + rj[m];
+            asm volatile("" : "+v"(b));
+            const int i = b / HN, j = b - (b / HN) * HN;
+            const real2 x = rho[b];
+            const real2 cf = s_cf[b];
             real dvi = R(0.0), dvj = R(0.0);
 #pragma unroll
             for (int k = 0; k < GQ_NBASIS; ++k) {
@@ -184,10 +195,12 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_spar
             // -i sum_k v_k rho[i ^ mask_k][j] and +i sum_k conj(v_k) rho[i][j ^ mask_k]
 #pragma unroll
             for (int k = 0; k < HK; ++k) {
-                const real2 p = rho[(i ^ gq_xmask[k]) * HN + j];
+                // HN is a power of two: (i ^ mask) HN + j = b ^ (mask HN), i HN + (j ^ mask) =
+                // b ^ mask
+                const real2 p = rho[b ^ (gq_xmask[k] * HN)];
                 lr = fma(gq_xre[k], p.x, fma(-gq_xim[k], p.y, lr));
                 li = fma(gq_xre[k], p.y, fma(gq_xim[k], p.x, li));
-                const real2 u = rho[i * HN + (j ^ gq_xmask[k])];
+                const real2 u = rho[b ^ gq_xmask[k]];
                 ur = fma(gq_xre[k], u.x, fma(gq_xim[k], u.y, ur));
                 ui = fma(gq_xre[k], u.y, fma(-gq_xim[k], u.x, ui));
             }
@@ -214,7 +227,7 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_spar
 #pragma unroll
             for (int kk = 0; kk < GQ_NMONO; ++kk) {
                 if ((i & gq_mmask[kk]) == gq_mcond[kk] && (j & gq_mmask[kk]) == gq_mcond[kk]) {
-                    const real2 p = rho[(i ^ gq_mmask[kk]) * HN + (j ^ gq_mmask[kk])];
+                    const real2 p = rho[b ^ (gq_mmask[kk] * HN + gq_mmask[kk])];
                     dr = fma(gq_mw[kk], p.x, dr);
                     di = fma(gq_mw[kk], p.y, di);
                 }
@@ -257,10 +270,12 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_spar
     (void)bad_trace;
 
     stage_values(0);
+    double avg = 0.0;
+#ifndef GQ_SCHEME
+    // classical RK4 (table rows 2 step + 1 and 2 step + 2: midpoint and end of the step)
     real k[2 * HE];
     put(y, R(0.0), y);
     rhs(k);
-    double avg = 0.0;
     for (int step = 0; step < num_steps; ++step) {
         stage_values(2 * step + 1);
         real acc[2 * HE];
@@ -286,6 +301,121 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_spar
             put(y, R(0.0), y);
             rhs(k);
         }
+#else
+    // the other fixed-step methods (gqis/cuda_solvers.py:48-369, 520-702): GQ_NROWS table rows
+    // per step after row 0 (t = 0); the stage arithmetic of fixed_tableau.hip per element
+    (void)dt2;
+    (void)dt6;
+    auto row_at = [&](int step, int row) { return row < 0 ? 0 : 1 + step * GQ_NROWS + row; };
+#if GQ_SCHEME == 0
+    // explicit Runge-Kutta from a Butcher tableau, k1 of a step from the end of the previous one
+    const double p0 = a.solver_param[0];
+    real k[GQ_NSTAGES][2 * HE];
+    put(y, R(0.0), y);
+    rhs(k[0]);
+    for (int step = 0; step < num_steps; ++step) {
+#pragma unroll
+        for (int s = 1; s < GQ_NSTAGES; ++s) {
+#pragma unroll
+            for (int m = 0; m < HE; ++m) {
+                real ar = R(0.0), ai = R(0.0);
+#pragma unroll
+                for (int j = 0; j < s; ++j) {
+                    if (GQ_A[s][j] != 0.0 || GQ_A1[s][j] != 0.0) {
+                        const real c = (real)(GQ_A[s][j] + GQ_A1[s][j] * p0);
+                        ar = fma(c, k[j][m], ar);
+                        ai = fma(c, k[j][m + HE], ai);
+                    }
+                }
+                store(m, fma(dt, ar, y[m]), fma(dt, ai, y[m + HE]));
+            }
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            if (GQ_ROW_OF[s] != GQ_ROW_OF[s - 1]) stage_values(row_at(step, GQ_ROW_OF[s]));
+            rhs(k[s]);
+        }
+#pragma unroll
+        for (int e = 0; e < 2 * HE; ++e) {
+            real yn = y[e];
+#pragma unroll
+            for (int s = 0; s < GQ_NSTAGES; ++s)
+                if (GQ_B[s] != 0.0) yn = fma(dt * (real)GQ_B[s], k[s][e], yn);
+            y[e] = yn;
+        }
+        if (GQ_ROW_OF[GQ_NSTAGES - 1] != GQ_END_ROW) stage_values(row_at(step, GQ_END_ROW));
+        if (step + 1 < num_steps) {
+            put(y, R(0.0), y);
+            rhs(k[0]);
+        }
+#elif GQ_SCHEME == 1
+    // Carpenter-Kennedy 2N-storage LSRK4
+    real r[2 * HE], k[2 * HE];
+    for (int step = 0; step < num_steps; ++step) {
+#pragma unroll
+        for (int s = 0; s < GQ_NSTAGES; ++s) {
+            if (s > 0) stage_values(row_at(step, GQ_ROW_OF[s]));
+            put(y, R(0.0), y);
+            rhs(k);
+#pragma unroll
+            for (int e = 0; e < 2 * HE; ++e) {
+                r[e] = (s == 0) ? dt * k[e] : fma((real)GQ_A[s][0], r[e], dt * k[e]);
+                y[e] = fma((real)GQ_B[s], r[e], y[e]);
+            }
+        }
+        stage_values(row_at(step, GQ_END_ROW));
+#else
+    // Adams-Bashforth 5 after four classical RK4 steps (rows 0 and 1: midpoint and end)
+    real h0[2 * HE], h1[2 * HE], h2[2 * HE], h3[2 * HE], h4[2 * HE];   // f_n ... f_{n-4}
+    const real dth = dt * R(0.5), dts = dt / R(6.0);
+    put(y, R(0.0), y);
+    rhs(h0);
+    for (int step = 0; step < num_steps; ++step) {
+        if (step < 4) {
+            real acc[2 * HE], kk[2 * HE];
+            stage_values(row_at(step, 0));
+#pragma unroll
+            for (int e = 0; e < 2 * HE; ++e) acc[e] = h0[e];
+            put(y, dth, h0);
+            rhs(kk);
+#pragma unroll
+            for (int e = 0; e < 2 * HE; ++e) acc[e] = fma(R(2.0), kk[e], acc[e]);
+            put(y, dth, kk);
+            rhs(kk);
+#pragma unroll
+            for (int e = 0; e < 2 * HE; ++e) acc[e] = fma(R(2.0), kk[e], acc[e]);
+            put(y, dt, kk);
+            stage_values(row_at(step, 1));
+            rhs(kk);
+#pragma unroll
+            for (int e = 0; e < 2 * HE; ++e) {
+                acc[e] += kk[e];
+                y[e] = fma(dts, acc[e], y[e]);
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 2 * HE; ++e) {
+                real sum = R(0.0);
+                sum = fma(R(1901.0 / 720.0), h0[e], sum);
+                sum = fma(R(-1387.0 / 360.0), h1[e], sum);
+                sum = fma(R(109.0 / 30.0), h2[e], sum);
+                sum = fma(R(-637.0 / 360.0), h3[e], sum);
+                sum = fma(R(251.0 / 720.0), h4[e], sum);
+                y[e] = fma(dt, sum, y[e]);
+            }
+            stage_values(row_at(step, 1));
+        }
+        if (step + 1 < num_steps) {
+#pragma unroll
+            for (int e = 0; e < 2 * HE; ++e) {
+                h4[e] = h3[e];
+                h3[e] = h2[e];
+                h2[e] = h1[e];
+                h1[e] = h0[e];
+            }
+            put(y, R(0.0), y);
+            rhs(h0);
+        }
+#endif
+#endif
 #if GQ_OUT_MEAN
         if (step >= a.warmup_steps) {
             real yd = R(0.0);

@@ -658,12 +658,13 @@ def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, run
     reasons = []
     if n not in hilbert.SUPPORTED_N:
         reasons.append(f"N = {n} (supported: {hilbert.SUPPORTED_N})")
-    if spec.name != "rk4":
-        reasons.append(f"solver {spec.name} (RK4 only)")
+    if spec.adaptive or (spec.name != "rk4" and not sparse and liouville != "auto"):
+        reasons.append(f"solver {spec.name} (RK4 only; the sparse form takes every fixed-step "
+                       "method)")
     if mode not in ("mean", "final", "final_rho"):
         reasons.append(f"output mode {mode}")
-    if not wanted:
-        return None
+    if not wanted or (liouville == "auto" and spec.name != "rk4" and n < 32):
+        return None          # auto picks the sparse form (the one for other methods) at 32 levels
     if reasons:
         if forced:
             raise ValueError(f"liouville='{liouville}' does not apply: " + ", ".join(reasons))
@@ -687,6 +688,8 @@ def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, run
         # row) takes 29.7 ns per point-step in FP32 against 31.2 with FP16 hi/lo products and
         # 51 against 117 ns in FP64 (128^2, 200 RK4 steps)
         sparse = True
+    if spec.name != "rk4" and not sparse:
+        return None          # the dense matrix-core forms integrate with RK4 only
     values = dict(model["defaults"])
     values.update(runtime_consts or {})
     rt = [float(values[s]) for s in model["runtime_const_syms"]]
@@ -701,7 +704,7 @@ def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, run
                           warmup_steps=warmup_steps, block_size=block_size, collect=collect,
                           options=options, fp64=fp64, split=split, prepare=prepare,
                           num_trace=num_trace if trace else 0, trace_stride=trace_stride,
-                          trace_offset=trace_offset, sparse=sparse)
+                          trace_offset=trace_offset, sparse=sparse, spec=spec)
     if prepare:
         outputs["runtime_defaults"] = dict(model["defaults"])
     stage_s = {"ode_prepare_s": prep, "ode_optimize_s": 0.0, "emit_s": 0.0}
@@ -756,7 +759,9 @@ def auto_lanes_per_point(spec, d, fp64, nnz, points, natoms=0):
     * sparse rows, adaptive solvers: one lane up to 64 words on such a sweep (0.55 s against
       0.85 s with 8 lanes for Tsit5A at 63 FP32 states), else as the fixed-step tableaux;
     * sparse rows, other solvers (more stage vectors): 8 lanes from 48 states, 16 where 8 would
-      leave a wavefront more than 36 words."""
+      leave a wavefront more than 36 words of a vector or, fixed-step, more than 160 words of
+      stage vectors (the 4-qubit chain, 255 FP32 states: DOP853 6.7 against 35.4 ns per
+      point-step with 8 lanes, which spill; Tsit5 1.17 against 1.59)."""
     words = 2 if fp64 else 1
     if d < GROUPED_AUTO_MIN_STATES:
         return 1
@@ -764,7 +769,9 @@ def auto_lanes_per_point(spec, d, fp64, nnz, points, natoms=0):
     if sparse and spec.adaptive and d * words <= 64 and points >= 65536:
         groups = 1
     elif not sparse or spec.name != "rk4":
-        groups = 16 if sparse and -(-d // 8) * words > 36 else 8
+        rw = -(-d // 8) * words
+        many = not spec.adaptive and stage_vectors(spec) * rw > 160
+        groups = 16 if sparse and (rw > 36 or many) else 8
     elif d * words <= 128 and points >= 65536:
         groups = 1
     else:
@@ -776,6 +783,16 @@ def auto_lanes_per_point(spec, d, fp64, nnz, points, natoms=0):
     while groups > 1 and not grouped_applies(spec, d, fp64, groups, natoms):
         groups //= 2
     return groups
+
+
+def stage_vectors(spec):
+    """State-sized vectors a fixed-step method keeps per lane: the stages and the state (the
+    2N-storage LSRK4: state, register and stage; AB5: five past derivatives and the state)."""
+    if spec.name in ("lserk4", "lserk4f"):
+        return 3
+    if spec.name == "ab5":
+        return 6
+    return spec.rhs_evaluations + 1
 
 
 def grouped_buffers(spec, d, fp64, groups, natoms=0):

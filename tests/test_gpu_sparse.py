@@ -91,3 +91,45 @@ def test_sparse_table_form_and_a_dense_model(gpu, monkeypatch):
     ref, _ = integrate.rk4(rm.f, rm.ground_state(), 0.01, 100, obs=rm.obs)
     got = mesolve_2D(*_args(w), var_arrays=w["var_arrays"], liouville="sparse", fp64=True)
     assert np.max(np.abs(got.ravel() - ref)) <= 1e-10 * max(1.0, float(np.max(np.abs(ref))))
+
+
+FIXED_OTHER = ("tsit5", "dop853", "lserk4", "lserk4f", "dp5", "alshina6", "anas5", "ab5")
+
+
+@pytest.mark.parametrize("solver", FIXED_OTHER)
+def test_sparse_form_other_fixed_step_methods(gpu, solver):
+    # every fixed-step method on the sparse form against the FP64 oracle's same method: the
+    # 3-qubit chain in FP64 (1e-10) and FP32 (2e-5), 60 steps with a warmup, the mean and
+    # the final density matrix
+    nq, steps = 3, 60
+    w, m = _model(nq, steps, n_eps=3, n_amp=2)
+    kw = dict(var_arrays=w["var_arrays"], liouville="sparse", solver=solver,
+              solver_frequency=1.14)
+    ref, _ = integrate.fixed_step(solver, m.f, m.ground_state(), 0.01, steps, obs=m.obs,
+                                  warmup_steps=15, frequency=1.14)
+    fin, _ = integrate.fixed_step(solver, m.f, m.ground_state(), 0.01, steps, mode="final_rho",
+                                  frequency=1.14)
+    print(f"sparse form: {solver}, {nq} qubits, {steps} steps", flush=True)
+    for fp64, tol in ((True, 1e-10), (False, 2e-5)):
+        got, info = mesolve_2D(*_args(w), fp64=fp64, warmup_time=0.25, return_timing_info=True,
+                               **kw)
+        assert info["kernel"].startswith("gq_hilbert_sparse")
+        assert np.max(np.abs(got.ravel() - ref)) <= tol
+        rho = mesolve_2D(*_args(w), fp64=fp64, output_mode="final_rho", **kw)
+        assert np.max(np.abs(rho.reshape(6, -1) - fin)) <= tol
+
+
+@pytest.mark.parametrize("solver", ("tsit5", "dop853", "lserk4"))
+def test_sparse_form_five_qubits_other_methods(gpu, solver):
+    # the published 5-qubit rows' methods at their full state (1023 reduced states), FP64 and
+    # FP32 against the oracle over 20 steps on a 2 x 2 grid; auto picks the sparse form
+    nq, steps = 5, 20
+    w, m = _model(nq, steps)
+    ref, _ = integrate.fixed_step(solver, m.f, m.ground_state(), 0.01, steps, obs=m.obs)
+    scale = max(1.0, float(np.max(np.abs(ref))))
+    print(f"sparse form: {solver}, {nq} qubits, {steps} steps", flush=True)
+    for fp64, tol in ((True, 1e-10), (False, 2e-5)):
+        got, info = mesolve_2D(*_args(w), var_arrays=w["var_arrays"], solver=solver, fp64=fp64,
+                               return_timing_info=True)
+        assert info["kernel"].startswith("gq_hilbert_sparse")
+        assert np.max(np.abs(got.ravel() - ref)) <= tol * scale

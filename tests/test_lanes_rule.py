@@ -36,6 +36,16 @@ def test_adaptive_and_tableau_defaults():
     assert auto_lanes_per_point(TSIT5A, 399, False, 3000, BIG, 2) == 16  # 4.79 s vs 7.60 at 8
     assert auto_lanes_per_point(DP5, 63, False, 409, BIG, 2) == 8
     assert auto_lanes_per_point(DP5, 40, False, 200, BIG, 2) == 1
+    # fixed-step tableaux keep their stage vectors per lane: 16 lanes once 8 would leave more
+    # than 160 words (4-qubit chain, 255 FP32 states: DOP853 6.7 ns per point-step against
+    # 35.4 with 8 lanes, Tsit5 1.17 against 1.59); the 2N-storage LSRK4 keeps three vectors
+    assert auto_lanes_per_point(get_solver_spec("dop853", step_size=0.1), 255, False, 2000, BIG,
+                                2) == 16
+    assert auto_lanes_per_point(get_solver_spec("tsit5", step_size=0.1), 255, False, 2000, BIG,
+                                2) == 16
+    assert auto_lanes_per_point(get_solver_spec("lserk4", step_size=0.1), 255, False, 2000, BIG,
+                                2) == 8
+    assert auto_lanes_per_point(TSIT5A, 255, False, 2000, BIG, 2) == 8   # adaptive: unchanged
 
 
 def test_lds_copies():
