@@ -749,7 +749,9 @@ def auto_lanes_per_point(spec, d, fp64, nnz, points, natoms=0):
     * rows with many coefficients (``nnz`` >= 16 d, e.g. a dense 8-level H) or a non-affine
       RHS (``nnz`` None): 8 lanes from 48 states (the one-lane kernel spills; 1.2x FP32, 1.8x
       FP64 at 63 states);
-    * sparse rows, RK4: one lane while a point's state is at most 128 words (FP32 words; FP64
+    * sparse rows, RK4 and the 2N-storage LSRK4 (three state vectors; at 63 FP32 states one
+      lane takes 15.8 ms against 30.4 with 8): one lane while a point's state is at most 128
+      words (FP32 words; FP64
       counts two) and the sweep gives every SIMD a wavefront (65536 points) -- the grouped
       kernels' LDS round trips cost more than they save there (1.4x at 63 FP32 states) --
       otherwise the fewest lanes that leave each wavefront at most 50 words of a vector and
@@ -768,7 +770,7 @@ def auto_lanes_per_point(spec, d, fp64, nnz, points, natoms=0):
     sparse = nnz is not None and nnz < 16 * d
     if sparse and spec.adaptive and d * words <= 64 and points >= 65536:
         groups = 1
-    elif not sparse or spec.name != "rk4":
+    elif not sparse or spec.name not in ("rk4", "lserk4", "lserk4f"):
         rw = -(-d // 8) * words
         many = not spec.adaptive and stage_vectors(spec) * rw > 160
         groups = 16 if sparse and (rw > 36 or many) else 8

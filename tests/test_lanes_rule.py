@@ -45,6 +45,9 @@ def test_adaptive_and_tableau_defaults():
                                 2) == 16
     assert auto_lanes_per_point(get_solver_spec("lserk4", step_size=0.1), 255, False, 2000, BIG,
                                 2) == 8
+    # LSRK4 follows the RK4 rule: one lane at 63 FP32 states (15.8 ms against 30.4 with 8)
+    assert auto_lanes_per_point(get_solver_spec("lserk4f", step_size=0.1), 63, False, 409, BIG,
+                                2) == 1
     assert auto_lanes_per_point(TSIT5A, 255, False, 2000, BIG, 2) == 8   # adaptive: unchanged
 
 
