@@ -804,9 +804,8 @@ def bench_multiqubit(args):
     sx gaps, ZZ neighbours, drive on qubit 0, relaxation and dephasing on every qubit) on the
     published grid (1024 x 1024), 40 drive periods, published step counts and FP32, beside the
     published RTX 3080 kernel times.  Up to 4 qubits every solver runs on the per-point
-    kernels; at 5 qubits (1023 reduced states) the fixed-step solvers run on the sparse
-    Hilbert-space form, in 8 row blocks of the grid (kernel times summed; a block is 128 x 1024
-    points)."""
+    kernels; at 5 qubits (1023 reduced states) every solver runs on the sparse Hilbert-space
+    form, in 8 row blocks of the grid (kernel times summed; a block is 128 x 1024 points)."""
     import numpy as np
     import torch
     from gqis import mesolve_2D, workloads
@@ -830,12 +829,6 @@ def bench_multiqubit(args):
         for solver in args.solvers:
             label = f"{nq} qubits {solver}"
             pub = MULTI_QUBIT_PUBLISHED_S[solver][nq]
-            if nq >= 5 and solver == "tsit5a":
-                rows[label] = {"skipped": "no adaptive kernel for a 1023-state sweep: the per-"
-                                          "point kernels' LDS copies do not fit, the sparse "
-                                          "Hilbert-space form is fixed-step",
-                               "published_rtx3080_s": pub}
-                continue
             kw = {}
             if solver == "tsit5a":
                 tl = np.linspace(0.0, duration, 161)

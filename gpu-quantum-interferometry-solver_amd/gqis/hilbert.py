@@ -407,12 +407,13 @@ def block_for(m, fp64, block_size=256, valu=False, split=None, sparse=False):
 
 
 def emit(m, offs, mode, block_size=256, fp64=False, split=None, valu=False, trace=False,
-         sparse=False, solver="rk4"):
+         sparse=False, solver="rk4", count_steps=False):
     """hiprtc source of the Hilbert-space RK4 kernel for model m (``split``: the commutator
     GEMM as three FP16 or BF16 products of hi/lo parts with FP32 accumulation; FP64 runs on the
     FP64 matrix cores, or with ``valu`` as VALU dot products; ``sparse``: the wavefront-per-
     point kernel over an ELL table of H_eff, any precision, which also integrates with the
-    other fixed-step methods, ``solver``)."""
+    other fixed-step methods and the adaptive ones, ``solver``; ``count_steps``: the adaptive
+    kernel writes accepted / rejected step counts)."""
     if solver != "rk4" and not sparse:
         raise ValueError("the matrix-core forms integrate with RK4 only")
     split_kind = SPLIT_KINDS[split]
@@ -476,7 +477,15 @@ def emit(m, offs, mode, block_size=256, fp64=False, split=None, valu=False, trac
     template = ("hilbert_sparse.hip" if sparse else
                 "hilbert_rk4_dmfma.hip" if fp64 and not valu else "hilbert_rk4.hip")
     table = "rk4_table.hip"
-    if solver != "rk4":
+    if solver in ADAPTIVE:
+        # stage times come from the step controller: atoms per stage in FP64, no table
+        from . import tableaux
+        model_src += "\n" + tableaux.emit_arrays(tableaux.DOP853 if solver == "dop853a"
+                                                  else tableaux.TSIT5)
+        defines.update({"GQ_ADAPTIVE": 1, "GQ_METHOD_DOP853": int(solver == "dop853a"),
+                        "GQ_COUNT_STEPS": int(bool(count_steps))})
+        table = "step_control.hip"
+    elif solver != "rk4":
         from . import tableaux
         model_src += "\n" + tableaux.emit_fixed(solver)
         defines["GQ_TIME_TABLE"] = 1
@@ -500,6 +509,7 @@ def points_per_wave(n, fp64, valu=False, sparse=False):
 
 _SOURCES = {}
 SPARSE_UNROLL_OPTIONS = ("-mllvm", "-pragma-unroll-threshold=200000")
+ADAPTIVE = ("tsit5a", "dop853a")
 
 
 def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps, warmup_steps,
@@ -524,16 +534,27 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
     if sparse and (split or valu):
         raise ValueError("the sparse form has no split-precision or VALU variant")
     solver = spec.name if spec is not None else "rk4"
+    adaptive = solver in ADAPTIVE
+    count_steps = adaptive and bool(spec.options.get("count_steps"))
+    if adaptive:
+        if not sparse:
+            raise ValueError("the matrix-core forms integrate with fixed-step methods only")
+        if prepare:
+            raise ValueError("adaptive sweeps on the sparse form are not prepared (graph) sweeps")
+        # one wavefront per work-group: a work-group ends with its point, so the hardware's
+        # dispatch of new work-groups balances points of different step counts
+        block_size = 64
     block_size = block_for(m, fp64, block_size, valu, split, sparse)
     tune = block_size == "auto"
     if tune:
         block_size = block_for(m, fp64, 256, valu, split, sparse)
     trace = num_trace > 0
     skey = (m.key, mode, block_size, uses_rho0, tuple(options), bool(fp64), split, bool(valu),
-            trace, bool(sparse), bool(sparse and SPARSE_XOR), solver)
+            trace, bool(sparse), bool(sparse and SPARSE_XOR), solver, count_steps)
     src = _SOURCES.get(skey)
     if src is None:
-        src = emit(m, offs, mode, block_size, fp64, split, valu, trace, sparse, solver).replace(
+        src = emit(m, offs, mode, block_size, fp64, split, valu, trace, sparse, solver,
+                   count_steps).replace(
             "GQ_RHO0_FROM_ARGS", "1" if uses_rho0 else "0")
         _SOURCES[skey] = src
     options = codegen.HIPRTC_BASE_OPTIONS + tuple(options)
@@ -561,10 +582,12 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
                              dtype=torch.complex128 if fp64 else torch.complex64, device=dev)
                  if trace else None)
     from . import tableaux
-    rows = tableaux.rows_per_step(solver) * num_steps + 1
+    rows = (tableaux.rows_per_step(solver) * num_steps + 1) if not adaptive else 0
     table_kernel = codegen.kernel_name("gq_rk4_time_table" if solver == "rk4"
                                        else "gq_fixed_time_table", fp64)
-    n_atoms = int(src.split("#define GQ_NATOMS ")[1].split()[0])
+    n_atoms = int(src.split("#define GQ_NATOMS ")[1].split()[0]) if not adaptive else 0
+    steps_out = (torch.zeros((max(npts, 1), 2), dtype=torch.int32, device=dev)
+                 if count_steps else None)
     table = torch.empty(max(1, rows * max(1, n_atoms)), dtype=rdt, device=dev)
     a = runtime.SweepArgs()
     a.dt, a.t_in = float(dt), float(t_in)
@@ -580,6 +603,7 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
     a.result, a.status = result.data_ptr(), status.data_ptr()
     for i, v in enumerate(spec.parameters if spec is not None else ()):
         a.solver_param[i] = float(v)
+    a.adaptive_steps = steps_out.data_ptr() if steps_out is not None else None
     stream = torch.cuda.current_stream(dev).cuda_stream
     kname = codegen.kernel_name(kernel_base(split, fp64, valu, sparse), fp64)
     tune_s, tune_times = 0.0, {}
@@ -660,6 +684,8 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
     tr = (trace_buf[:, :npts].t().reshape(nx, ny, num_trace).contiguous()
           if trace_buf is not None else None)
     return {"result": out, "trace": tr, "status": flags, "gpu_s": gpu_s, "build_s": build_s,
-            "cached_kernel": cached, "setup_s": setup_s, "adaptive_steps": None,
+            "cached_kernel": cached, "setup_s": setup_s,
+            "adaptive_steps": (steps_out[:npts].reshape(nx, ny, 2).cpu().numpy()
+                               if steps_out is not None else None),
             "block_size": int(block_size), "autotune_s": tune_s, "autotune": tune_times,
             "kernel": kname}

@@ -12,7 +12,8 @@
 // nonzero per row add l_i conj(l_j) rho[col_i][col_j].  Methods: classical RK4 (GQ_SCHEME
 // undefined; stage times, k1 pipelining: gqis/cuda_solvers.py:470-517) or, with GQ_SCHEME and
 // the tableau constants of fixed_tableau.hip, an explicit Butcher tableau (0), the 2N-storage
-// LSRK4 (1) or AB5 (2).  Output semantics are the reference's (gqis/solver.py:2126-2193).
+// LSRK4 (1) or AB5 (2); with GQ_ADAPTIVE, adaptive Tsit5 / DOP853 under the controller of
+// adaptive.hip.  Output semantics are the reference's (gqis/solver.py:2126-2193).
 //
 // Off-diagonal structure, two forms:
 //   GQ_XOR 1: every row i has its nonzeros at columns i ^ mask_k with values v_k that do not
@@ -35,6 +36,9 @@
 #define HT (HN * (HN + 1) / 2)    // elements i <= j
 #define HE ((HT + 63) / 64)       // complex elements (slots) per lane
 #define HWAVES (GQ_BLOCK / 64)
+#ifndef GQ_ADAPTIVE
+#define GQ_ADAPTIVE 0
+#endif
 #if GQ_XOR
 static_assert((HN & (HN - 1)) == 0, "the XOR form addresses a register of qubits");
 #endif
@@ -268,9 +272,197 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_spar
     };
     bool bad_trace = false;
     (void)bad_trace;
-
-    stage_values(0);
     double avg = 0.0;
+    bool failed = false;
+
+#if GQ_ADAPTIVE
+    // ---- adaptive Tsit5 / DOP853 between the output points of tlist ---------------------------
+    // The controller of adaptive.hip (gqis/cuda_solvers.py:731-908: snap to the output point
+    // when h * 1.01 >= the remaining time, h kept across intervals, FSAL derivative kept on a
+    // rejection, NaN after more than max_substeps attempts in an interval, trapezoid mean over
+    // accepted steps).  One point per wavefront: the error norm is a wavefront sum (a butterfly,
+    // so every lane holds the same bits) and the controller is uniform.  The norm runs over the
+    // reduced state's N^2 - 1 components: the populations but the last, and the real and
+    // imaginary parts above the diagonal.
+    (void)dt2;
+    (void)dt6;
+    const double dt_out = a.dt;
+    const double rtol = a.solver_param[0], atol = a.solver_param[1];
+    const int max_sub = (int)a.solver_param[2];
+    const double inv_avg_time = 1.0 / ((double)(num_steps - a.warmup_steps) * dt_out);
+    bool cre[HE], cim[HE];
+#pragma unroll
+    for (int m = 0; m < HE; ++m) {
+        cre[m] = used[m] && (ri[m] < rj[m] || ri[m] < HN - 1);
+        cim[m] = used[m] && ri[m] < rj[m];
+    }
+    auto uniform = [](double v) {
+        const long long b = __double_as_longlong(v);
+        const int lo = __builtin_amdgcn_readfirstlane((int)b);
+        const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+        return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+    };
+    auto stage_at = [&](double t_abs) {
+        double T64[GQ_NATOMS_ALLOC];
+        gq_atoms_f64(t_abs, cst, T64);
+        real TT[GQ_NATOMS_ALLOC];
+#pragma unroll
+        for (int k = 0; k < GQ_NATOMS; ++k) TT[k] = (real)T64[k];
+        gq_basis(px, py, TT, cst, g);
+    };
+    auto obs_part = [&](const real* __restrict__ x) {
+        real yd = R(0.0);
+#pragma unroll
+        for (int m = 0; m < HE; ++m) yd = fma(od[m], x[m], yd);
+        return yd;
+    };
+    constexpr double ncomp = (double)(HN * HN - 1);
+    real k[GQ_NSTAGES][2 * HE];
+    real kf[2 * HE];
+    stage_at(a.t_in);
+    put(y, R(0.0), y);
+    rhs(k[0]);
+    double tau = 0.0, h = dt_out;
+    int step = 0, sub = 0, n_acc = 0, n_rej = 0;
+    real op = obs_part(y);
+    (void)op;
+    while (step < num_steps) {
+        const double remaining = dt_out - tau;
+        const bool last = h * 1.01 >= remaining;
+        const double hs = last ? remaining : h;
+        const double t0 = (double)step * dt_out + tau;
+        const real hr = (real)hs;
+#pragma unroll
+        for (int s = 1; s < GQ_NSTAGES; ++s) {
+#pragma unroll
+            for (int m = 0; m < HE; ++m) {
+                real ar = R(0.0), ai = R(0.0);
+#pragma unroll
+                for (int j = 0; j < s; ++j) {
+                    if (GQ_A[s][j] != 0.0) {
+                        ar = fma((real)GQ_A[s][j], k[j][m], ar);
+                        ai = fma((real)GQ_A[s][j], k[j][m + HE], ai);
+                    }
+                }
+                store(m, fma(hr, ar, y[m]), fma(hr, ai, y[m + HE]));
+            }
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            stage_at(a.t_in + (t0 + GQ_C[s] * hs));
+            rhs(k[s]);
+        }
+        real yn[2 * HE];
+#pragma unroll
+        for (int e = 0; e < 2 * HE; ++e) {
+            real acc = R(0.0);
+#pragma unroll
+            for (int j = 0; j < GQ_NSTAGES; ++j)
+                if (GQ_B[j] != 0.0) acc = fma((real)GQ_B[j], k[j][e], acc);
+            yn[e] = fma(hr, acc, y[e]);
+        }
+        // the stage values now hold t0 + hs (the last stage has c = 1)
+        double err;
+#if GQ_METHOD_DOP853
+        {
+            double s5 = 0.0, s3 = 0.0;
+#pragma unroll
+            for (int e = 0; e < 2 * HE; ++e) {
+                if (!(e < HE ? cre[e] : cim[e - HE])) continue;
+                double e5 = 0.0, e3 = 0.0;
+#pragma unroll
+                for (int j = 0; j < GQ_NSTAGES; ++j) {
+                    if (GQ_E5[j] != 0.0) e5 = fma(GQ_E5[j], (double)k[j][e], e5);
+                    if (GQ_E3[j] != 0.0) e3 = fma(GQ_E3[j], (double)k[j][e], e3);
+                }
+                const double sc = fma(rtol, fmax(fabs((double)y[e]), fabs((double)yn[e])), atol);
+                s5 = fma(e5 / sc, e5 / sc, s5);
+                s3 = fma(e3 / sc, e3 / sc, s3);
+            }
+            s5 = wave_sum(s5);
+            s3 = wave_sum(s3);
+            double deno = s5 + 0.01 * s3;
+            if (!(deno > 0.0)) deno = 1.0;
+            err = fabs(hs) * s5 / sqrt(deno * ncomp);
+        }
+#else
+        put(yn, R(0.0), yn);
+        rhs(kf);
+        {
+            double ssum = 0.0;
+#pragma unroll
+            for (int e = 0; e < 2 * HE; ++e) {
+                if (!(e < HE ? cre[e] : cim[e - HE])) continue;
+                double d = GQ_E[GQ_NSTAGES] * (double)kf[e];
+#pragma unroll
+                for (int j = 0; j < GQ_NSTAGES; ++j) d = fma(GQ_E[j], (double)k[j][e], d);
+                d *= hs;
+                const double sc = fma(rtol, fmax(fabs((double)y[e]), fabs((double)yn[e])), atol);
+                ssum = fma(d / sc, d / sc, ssum);
+            }
+            err = sqrt(wave_sum(ssum) / ncomp);
+        }
+#endif
+        err = uniform(err);
+        const double factor = gq_step_factor(err);   // step_control.hip
+        if (err <= 1.0) {
+            ++n_acc;
+#if GQ_METHOD_DOP853
+            put(yn, R(0.0), yn);
+            rhs(kf);
+#endif
+#if GQ_OUT_MEAN
+            const real o = obs_part(yn);
+            if (step >= a.warmup_steps)
+                avg = fma(0.5 * hs * inv_avg_time, (double)o + (double)op, avg);
+            op = o;
+#endif
+#pragma unroll
+            for (int e = 0; e < 2 * HE; ++e) {
+                y[e] = yn[e];
+                k[0][e] = kf[e];
+            }
+            tau = last ? dt_out : tau + hs;
+            h = last ? fmax(h, hs * factor) : hs * factor;
+        } else {
+            ++n_rej;
+            h = hs * factor;
+        }
+        if (++sub > max_sub) {
+            failed = true;
+#pragma unroll
+            for (int e = 0; e < 2 * HE; ++e) y[e] = (real)__builtin_nan("");
+        }
+        if (failed || tau >= dt_out) {
+#if GQ_TRACE
+            if (step >= a.trace_offset && (step - a.trace_offset) % a.trace_stride == 0) {
+                const int kk = (step - a.trace_offset) / a.trace_stride;
+                if (kk < a.num_trace) {
+                    const double s = wave_sum((double)obs_part(y));
+                    if (lane == 0 && valid) {
+                        const real r = (real)s;
+                        bad_trace |= !gq_finite(r);
+                        gq_store_complex(a.trace, (long long)kk * a.trace_point_stride + q, r,
+                                         R(0.0));
+                    }
+                }
+            }
+#endif
+            ++step;
+            tau = 0.0;
+            sub = 0;
+            if (failed) break;
+        }
+    }
+#if GQ_COUNT_STEPS
+    if (lane == 0 && valid) {
+        a.adaptive_steps[2 * q] = n_acc;
+        a.adaptive_steps[2 * q + 1] = n_rej;
+    }
+#else
+    (void)n_acc;
+    (void)n_rej;
+#endif
+#else
+    stage_values(0);
 #ifndef GQ_SCHEME
     // classical RK4 (table rows 2 step + 1 and 2 step + 2: midpoint and end of the step)
     real k[2 * HE];
@@ -442,6 +634,7 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_spar
         }
 #endif
     }
+#endif  // GQ_ADAPTIVE
 
     bool bad = false;
 #if GQ_OUT_MEAN || GQ_OUT_FINAL
@@ -455,9 +648,10 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_spar
 #endif
     double s = wave_sum(avg);
     if (lane == 0) {
-#if GQ_OUT_MEAN
+#if GQ_OUT_MEAN && !GQ_ADAPTIVE
         s /= (double)(num_steps - a.warmup_steps);
 #endif
+        if (failed) s = __builtin_nan("");
         if (valid) {
             const real r = (real)s;
             bad = !gq_finite(r);
@@ -483,6 +677,7 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_spar
     }
 #endif
     gq_flag(a.status, bad, GQ_STATUS_NONFINITE_RESULT);
+    gq_flag(a.status, failed && lane == 0 && valid, GQ_STATUS_SUBSTEP_OVERFLOW);
 #if GQ_TRACE
     gq_flag(a.status, bad_trace, GQ_STATUS_NONFINITE_TRACE);
 #endif

@@ -658,9 +658,11 @@ def _try_hilbert(H, Drive, Col_Ops, mean_operator, var_arrays, const_values, run
     reasons = []
     if n not in hilbert.SUPPORTED_N:
         reasons.append(f"N = {n} (supported: {hilbert.SUPPORTED_N})")
-    if spec.adaptive or (spec.name != "rk4" and not sparse and liouville != "auto"):
-        reasons.append(f"solver {spec.name} (RK4 only; the sparse form takes every fixed-step "
-                       "method)")
+    if spec.name != "rk4" and not sparse and liouville != "auto":
+        reasons.append(f"solver {spec.name} (RK4 only; the sparse form takes every method)")
+    if spec.adaptive and spec.options.get("state_fp64") and not fp64:
+        reasons.append("adaptive_options state_fp64 (the Hilbert-space forms keep the state in "
+                       "the kernel's precision)")
     if mode not in ("mean", "final", "final_rho"):
         reasons.append(f"output mode {mode}")
     if not wanted or (liouville == "auto" and spec.name != "rk4" and n < 32):

@@ -133,3 +133,71 @@ def test_sparse_form_five_qubits_other_methods(gpu, solver):
                                return_timing_info=True)
         assert info["kernel"].startswith("gq_hilbert_sparse")
         assert np.max(np.abs(got.ravel() - ref)) <= tol * scale
+
+
+@pytest.mark.parametrize("solver", ["tsit5a", "dop853a"])
+def test_sparse_form_adaptive_methods(gpu, solver):
+    # adaptive Tsit5 / DOP853 on the sparse form, 3-qubit chain on a 3 x 2 grid, 40 output
+    # intervals of 0.1: FP64 against the FP64 oracle within 1e-10 with the oracle's accepted and
+    # rejected step counts on every point; FP32 at the reference's default tolerances within
+    # 2e-5 (a CPU emulation with the RHS rounded to FP32 stays within 2e-7)
+    w = workloads.multi_qubit(3, n_eps=3, n_amp=2, n_t=41, t_end=4.0)
+    eps_v, amp_v = w["var_arrays"].values()
+    m = models.multi_qubit(3, eps_v, amp_v)
+    opts = {"step_rtol": 1e-6, "step_atol": 1e-8, "max_substeps": 1000, "count_steps": True}
+    kw = dict(var_arrays=w["var_arrays"], liouville="sparse", solver=solver)
+    fn = integrate.tsit5a if solver == "tsit5a" else integrate.dop853a
+    print(f"sparse form: {solver}, 3 qubits, 40 output intervals", flush=True)
+    if solver == "tsit5a":
+        ref, _, acc, rej = fn(m.f, m.ground_state(), 0.1, 40, rtol=1e-6, atol=1e-8,
+                              max_substeps=1000, obs=m.obs, mode="mean", warmup_steps=10)
+        got, info = mesolve_2D(*_args(w), fp64=True, warmup_time=0.25, adaptive_options=opts,
+                               return_timing_info=True, **kw)
+    else:
+        ref, _, acc, rej = fn(m.f, m.ground_state(), 0.1, 40, rtol=1e-6, atol=1e-8,
+                              max_substeps=1000, obs=m.obs, mode="final")
+        got, info = mesolve_2D(*_args(w), fp64=True, output_mode="final", adaptive_options=opts,
+                               return_timing_info=True, **kw)
+    assert info["kernel"].startswith("gq_hilbert_sparse")
+    np.testing.assert_array_equal(info["adaptive_steps"]["accepted"].reshape(-1), acc)
+    np.testing.assert_array_equal(info["adaptive_steps"]["rejected"].reshape(-1), rej)
+    assert np.max(np.abs(got.ravel() - ref)) <= 1e-10
+    fin, _, _, _ = fn(m.f, m.ground_state(), 0.1, 40, rtol=1e-6, atol=1e-8, max_substeps=1000,
+                      mode="final_rho")
+    rho = mesolve_2D(*_args(w), fp64=True, output_mode="final_rho", adaptive_options=opts, **kw)
+    assert np.max(np.abs(rho.reshape(6, -1) - fin)) <= 1e-10
+    ref32, _, _, _ = fn(m.f, m.ground_state(), 0.1, 40, rtol=1e-5, atol=1e-5,
+                        max_substeps=1000, mode="final_rho")
+    rho32 = mesolve_2D(*_args(w), fp64=False, output_mode="final_rho",
+                       adaptive_options={"step_rtol": 1e-5, "step_atol": 1e-5}, **kw)
+    err = float(np.max(np.abs(rho32.reshape(6, -1) - ref32)))
+    print(f"  FP32 final state: max abs error {err:.2e}", flush=True)
+    assert err <= 2e-5
+
+
+def test_sparse_form_adaptive_five_qubits_trace_and_overflow(gpu):
+    # 5 qubits (1023 reduced states), automatic form: Tsit5A FP64 with a trace every 2nd
+    # output point against the oracle; too few substeps make every point NaN and raise
+    w = workloads.multi_qubit(5, n_eps=2, n_amp=2, n_t=11, t_end=1.0)
+    eps_v, amp_v = w["var_arrays"].values()
+    m = models.multi_qubit(5, eps_v, amp_v)
+    opts = {"step_rtol": 1e-6, "step_atol": 1e-8, "max_substeps": 1000, "count_steps": True}
+    ref, ref_tr, acc, rej = integrate.tsit5a(m.f, m.ground_state(), 0.1, 10, rtol=1e-6,
+                                             atol=1e-8, max_substeps=1000, obs=m.obs,
+                                             mode="final", trace_stride=2, n_trace=5)
+    print("sparse form: tsit5a, 5 qubits, 10 output intervals", flush=True)
+    got, tr, times, info = mesolve_2D(*_args(w), var_arrays=w["var_arrays"], solver="tsit5a",
+                                      fp64=True, output_mode="final", adaptive_options=opts,
+                                      return_time_trace=True, time_trace_every=2,
+                                      return_timing_info=True)
+    assert info["kernel"].startswith("gq_hilbert_sparse")
+    np.testing.assert_array_equal(info["adaptive_steps"]["accepted"].reshape(-1), acc)
+    assert np.max(np.abs(got.ravel() - ref)) <= 1e-10
+    assert np.max(np.abs(tr.reshape(4, 5) - ref_tr)) <= 1e-10
+    tight = {"step_rtol": 1e-14, "step_atol": 1e-14, "max_substeps": 2}
+    with pytest.raises(RuntimeError, match="Non-finite"):
+        mesolve_2D(*_args(w), var_arrays=w["var_arrays"], solver="tsit5a", fp64=True,
+                   adaptive_options=tight)
+    res = mesolve_2D(*_args(w), var_arrays=w["var_arrays"], solver="tsit5a", fp64=True,
+                     adaptive_options=tight, ignore_non_finite_output=True)
+    assert np.all(np.isnan(res))
