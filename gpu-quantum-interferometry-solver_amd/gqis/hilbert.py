@@ -357,10 +357,11 @@ def constants(m, runtime_values, split=None):
 LDS_BYTES_PER_CU = 160 * 1024      # gfx950
 
 
-def lds_bytes(n, fp64, block, jmax, split=None, valu=False, mono=0, sparse_k=0):
+def lds_bytes(n, fp64, block, jmax, split=None, valu=False, mono=0, sparse_k=0, nbasis=0):
     """Static LDS of the kernel (the ``__shared__`` arrays of ``kernels/hilbert_rk4.hip`` and,
     for FP64 without ``valu``, ``kernels/hilbert_rk4_dmfma.hip``); ``mono`` general jump
-    operators add a per-wavefront copy of the state and their row tables.  ``sparse_k`` > 0:
+    operators add a per-wavefront copy of the state and their row tables; ``nbasis`` drive
+    basis terms add their row coefficients (``hilbert_rk4.hip``).  ``sparse_k`` > 0:
     ``kernels/hilbert_sparse.hip`` with that ELL width (a state copy per wavefront)."""
     if sparse_k:
         # state copies, the static diagonal coefficients, drive rows (up to 8 basis terms),
@@ -379,10 +380,10 @@ def lds_bytes(n, fp64, block, jmax, split=None, valu=False, mono=0, sparse_k=0):
     if mono:
         waves, pts = block // 64, 32 // n
         extra = waves * 2 * pts * n * (n + 1) * r + mono * n * (4 + 2 * r)
-        return lds_bytes(n, fp64, block, jmax, split, valu) + extra
+        return lds_bytes(n, fp64, block, jmax, split, valu, nbasis=nbasis) + extra
     waves, pts, tiles = block // 64, 32 // n, (2 * n + 31) // 32
     total = waves * 2 * pts * n * (n + 1) * r + waves * pts * n * r + n * n * r
-    total += n * jmax * (r + 4) + waves * 64 * 8
+    total += n * jmax * (r + 4) + waves * 64 * 8 + nbasis * n * r
     if fp64:
         total += 4 * n * n * 8 + waves * pts * 2 * n * n * 8
     elif split:
@@ -401,7 +402,7 @@ def block_for(m, fp64, block_size=256, valu=False, split=None, sparse=False):
     b = int(block_size)
     kw = ell_table(m)[0] if sparse else 0
     while b > 64 and lds_bytes(m.n, fp64, b, jump_width(m), split, valu,
-                               n_mono(m), kw) > LDS_BYTES_PER_CU:
+                               n_mono(m), kw, len(m.basis)) > LDS_BYTES_PER_CU:
         b //= 2
     return b
 
@@ -618,8 +619,8 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
         jmax = jump_width(m)
         kw = ell_table(m)[0] if sparse else 0
         fits = tuple(b for b in autotune.CANDIDATES
-                     if lds_bytes(m.n, fp64, b, jmax, split, valu, n_mono(m), kw)
-                     <= LDS_BYTES_PER_CU)
+                     if lds_bytes(m.n, fp64, b, jmax, split, valu, n_mono(m), kw,
+                                  len(m.basis)) <= LDS_BYTES_PER_CU)
         spare = torch.zeros(1, dtype=torch.int32, device=dev)
 
         def trial(mod, block):

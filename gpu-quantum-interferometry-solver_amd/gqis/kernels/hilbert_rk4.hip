@@ -49,6 +49,24 @@ typedef gq_half gq_half8 __attribute__((ext_vector_type(8)));
 #define HNB (HN / 8)             // K blocks of the split-precision products
 
 __device__ __forceinline__ int gq_row_of(int m, int h) { return 8 * (m >> 2) + 4 * h + (m & 3); }
+// the part of gq_row_of that does not depend on the lane: a compile-time constant per m
+__device__ __forceinline__ constexpr int gq_row_off(int m) { return 8 * (m >> 2) + (m & 3); }
+
+// Lean register layout (N = 32, FP32 and split products): the drive row coefficients, the
+// dephasing weights and the Y parts between the two passes of the RHS are read from LDS
+// instead of registers, and the LDS addresses derive from four bases hidden from the compiler
+// (gq_opaque), recomputed per RHS as base + a constant folded into the LDS instruction instead
+// of one hoisted register per address.  Two wavefronts per SIMD then fit (250 registers, no
+// spills): 256^2, 2000 steps, FP32 1.393 -> 1.242 s, FP16 hi/lo 0.765 -> 0.632 s.  At N = 16
+// the same layout ran 9-16% slower (the extra LDS reads cost more than a third wavefront
+// gains), so smaller N keep the values in registers.
+#define GQ_LEAN (HN == 32 && !GQ_FP64)
+__device__ __forceinline__ int gq_opaque(int v) {
+#if GQ_LEAN
+    asm volatile("" : "+v"(v));
+#endif
+    return v;
+}
 
 // the split-precision forms carry their product type in the name (gq_hilbert_rk4_f16x3_f32)
 #if GQ_MFMA_SPLIT == 1
@@ -58,7 +76,16 @@ __device__ __forceinline__ int gq_row_of(int m, int h) { return 8 * (m >> 2) + 4
 #else
 #define GQ_HILBERT_NAME gq_hilbert_rk4
 #endif
-extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME)(const gq_sweep_args a) {
+// At N = 32 two wavefronts per SIMD fit in 250 registers (no spills) once the state is kept
+// in three vectors and the LDS addresses derive from four opaque bases; without the hint the
+// compiler spends 281 and runs one wavefront, with no second one to issue VALU work while
+// the matrix cores run.
+#if HN == 32 && !GQ_FP64
+#define GQ_HILBERT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(2)))
+#else
+#define GQ_HILBERT_OCCUPANCY
+#endif
+extern "C" __global__ __launch_bounds__(GQ_BLOCK) GQ_HILBERT_OCCUPANCY void GQ_KERNEL(GQ_HILBERT_NAME)(const gq_sweep_args a) {
 #if GQ_FP64
     __shared__ real s_hbig[2 * HN * 2 * HN];
     __shared__ real s_x[HWAVES][HP * 2 * HN * HN];
@@ -79,6 +106,7 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
 #endif
     __shared__ real s_diag[HWAVES][HP * HN];
     __shared__ real s_dep[HN * HN];           // dephasing weights sum_k d_k(i) d_k(j)
+    __shared__ real s_vco[GQ_NBASIS * HN];    // drive row coefficients per basis term
     __shared__ real s_jw[HN * GQ_JMAX];       // population transfer into a: weights |l|^2 ...
     __shared__ int s_jb[HN * GQ_JMAX];        // ... from the populations b (sparse rows)
     __shared__ double s_red[HWAVES][64];
@@ -96,6 +124,7 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
     for (int e = threadIdx.x; e < HT * HN * 64; e += GQ_BLOCK) s_apack[e] = cst[GQ_OFF_APACK + e];
 #endif
     for (int e = threadIdx.x; e < HN * HN; e += GQ_BLOCK) s_dep[e] = cst[GQ_OFF_DEP + e];
+    for (int e = threadIdx.x; e < GQ_NBASIS * HN; e += GQ_BLOCK) s_vco[e] = cst[GQ_OFF_VCOEF + e];
     for (int e = threadIdx.x; e < HN * GQ_JMAX; e += GQ_BLOCK) {
         s_jw[e] = cst[GQ_OFF_JW + e];
         s_jb[e] = (int)cst[GQ_OFF_JB + e];
@@ -155,19 +184,23 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
         y[m + HM] = im;
     }
 
+#if !GQ_LEAN
     // per-row diagonal drive coefficients v_i = sum_k coef[k][i] g_k(px, py, t)
     real vco[GQ_NBASIS][HM];
 #pragma unroll
     for (int k = 0; k < GQ_NBASIS; ++k)
 #pragma unroll
-        for (int m = 0; m < HM; ++m) vco[k][m] = cst[GQ_OFF_VCOEF + k * HN + gq_row_of(m, h)];
+        for (int m = 0; m < HM; ++m) vco[k][m] = s_vco[k * HN + gq_row_of(m, h)];
+#endif
     real odiag[HM];
 #pragma unroll
     for (int m = 0; m < HM; ++m) odiag[m] = cst[GQ_OFF_OBS + gq_row_of(m, h)];
 
-    // Per-lane constants of the dissipator, kept in registers: the dephasing weights of the
-    // lane's elements and, for the lane holding its column's diagonal element (m = mdiag), that
-    // population's jump list (zero weights elsewhere), so the RHS has no per-element branch.
+    // Per-lane constants of the dissipator, kept in registers: for the lane holding its
+    // column's diagonal element (m = mdiag), that population's jump list (zero weights
+    // elsewhere), so the RHS has no per-element branch.  The drive row coefficients and the
+    // dephasing weights are read from LDS when used: in registers they took 16 (1 + NBASIS)
+    // of them at N = 32, enough to keep the kernel at one wavefront per SIMD.
 #if GQ_NMONO
     real* __restrict__ xm_r = s_xm[wave][0] + pw * HN * HLD;
     real* __restrict__ xm_i = s_xm[wave][1] + pw * HN * HLD;
@@ -180,14 +213,15 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
         mli[kk] = s_mim[kk * HN + j];
     }
 #endif
-    real depw[HM];
     int mdiag = -1;
 #pragma unroll
-    for (int m = 0; m < HM; ++m) {
-        const int i = gq_row_of(m, h);
-        depw[m] = s_dep[i * HN + j];
-        if (i == j) mdiag = m;
-    }
+    for (int m = 0; m < HM; ++m)
+        if (gq_row_of(m, h) == j) mdiag = m;
+#if !GQ_LEAN
+    real depw[HM];
+#pragma unroll
+    for (int m = 0; m < HM; ++m) depw[m] = s_dep[gq_row_of(m, h) * HN + j];
+#endif
     real od = R(0.0);   // observable weight of that diagonal element (0 without one)
 #pragma unroll
     for (int m = 0; m < HM; ++m) od = (m == mdiag) ? odiag[m] : od;
@@ -203,15 +237,18 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
     const real dt2 = dt * R(0.5);
     const real dt6 = dt / R(6.0);
     const int num_steps = a.num_steps;
-    real v[HM];
+    real g[GQ_NBASIS];   // drive basis terms at the current stage time
+#if !GQ_LEAN
+    real v[HM];          // the diagonal drive of the lane's rows at that time
+#endif
 
     auto stage_values = [&](int row) {
         const real* __restrict__ tab = (const real*)a.time_table;
         real TT[GQ_NATOMS_ALLOC];
 #pragma unroll
         for (int k = 0; k < GQ_NATOMS; ++k) TT[k] = tab[row * GQ_NATOMS + k];
-        real g[GQ_NBASIS];
         gq_basis(px, py, TT, cst, g);
+#if !GQ_LEAN
 #pragma unroll
         for (int m = 0; m < HM; ++m) {
             real s = R(0.0);
@@ -219,9 +256,13 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
             for (int k = 0; k < GQ_NBASIS; ++k) s = fma(vco[k][m], g[k], s);
             v[m] = s;
         }
+#endif
     };
 
-    auto rhs = [&](const real* __restrict__ x, real* __restrict__ k) {
+    // the RHS at the stage input x, written over x: each element is read before the same
+    // element is written, so RK4 keeps three state vectors, not four
+    auto rhs = [&](real* x) {
+        real* k = x;
         real X[HN];   // rows row(m, h) of H_eff rho in the Rbig layout, column c
 #if GQ_FP64
 #pragma unroll
@@ -289,21 +330,40 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
 #pragma unroll
         for (int m = 0; m < HN; ++m) X[m] = acc[m >> 4][m & 15];
 #endif
-        real yr[HM], yi[HM];
+        // element m of the lane is (i, j) with i = gq_row_off(m) + 4 h: its tile index is
+        // b_own + gq_row_off(m) HLD, its transpose's b_tr + gq_row_off(m)
+        const int b_own = gq_opaque(4 * h * HLD + j);
+        const int b_tr = gq_opaque(j * HLD + 4 * h);
+        const int b_dep = gq_opaque(4 * h * HN + j);
+        const int b_row = gq_opaque(4 * h);
         real xd = R(0.0);   // the lane's diagonal element (population), if it holds one
+#if !GQ_LEAN
+        real yr[HM], yi[HM];
+#endif
 #pragma unroll
         for (int m = 0; m < HM; ++m) {
             const real xr = X[m];
             const real xi = X[m + HM];
-            // Y = -i X - i v rho
+            const int own = b_own + gq_row_off(m) * HLD;
+#if GQ_LEAN
+            // v_i = sum_k coef[k][i] g_k (the diagonal drive of row i)
+            real vm = R(0.0);
+#pragma unroll
+            for (int kb = 0; kb < GQ_NBASIS; ++kb)
+                vm = fma(s_vco[kb * HN + b_row + gq_row_off(m)], g[kb], vm);
+            // Y = -i X - i v rho, into the transpose tile (read back below)
+            ybuf_r[own] = fma(vm, x[m + HM], xi);
+            ybuf_i[own] = -fma(vm, x[m], xr);
+#else
+            (void)b_row;
             yr[m] = fma(v[m], x[m + HM], xi);
             yi[m] = -fma(v[m], x[m], xr);
-            const int i = gq_row_of(m, h);
-            ybuf_r[i * HLD + j] = yr[m];
-            ybuf_i[i * HLD + j] = yi[m];
+            ybuf_r[own] = yr[m];
+            ybuf_i[own] = yi[m];
+#endif
 #if GQ_NMONO
-            xm_r[i * HLD + j] = x[m];
-            xm_i[i * HLD + j] = x[m + HM];
+            xm_r[own] = x[m];
+            xm_i[own] = x[m + HM];
 #endif
             xd = (m == mdiag) ? x[m] : xd;
         }
@@ -315,10 +375,20 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
 #pragma unroll
         for (int m = 0; m < HM; ++m) {
             const int i = gq_row_of(m, h);
-            real dr = yr[m] + ybuf_r[j * HLD + i];
-            real di = yi[m] - ybuf_i[j * HLD + i];
-            dr = fma(depw[m], x[m], dr);
-            di = fma(depw[m], x[m + HM], di);
+            const int tr = b_tr + gq_row_off(m);
+#if GQ_LEAN
+            const int own = b_own + gq_row_off(m) * HLD;
+            real dr = ybuf_r[own] + ybuf_r[tr];
+            real di = ybuf_i[own] - ybuf_i[tr];
+            const real dw = s_dep[b_dep + gq_row_off(m) * HN];
+#else
+            (void)b_dep;
+            real dr = yr[m] + ybuf_r[tr];
+            real di = yi[m] - ybuf_i[tr];
+            const real dw = depw[m];
+#endif
+            dr = fma(dw, x[m], dr);
+            di = fma(dw, x[m + HM], di);
 #if GQ_NMONO
 #pragma unroll
             for (int kk = 0; kk < GQ_NMONO; ++kk) {
@@ -354,37 +424,43 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(GQ_HILBERT_NAME
     (void)bad_trace;
 
     stage_values(0);
-    real k[HN];
-    rhs(y, k);
+    real kb[HN];   // k1 of a step, then each stage input and, in place, its derivative
+#pragma unroll
+    for (int i = 0; i < HN; ++i) kb[i] = y[i];
+    rhs(kb);
     double avg = 0.0;
     for (int step = 0; step < num_steps; ++step) {
         stage_values(2 * step + 1);
-        real yt[HN], acc[HN];
+        real acc[HN];
 #pragma unroll
         for (int i = 0; i < HN; ++i) {
-            yt[i] = fma(dt2, k[i], y[i]);
-            acc[i] = k[i];
+            acc[i] = kb[i];
+            kb[i] = fma(dt2, kb[i], y[i]);
         }
-        rhs(yt, k);
+        rhs(kb);
 #pragma unroll
         for (int i = 0; i < HN; ++i) {
-            yt[i] = fma(dt2, k[i], y[i]);
-            acc[i] = fma(R(2.0), k[i], acc[i]);
+            acc[i] = fma(R(2.0), kb[i], acc[i]);
+            kb[i] = fma(dt2, kb[i], y[i]);
         }
-        rhs(yt, k);
+        rhs(kb);
 #pragma unroll
         for (int i = 0; i < HN; ++i) {
-            yt[i] = fma(dt, k[i], y[i]);
-            acc[i] = fma(R(2.0), k[i], acc[i]);
+            acc[i] = fma(R(2.0), kb[i], acc[i]);
+            kb[i] = fma(dt, kb[i], y[i]);
         }
         stage_values(2 * step + 2);
-        rhs(yt, k);
+        rhs(kb);
 #pragma unroll
         for (int i = 0; i < HN; ++i) {
-            acc[i] += k[i];
+            acc[i] += kb[i];
             y[i] = fma(dt6, acc[i], y[i]);
         }
-        if (step + 1 < num_steps) rhs(y, k);
+        if (step + 1 < num_steps) {
+#pragma unroll
+            for (int i = 0; i < HN; ++i) kb[i] = y[i];
+            rhs(kb);
+        }
 #if GQ_OUT_MEAN
         if (step >= a.warmup_steps) {
             // the lane's diagonal element (if any) weighted by the observable's diagonal

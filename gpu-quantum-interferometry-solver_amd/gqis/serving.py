@@ -123,7 +123,7 @@ class SweepServer:
             natoms = int(src.split("#define GQ_NATOMS ")[1].split()[0])
             cands = tuple((1, b) for b in autotune.CANDIDATES
                           if hilbert.lds_bytes(hm.n, r.fp64, b, hilbert.jump_width(hm), split,
-                                               mono=hilbert.n_mono(hm))
+                                               mono=hilbert.n_mono(hm), nbasis=len(hm.basis))
                           <= hilbert.LDS_BYTES_PER_CU)
 
             def variant(c):

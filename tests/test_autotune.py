@@ -92,7 +92,8 @@ def test_hilbert_lds_formula_matches_the_compiler():
     for block, valu in ((256, True), (512, False)):
         src = hilbert.emit(hm, offs, "mean", block, True, valu=valu).replace(
             "GQ_RHO0_FROM_ARGS", "0")
-        est = hilbert.lds_bytes(32, True, block, hilbert.jump_width(hm), valu=valu)
+        est = hilbert.lds_bytes(32, True, block, hilbert.jump_width(hm), valu=valu,
+                                nbasis=len(hm.basis))
         assert est > hilbert.LDS_BYTES_PER_CU
         with pytest.raises(runtime.GqError, match=rf"local memory \({est}\) exceeds limit"):
             runtime.compile_check(src, codegen.HIPRTC_BASE_OPTIONS)
@@ -111,7 +112,7 @@ def test_hilbert_lds_formula_with_general_jump_operators():
     assert hilbert.n_mono(hm) == 5
     _, offs = hilbert.constants(hm, [])
     src = hilbert.emit(hm, offs, "mean", 256, True).replace("GQ_RHO0_FROM_ARGS", "0")
-    est = hilbert.lds_bytes(32, True, 256, hilbert.jump_width(hm), mono=5)
+    est = hilbert.lds_bytes(32, True, 256, hilbert.jump_width(hm), mono=5, nbasis=len(hm.basis))
     assert est > hilbert.LDS_BYTES_PER_CU
     with pytest.raises(runtime.GqError, match=rf"local memory \({est}\) exceeds limit"):
         runtime.compile_check(src, codegen.HIPRTC_BASE_OPTIONS)
