@@ -631,8 +631,13 @@ def bench_scaling(args):
                 entry.update({"flops_per_point_step": flops, "achieved_tflops": ach,
                               f"mfma_{'fp64' if fp64 else 'fp32'}_peak_frac": ach / peak})
                 if form not in ("mfma", "mfma64"):
-                    # three half-precision products per FP32-equivalent product
-                    entry["half_mfma_issued_frac"] = 3 * ach / 2500.0
+                    # three half-precision products per FP32-equivalent product; from 16
+                    # levels the two-sided kernel runs both sides of the commutator on the
+                    # matrix cores (twice the products of H_eff rho alone)
+                    sides = 2 if n >= 16 else 1
+                    issued = sides * 3 * 4 * 8 * n ** 3 * ps / k / 1e12
+                    entry.update({"half_mfma_issued_tflops": issued,
+                                  "half_mfma_issued_frac": issued / 2500.0})
             out["results"][f"N={n} {form}"] = entry
     print(json.dumps(out), flush=True)
 

@@ -1,5 +1,7 @@
 """Hilbert-space (MFMA) path for larger Lindblad models: RK4 on the full density matrix with
-the commutator as one complex GEMM per stage on the matrix cores (``kernels/hilbert_rk4.hip``).
+the commutator as one complex GEMM per stage on the matrix cores (``kernels/hilbert_rk4.hip``),
+or, for the split-precision forms at N = 16 and 32, as both of its products
+(``kernels/hilbert_split.hip``).
 
 Applies when H = H0 + diag(v) with H0 constant and v(ParX, ParY, t) a linear combination of
 a few basis functions with numeric row coefficients, every collapse operator is constant and
@@ -255,6 +257,63 @@ def pack_a_split(m, kind):
     return hi, lo
 
 
+# The two-sided split-precision kernel (kernels/hilbert_split.hip, N = 16 and 32) runs both
+# products of the commutator on the matrix cores, so it needs no transposed copy of the state.
+# Its derivative part from H_eff is, per element (i, j) and real part P (0 = Re, 1 = Im),
+#     -i (H_eff rho)_ij + i (rho H_eff^+)_ij
+#   = sum_{Q,k} left[P][i][Q][k] rho_Q[k][j] + sum_{Q,k} rho_Q[k][i] right[P][Q][k][j],
+# the second sum using rho_ik = conj(rho_ki) (rho is Hermitian): the state values a lane holds
+# for column i of rho are, unchanged, the A operand of row i of the second product.  Since
+# right[P][Q][k][j] = (+1, -1)[P] left[P][j][Q][k], the right coefficients in the B layout are
+# the left ones in the A layout (the same registers), negated for P = 1: the kernel loads the
+# left operands only and subtracts the P = 1 right products.
+def two_sided_coefficients(m):
+    """Real coefficient arrays ``left`` [P][i][Q][k] and ``right`` [P][Q][k][j] of the
+    two-sided form (see above)."""
+    h = m.h_eff
+    n = m.n
+    left = np.zeros((2, n, 2, n))
+    left[0, :, 0, :], left[0, :, 1, :] = h.imag, h.real      # Re(-i H rho) = Im(H rho)
+    left[1, :, 0, :], left[1, :, 1, :] = -h.real, h.imag     # Im(-i H rho) = -Re(H rho)
+    right = np.zeros((2, 2, n, n))
+    ht = h.T                                                 # [k][j] = H_jk
+    right[0, 0], right[0, 1] = ht.imag, ht.real
+    right[1, 0], right[1, 1] = ht.real, -ht.imag
+    return left, right
+
+
+def pack_two_sided(m, kind):
+    """Operands of ``kernels/hilbert_split.hip`` (v_mfma_f32_16x16x32: A lane l holds
+    A[l % 16][8 (l // 16) + e], B lane l holds B[8 (l // 16) + e][l % 16], e < 8), as hi and lo
+    parts.  With NB = N / 16 blocks of 16 rows, K block kb covers the Rbig rows 16 kb .. 16 kb
+    + 15 of both real parts: K slot 8 g + e is part Q = e // 4 of row k = 16 kb + 4 g + e % 4.
+    Returns (left_hi, left_lo, right_hi, right_lo):
+      left  [P][rb][kb][lane][8]: A operand, row i = 16 rb + lane % 16 of part P;
+      right [P][kb][cb][lane][8]: B operand, column j = 16 cb + lane % 16 of part P."""
+    n = m.n
+    nb = n // 16
+    left, right = two_sided_coefficients(m)
+    lp = np.zeros((2, nb, nb, 64, 8))
+    rp = np.zeros((2, nb, nb, 64, 8))
+    for p in range(2):
+        for b1 in range(nb):
+            for kb in range(nb):
+                for lane in range(64):
+                    c, g = lane % 16, lane // 16
+                    for e in range(8):
+                        q, k = e // 4, 16 * kb + 4 * g + e % 4
+                        lp[p, b1, kb, lane, e] = left[p, 16 * b1 + c, q, k]
+                        rp[p, kb, b1, lane, e] = right[p, q, k, 16 * b1 + c]
+    lhi = _round_half(lp, kind)
+    rhi = _round_half(rp, kind)
+    return lhi, _round_half(lp - lhi, kind), rhi, _round_half(rp - rhi, kind)
+
+
+def two_sided_applies(n, split):
+    """The split-precision forms at N = 16 and 32 use the two-sided kernel."""
+    return bool(split) and n in (16, 32)
+
+
 def ell_table(m):
     """Off-diagonal nonzeros of H_eff as an ELL table: K = the largest count in a row (at
     least 1); row i's entries (column, value), padded with (i, 0)."""
@@ -346,6 +405,9 @@ def constants(m, runtime_values, split=None):
     if split:
         hi, lo = pack_a_split(m, split)
         parts.update(AHI=hi.ravel(), ALO=lo.ravel())
+    if two_sided_applies(n, split):
+        lhi, llo, _, _ = pack_two_sided(m, split)
+        parts.update(LHI=lhi.ravel(), LLO=llo.ravel())
     offs, buf, pos = {}, [], 0
     for k, v in parts.items():
         offs[k] = pos
@@ -377,6 +439,16 @@ def lds_bytes(n, fp64, block, jmax, split=None, valu=False, mono=0, sparse_k=0, 
             total += waves * 2 * pts * n * (n + 1) * 8 + mono * n * (4 + 16)
         return total
     r = 8 if fp64 else 4
+    if two_sided_applies(n, split) and not fp64:
+        # kernels/hilbert_split.hip: coefficient operands (N = 32 only), the per-wavefront
+        # populations and state copy (for its Hermitian part) and, with general jump
+        # operators, a second state copy and their row tables
+        waves, nb = block // 64, n // 16
+        total = (2 * 2 * nb * nb * 64 * 16 if nb > 1 else 0) + waves * n * 4 \
+            + waves * 2 * n * (n + 1) * 4 + nbasis * n * 4
+        if mono:
+            total += waves * 2 * n * (n + 1) * 4 + mono * n * 12
+        return total
     if mono:
         waves, pts = block // 64, 32 // n
         extra = waves * 2 * pts * n * (n + 1) * r + mono * n * (4 + 2 * r)
@@ -418,6 +490,7 @@ def emit(m, offs, mode, block_size=256, fp64=False, split=None, valu=False, trac
     if solver != "rk4" and not sparse:
         raise ValueError("the matrix-core forms integrate with RK4 only")
     split_kind = SPLIT_KINDS[split]
+    split_name = split
     atoms = {}
     names = {"ParX": "px", "ParY": "py"}
     split = [_split_time_atoms(sp.sympify(b), {TIME}, m.runtime_syms, atoms) for b in m.basis]
@@ -476,7 +549,10 @@ def emit(m, offs, mode, block_size=256, fp64=False, split=None, valu=False, trac
                "GQ_RHO0_VALUES": "GQ_RHO0_FROM_ARGS",
                "GQ_MFMA_SPLIT": split_kind}
     template = ("hilbert_sparse.hip" if sparse else
-                "hilbert_rk4_dmfma.hip" if fp64 and not valu else "hilbert_rk4.hip")
+                "hilbert_rk4_dmfma.hip" if fp64 and not valu else
+                "hilbert_split.hip" if two_sided_applies(m.n, split_kind and split_name) else
+                "hilbert_rk4.hip")
+    defines["GQ_HAS_DEP"] = int(bool(np.any(m.dep != 0)))
     table = "rk4_table.hip"
     if solver in ADAPTIVE:
         # stage times come from the step controller: atoms per stage in FP64, no table
@@ -497,11 +573,11 @@ def emit(m, offs, mode, block_size=256, fp64=False, split=None, valu=False, trac
     return "\n".join(parts)
 
 
-def points_per_wave(n, fp64, valu=False, sparse=False):
+def points_per_wave(n, fp64, valu=False, sparse=False, split=None):
     """Points per wavefront: 32 / N columns of one 32-column MFMA tile (FP32 forms, FP64 VALU),
     16 / N for the FP64 matrix-core form (one point in two column tiles at N = 32), one for the
-    sparse form."""
-    if sparse:
+    sparse form and the two-sided split-precision form."""
+    if sparse or (two_sided_applies(n, split) and not fp64):
         return 1
     if fp64 and not valu:
         return 2 if n == 8 else 1
@@ -626,7 +702,7 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
         def trial(mod, block):
             t = runtime.SweepArgs.from_buffer_copy(a)
             t.num_steps, t.warmup_steps, t.num_trace, t.trace = trial_steps, 0, 0, None
-            per = (block // 64) * points_per_wave(m.n, fp64, valu, sparse)
+            per = (block // 64) * points_per_wave(m.n, fp64, valu, sparse, split)
             return _timed_launches(ctx, mod, kname, (npts + per - 1) // per, block, t, stream,
                                    spare)
 
@@ -638,7 +714,7 @@ def run(m, *, mode, var_arrays, rho0_values, runtime_values, dt, t_in, num_steps
         module, _, _ = ctx.compile(autotune.with_block(src, block_size), options)
         ctx.read_status(status, stream, clear=True)
         tune_s = time.time() - t0
-    per_block = (block_size // 64) * points_per_wave(m.n, fp64, valu, sparse)
+    per_block = (block_size // 64) * points_per_wave(m.n, fp64, valu, sparse, split)
     if prepare:
         launches = []
         if npts > 0:

@@ -1,0 +1,566 @@
+// RK4 on the full density matrix with both products of the commutator on the matrix cores, in
+// split precision (N = 16 and 32): FP16 or BF16 hi/lo parts, FP32 accumulation.
+//
+// Model structure, stage times and output semantics are those of hilbert_rk4.hip (H = H0 +
+// diag(v(ParX, ParY, t)), diagonal / single-entry / one-nonzero-per-row collapse operators,
+// diagonal observable; gqis/cuda_solvers.py:470-517, gqis/solver.py:2126-2152).  With
+// H_eff = H0 - (i/2) sum L^+ L,
+//     d rho/dt = -i H_eff rho + i rho H_eff^+ - i [diag(v), rho] + sum_k d_k d_k^T o rho + jumps.
+// The first two terms are two real products per stage on v_mfma_f32_16x16x32_{f16,bf16}:
+//     left:  coefficients (A, 16 x 32 per K block) x state (B, 32 x 16)
+//     right: state (A) x coefficients (B)
+// The right product uses rho_ik = conj(rho_ki) (rho is Hermitian): the values a lane holds for
+// column i of rho are, as they are, the A operand of row i, with the conjugation folded into
+// the coefficients (packed on the host, hilbert.pack_two_sided).  The right coefficients in the
+// B layout are the left ones in the A layout, negated for the imaginary part: one set of
+// operands serves both products, and the imaginary part's right products accumulate apart and
+// are subtracted.  Both products of a stage land in the layout of the state: no transposed
+// copy, no data movement between stages.  The three products per side are lo x hi + hi x lo +
+// hi x hi (lo x lo dropped: FP16 parts carry 11 + 11 significant bits, BF16 8 + 8).
+//
+// One point per wavefront.  Lane (c, g) = (l % 16, l / 16), NB = N / 16 blocks: register r of
+// tile (P, rb, cb) holds part P (0 Re, 1 Im) of rho[16 rb + 4 g + r][16 cb + c] (the
+// instruction's result layout).  As an operand, K block kb of column block cb is element
+// e = part e / 4 of row 16 kb + 4 g + e % 4: the lane's registers (e / 4, kb, cb, e % 4).
+
+typedef float gq_f32x4 __attribute__((ext_vector_type(4)));
+#if GQ_MFMA_SPLIT == 1
+typedef _Float16 gq_half;
+#define GQ_MFMA16(a, b, c) __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0)
+#define GQ_HILBERT_NAME gq_hilbert_rk4_f16x3
+#else
+typedef __bf16 gq_half;
+#define GQ_MFMA16(a, b, c) __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0)
+#define GQ_HILBERT_NAME gq_hilbert_rk4_bf16x3
+#endif
+typedef gq_half gq_half8 __attribute__((ext_vector_type(8)));
+typedef gq_half gq_half2 __attribute__((ext_vector_type(2)));
+typedef float gq_f32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned gq_u32x4 __attribute__((ext_vector_type(4)));
+
+// hi / lo parts of two FP32 values, packed: hi = RNE(v), lo = RNE(v - hi) (v - hi is exact in
+// FP32).  FP16: one packed conversion and two mixed-precision FMAs that round fma(hi, -1, v)
+// once to FP16 (three instructions for two values, against six for converting hi back).
+__device__ __forceinline__ void gq_split2(float v0, float v1, unsigned& hi, unsigned& lo) {
+    const gq_half2 h = __builtin_convertvector((gq_f32x2){v0, v1}, gq_half2);
+    hi = __builtin_bit_cast(unsigned, h);
+#if GQ_MFMA_SPLIT == 1
+    unsigned l;
+    asm("v_fma_mixlo_f16 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "=v"(l) : "v"(hi), "v"(v0));
+    asm("v_fma_mixhi_f16 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "+v"(l) : "v"(hi), "v"(v1));
+    lo = l;
+#else
+    const gq_half2 l = __builtin_convertvector((gq_f32x2){v0 - (float)h[0], v1 - (float)h[1]}, gq_half2);
+    lo = __builtin_bit_cast(unsigned, l);
+#endif
+}
+
+#define HN GQ_HN
+#define NB (HN / 16)              // 16-row (and 16-column) blocks
+#define NX (8 * NB * NB)          // state values per lane
+#define NOPS (2 * NB * NB)        // coefficient operands: (P, block, K block)
+#define HWAVES (GQ_BLOCK / 64)
+#define HLD (HN + 1)              // padded row of the LDS state copy
+// N = 16: the four coefficient operands (hi and lo) stay in registers; N = 32: 16, in LDS
+#define GQ_CONST_LDS (NB > 1)
+#ifndef GQ_HAS_DEP
+#define GQ_HAS_DEP 1
+#endif
+
+__device__ __forceinline__ constexpr int gq_xi(int p, int rb, int cb, int r) {
+    return ((p * NB + rb) * NB + cb) * 4 + r;
+}
+__device__ __forceinline__ constexpr int gq_op(int p, int b, int kb) { return (p * NB + b) * NB + kb; }
+
+// Wavefronts per SIMD: two at N = 32 (at most 256 registers), so that one wavefront's
+// element-wise work issues while the other's products run, four at N = 16.  The bound also
+// keeps the products' accumulators in VGPRs: without it the compiler puts them in AGPRs and
+// copies them in and out around every product chain.
+#ifndef GQ_SPLIT_OCCUPANCY
+#if NB > 1
+#define GQ_SPLIT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(2)))
+#else
+#define GQ_SPLIT_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4)))
+#endif
+#endif
+extern "C" __global__ __launch_bounds__(GQ_BLOCK) GQ_SPLIT_OCCUPANCY void GQ_KERNEL(GQ_HILBERT_NAME)(const gq_sweep_args a) {
+#if GQ_CONST_LDS
+    __shared__ gq_half8 s_left[2][NOPS][64];    // [hi, lo][(P, block, kb)][lane]
+#endif
+    __shared__ real s_diag[HWAVES][HN];         // populations, for the jump terms
+    __shared__ real s_t[HWAVES][2][HN * HLD];   // the state, for its Hermitian part
+    __shared__ real s_vco[GQ_NBASIS * HN];      // drive row coefficients per basis term
+#if GQ_NMONO
+    // general jump operators (one nonzero per row): a per-wavefront copy of the state and the
+    // operators' row tables, (L rho L^+)_ij = l_i conj(l_j) rho[col_i][col_j]
+    __shared__ real s_xm[HWAVES][2][HN * HLD];
+    __shared__ int s_mcol[GQ_NMONO * HN];
+    __shared__ real s_mre[GQ_NMONO * HN];
+    __shared__ real s_mim[GQ_NMONO * HN];
+#endif
+    const real* __restrict__ cst = (const real*)a.consts;   // packed operator constants
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+#if GQ_CONST_LDS
+    // the FP32 constants hold FP16 / BF16 values exactly
+    for (int e = threadIdx.x; e < NOPS * 64 * 8; e += GQ_BLOCK) {
+        ((gq_half*)s_left[0])[e] = (gq_half)cst[GQ_OFF_LHI + e];
+        ((gq_half*)s_left[1])[e] = (gq_half)cst[GQ_OFF_LLO + e];
+    }
+#endif
+    for (int e = threadIdx.x; e < GQ_NBASIS * HN; e += GQ_BLOCK) s_vco[e] = cst[GQ_OFF_VCOEF + e];
+#if GQ_NMONO
+    for (int e = threadIdx.x; e < GQ_NMONO * HN; e += GQ_BLOCK) {
+        s_mcol[e] = (int)cst[GQ_OFF_MCOL + e];
+        s_mre[e] = cst[GQ_OFF_MRE + e];
+        s_mim[e] = cst[GQ_OFF_MIM + e];
+    }
+#endif
+    __syncthreads();
+
+    // one point per wavefront: a wavefront past the last point has nothing to do (no barrier
+    // follows)
+    const long long q = (long long)blockIdx.x * HWAVES + wave;
+    if (q >= a.point_count) return;
+    const int c = lane & 15, g = lane >> 4;
+    const gq_point pt = gq_locate(a, q);
+    const real px = ((const real*)a.par_x)[pt.ix];
+    const real py = ((const real*)a.par_y)[pt.iy];
+    real* __restrict__ dbuf = s_diag[wave];
+    real* __restrict__ t_r = s_t[wave][0];
+    real* __restrict__ t_i = s_t[wave][1];
+#if GQ_NMONO
+    real* __restrict__ xm_r = s_xm[wave][0];
+    real* __restrict__ xm_i = s_xm[wave][1];
+#endif
+
+#if !GQ_CONST_LDS
+    gq_half8 lop[2][NOPS];                      // [hi, lo][operand]
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int o = 0; o < NOPS; ++o)
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                lop[h][o][e] = (gq_half)cst[(h ? GQ_OFF_LLO : GQ_OFF_LHI) + (o * 64 + lane) * 8 + e];
+#define GQ_LOP(h, o) lop[h][o]
+#else
+#define GQ_LOP(h, o) s_left[h][o][lane]
+#endif
+
+    // the lanes with g == c / 4 hold the diagonal element of column c in register c % 4 of
+    // the tiles (0, b, b)
+    const bool dlane = (g == (c >> 2));
+    const int rd = c & 3;
+
+    // ---- initial state: rho0 values (reduced layout) or |0><0| ------------------------------
+    real y[NX];
+#pragma unroll
+    for (int rb = 0; rb < NB; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < NB; ++cb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 16 * rb + 4 * g + r, j = 16 * cb + c;
+                real re = R(0.0), im = R(0.0);
+#if GQ_RHO0_VALUES
+                const real* __restrict__ r0 = (const real*)a.rho0 + pt.q * (HN * HN - 1);
+                if (i == j) {
+                    if (i < HN - 1) {
+                        re = r0[i];
+                    } else {
+                        real s = R(0.0);
+                        for (int k = 0; k < HN - 1; ++k) s += r0[k];
+                        re = R(1.0) - s;
+                    }
+                } else {
+                    const int lo = i < j ? i : j, hi = i < j ? j : i;
+                    const int u = lo * HN - (lo * (lo + 1)) / 2 + (hi - lo - 1);
+                    re = r0[HN - 1 + 2 * u];
+                    im = (i < j ? R(1.0) : R(-1.0)) * r0[HN - 1 + 2 * u + 1];
+                }
+#else
+                re = (i == 0 && j == 0) ? R(1.0) : R(0.0);
+#endif
+                y[gq_xi(0, rb, cb, r)] = re;
+                y[gq_xi(1, rb, cb, r)] = im;
+            }
+
+#if GQ_HAS_DEP
+    real depw[NB][NB][4];   // dephasing weights sum_k d_k(i) d_k(j)
+#pragma unroll
+    for (int rb = 0; rb < NB; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < NB; ++cb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                depw[rb][cb][r] = cst[GQ_OFF_DEP + (16 * rb + 4 * g + r) * HN + 16 * cb + c];
+#endif
+    // population transfer into the lane's diagonal elements (zero weights elsewhere) and the
+    // observable's weight of those elements
+    real jwl[NB][GQ_JMAX], od[NB];
+    int jbl[NB][GQ_JMAX];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+        const int i = 16 * b + c;
+        od[b] = dlane ? cst[GQ_OFF_OBS + i] : R(0.0);
+#pragma unroll
+        for (int e = 0; e < GQ_JMAX; ++e) {
+            jwl[b][e] = dlane ? cst[GQ_OFF_JW + i * GQ_JMAX + e] : R(0.0);
+            jbl[b][e] = dlane ? (int)cst[GQ_OFF_JB + i * GQ_JMAX + e] : 0;
+        }
+    }
+#if GQ_NMONO
+    int mcj[GQ_NMONO][NB];     // the general jump operators at the lane's columns
+    real mlr[GQ_NMONO][NB], mli[GQ_NMONO][NB];
+#pragma unroll
+    for (int kk = 0; kk < GQ_NMONO; ++kk)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            mcj[kk][b] = s_mcol[kk * HN + 16 * b + c];
+            mlr[kk][b] = s_mre[kk * HN + 16 * b + c];
+            mli[kk][b] = s_mim[kk * HN + 16 * b + c];
+        }
+#endif
+
+    const real dt = (real)a.dt;
+    const real dt2 = dt * R(0.5);
+    const real dt6 = dt / R(6.0);
+    const int num_steps = a.num_steps;
+    real wdr[NB][NB][4];      // v_i - v_j of the lane's elements at the stage time
+
+    auto stage_values = [&](int row) {
+        const real* __restrict__ tab = (const real*)a.time_table;
+        real TT[GQ_NATOMS_ALLOC];
+#pragma unroll
+        for (int k = 0; k < GQ_NATOMS; ++k) TT[k] = tab[row * GQ_NATOMS + k];
+        real gb[GQ_NBASIS];
+        gq_basis(px, py, TT, cst, gb);
+        // the diagonal drive v_i = sum_k coef[k][i] g_k of the lane's rows and columns
+        real vr[NB][4], vc[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                real s = R(0.0);
+#pragma unroll
+                for (int k = 0; k < GQ_NBASIS; ++k) s = fma(s_vco[k * HN + 16 * b + 4 * g + r], gb[k], s);
+                vr[b][r] = s;
+            }
+            real s = R(0.0);
+#pragma unroll
+            for (int k = 0; k < GQ_NBASIS; ++k) s = fma(s_vco[k * HN + 16 * b + c], gb[k], s);
+            vc[b] = s;
+        }
+#pragma unroll
+        for (int rb = 0; rb < NB; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < NB; ++cb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) wdr[rb][cb][r] = vr[rb][r] - vc[cb];
+    };
+
+    // the RHS at the stage input x, written over x.  The element-wise terms initialise the
+    // accumulators, so x is dead once split and the products accumulate in its place.
+    auto rhs = [&](real* x) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            real xd = R(0.0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) xd = (r == rd) ? x[gq_xi(0, b, b, r)] : xd;
+            if (dlane) dbuf[16 * b + c] = xd;
+        }
+#if GQ_NMONO
+#pragma unroll
+        for (int rb = 0; rb < NB; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < NB; ++cb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int own = (16 * rb + 4 * g + r) * HLD + 16 * cb + c;
+                    xm_r[own] = x[gq_xi(0, rb, cb, r)];
+                    xm_i[own] = x[gq_xi(1, rb, cb, r)];
+                }
+#endif
+        // hi / lo parts of the state, operand blocks [kb][cb]
+        gq_half8 sh[NB][NB], sl[NB][NB];
+#pragma unroll
+        for (int kb = 0; kb < NB; ++kb)
+#pragma unroll
+            for (int cb = 0; cb < NB; ++cb) {
+                gq_u32x4 hw, lw;
+#pragma unroll
+                for (int e2 = 0; e2 < 4; ++e2) {
+                    const int e = 2 * e2;
+                    unsigned hv, lv;
+                    gq_split2(x[gq_xi(e >> 2, kb, cb, e & 3)], x[gq_xi(e >> 2, kb, cb, (e & 3) + 1)], hv, lv);
+                    hw[e2] = hv;
+                    lw[e2] = lv;
+                }
+                sh[kb][cb] = __builtin_bit_cast(gq_half8, hw);
+                sl[kb][cb] = __builtin_bit_cast(gq_half8, lw);
+            }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+        real jsum[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            real s = R(0.0);
+#pragma unroll
+            for (int e = 0; e < GQ_JMAX; ++e) s = fma(jwl[b][e], dbuf[jbl[b][e]], s);
+            jsum[b] = s;
+        }
+        // element-wise terms: drive -i (v_i - v_j) rho_ij, dephasing, jumps; they start the
+        // accumulators of the products
+        gq_f32x4 acc[2][NB][NB];
+#pragma unroll
+        for (int rb = 0; rb < NB; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < NB; ++cb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const real xr = x[gq_xi(0, rb, cb, r)];
+                    const real xi = x[gq_xi(1, rb, cb, r)];
+                    const real w = wdr[rb][cb][r];
+                    real dr = w * xi;
+                    real di = -w * xr;
+#if GQ_HAS_DEP
+                    dr = fma(depw[rb][cb][r], xr, dr);
+                    di = fma(depw[rb][cb][r], xi, di);
+#endif
+#if GQ_NMONO
+                    const int i = 16 * rb + 4 * g + r;
+#pragma unroll
+                    for (int kk = 0; kk < GQ_NMONO; ++kk) {
+                        const int ci = s_mcol[kk * HN + i];
+                        if (ci >= 0 && mcj[kk][cb] >= 0) {
+                            // w = l_i conj(l_j), times rho[col_i][col_j]
+                            const real lr = s_mre[kk * HN + i], li = s_mim[kk * HN + i];
+                            const real wr = fma(lr, mlr[kk][cb], li * mli[kk][cb]);
+                            const real wi = fma(li, mlr[kk][cb], -lr * mli[kk][cb]);
+                            const real pr = xm_r[ci * HLD + mcj[kk][cb]];
+                            const real pim = xm_i[ci * HLD + mcj[kk][cb]];
+                            dr = fma(wr, pr, fma(-wi, pim, dr));
+                            di = fma(wr, pim, fma(wi, pr, di));
+                        }
+                    }
+#endif
+                    if (rb == cb) dr += (dlane && r == rd) ? jsum[rb] : R(0.0);
+                    acc[0][rb][cb][r] = dr;
+                    acc[1][rb][cb][r] = di;
+                }
+        // The two products in groups (kb, P): the coefficient operands of one group (one per
+        // block, hi and lo) serve 6 NB^2 products, the left ones (as A operands, one per row
+        // block) and the right ones (as B operands, one per column block).  At N = 32 they come
+        // from LDS one group ahead, so that their latency hides behind the previous group's
+        // products.  The wavefront issues the products at raised priority: the other
+        // wavefront on the SIMD then fills the matrix core's busy cycles with its element-wise
+        // work.
+        __builtin_amdgcn_s_setprio(1);
+        gq_f32x4 accn[NB][NB];   // right products of the imaginary part, subtracted
+#pragma unroll
+        for (int rb = 0; rb < NB; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < NB; ++cb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) accn[rb][cb][r] = 0.0f;
+        gq_half8 ob[2][NB][2];
+        auto group_ops = [&](int gi, gq_half8 (&o)[NB][2]) {
+            const int kb = gi >> 1, p = gi & 1;
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int hl = 0; hl < 2; ++hl) o[b][hl] = GQ_LOP(hl, gq_op(p, b, kb));
+        };
+        auto group_products = [&](int gi, const gq_half8 (&o)[NB][2]) {
+            const int kb = gi >> 1, p = gi & 1;
+            // terms: coefficient lo x state hi, hi x lo, hi x hi
+#pragma unroll
+            for (int t = 0; t < 3; ++t)
+#pragma unroll
+                for (int rb = 0; rb < NB; ++rb)
+#pragma unroll
+                    for (int cb = 0; cb < NB; ++cb) {
+                        acc[p][rb][cb] = GQ_MFMA16(o[rb][t == 0 ? 1 : 0], t == 1 ? sl[kb][cb] : sh[kb][cb],
+                                                   acc[p][rb][cb]);
+                        const gq_half8 sa = t == 1 ? sl[kb][rb] : sh[kb][rb];
+                        if (p == 0)
+                            acc[0][rb][cb] = GQ_MFMA16(sa, o[cb][t == 0 ? 1 : 0], acc[0][rb][cb]);
+                        else
+                            accn[rb][cb] = GQ_MFMA16(sa, o[cb][t == 0 ? 1 : 0], accn[rb][cb]);
+                    }
+        };
+        group_ops(0, ob[0]);
+#pragma unroll
+        for (int gi = 0; gi < 2 * NB; ++gi) {
+            if (gi + 1 < 2 * NB) group_ops(gi + 1, ob[(gi + 1) & 1]);
+#if GQ_CONST_LDS
+            __builtin_amdgcn_sched_barrier(0);
+#endif
+            group_products(gi, ob[gi & 1]);
+#if GQ_CONST_LDS
+            __builtin_amdgcn_sched_barrier(0);
+#endif
+        }
+        __builtin_amdgcn_s_setprio(0);
+#pragma unroll
+        for (int p = 0; p < 2; ++p)
+#pragma unroll
+            for (int rb = 0; rb < NB; ++rb)
+#pragma unroll
+                for (int cb = 0; cb < NB; ++cb)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r)
+                        x[gq_xi(p, rb, cb, r)] = p == 0 ? acc[0][rb][cb][r] : acc[1][rb][cb][r] - accn[rb][cb][r];
+    };
+
+    // The state after each step is replaced by its Hermitian part (rho + rho^+) / 2, which is
+    // exactly Hermitian (element (j, i) takes the same two values in the other order).  The
+    // right product reads conj(rho_ki) for rho_ik, and the matrix cores round the two sides
+    // of one element in different orders: left alone, the anti-Hermitian part of that rounding
+    // builds up over the steps (the products of rho_ik and conj(rho_ki) do not damp it) and
+    // reaches 1.8e-7 after 2000 steps, with 2.1e-6 error in rho at N = 16.  The reference's
+    // reduced state holds the upper triangle only, Hermitian by construction.
+    auto hermitian_part = [&](real* v) {
+#pragma unroll
+        for (int rb = 0; rb < NB; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < NB; ++cb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int own = (16 * rb + 4 * g + r) * HLD + 16 * cb + c;
+                    t_r[own] = v[gq_xi(0, rb, cb, r)];
+                    t_i[own] = v[gq_xi(1, rb, cb, r)];
+                }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+#pragma unroll
+        for (int rb = 0; rb < NB; ++rb)
+#pragma unroll
+            for (int cb = 0; cb < NB; ++cb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int tr = (16 * cb + c) * HLD + 16 * rb + 4 * g + r;
+                    v[gq_xi(0, rb, cb, r)] = R(0.5) * (v[gq_xi(0, rb, cb, r)] + t_r[tr]);
+                    v[gq_xi(1, rb, cb, r)] = R(0.5) * (v[gq_xi(1, rb, cb, r)] - t_i[tr]);
+                }
+        // the buffer is written again only after these reads (in program order, per lane)
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    };
+
+    // the lane's diagonal elements weighted by the observable (zero outside the diagonal lanes)
+    auto observed = [&](const real* v) -> real {
+        real s = R(0.0);
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            real yd = R(0.0);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) yd = (r == rd) ? v[gq_xi(0, b, b, r)] : yd;
+            s = fma(od[b], yd, s);
+        }
+        return s;
+    };
+    // sum over the wavefront (every lane ends with the total)
+    auto wave_sum = [&](double v) -> double {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        return v;
+    };
+    bool bad_trace = false;
+    (void)bad_trace;
+
+    stage_values(0);
+    real kb[NX];   // k1 of a step, then each stage input and, in place, its derivative
+#pragma unroll
+    for (int i = 0; i < NX; ++i) kb[i] = y[i];
+    rhs(kb);
+    double avg = 0.0;
+    for (int step = 0; step < num_steps; ++step) {
+        stage_values(2 * step + 1);
+        real sum[NX];
+#pragma unroll
+        for (int i = 0; i < NX; ++i) {
+            sum[i] = kb[i];
+            kb[i] = fma(dt2, kb[i], y[i]);
+        }
+        rhs(kb);
+#pragma unroll
+        for (int i = 0; i < NX; ++i) {
+            sum[i] = fma(R(2.0), kb[i], sum[i]);
+            kb[i] = fma(dt2, kb[i], y[i]);
+        }
+        rhs(kb);
+#pragma unroll
+        for (int i = 0; i < NX; ++i) {
+            sum[i] = fma(R(2.0), kb[i], sum[i]);
+            kb[i] = fma(dt, kb[i], y[i]);
+        }
+        stage_values(2 * step + 2);
+        rhs(kb);
+#pragma unroll
+        for (int i = 0; i < NX; ++i) {
+            sum[i] += kb[i];
+            y[i] = fma(dt6, sum[i], y[i]);
+        }
+        hermitian_part(y);
+        if (step + 1 < num_steps) {
+#pragma unroll
+            for (int i = 0; i < NX; ++i) kb[i] = y[i];
+            rhs(kb);
+        }
+#if GQ_OUT_MEAN
+        if (step >= a.warmup_steps) avg += (double)observed(y);
+#endif
+#if GQ_TRACE
+        // observable trace after steps offset + k * stride (gqis/solver.py:2177-2193)
+        if (step >= a.trace_offset && (step - a.trace_offset) % a.trace_stride == 0) {
+            const int kk = (step - a.trace_offset) / a.trace_stride;
+            if (kk < a.num_trace) {
+                const double s = wave_sum((double)observed(y));
+                if (lane == 0) {
+                    const real r = (real)s;
+                    bad_trace |= !gq_finite(r);
+                    gq_store_complex(a.trace, (long long)kk * a.trace_point_stride + q, r, R(0.0));
+                }
+            }
+        }
+#endif
+    }
+
+    bool bad = false;
+#if GQ_OUT_MEAN || GQ_OUT_FINAL
+#if GQ_OUT_FINAL
+    avg = (double)observed(y);
+#endif
+    double s = wave_sum(avg);
+    if (lane == 0) {
+#if GQ_OUT_MEAN
+        s /= (double)(num_steps - a.warmup_steps);
+#endif
+        const real r = (real)s;
+        bad = !gq_finite(r);
+        gq_store_complex(a.result, q, r, R(0.0));
+    }
+#elif GQ_OUT_FINAL_RHO
+#pragma unroll
+    for (int rb = 0; rb < NB; ++rb)
+#pragma unroll
+        for (int cb = 0; cb < NB; ++cb)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 16 * rb + 4 * g + r, j = 16 * cb + c;
+                const real yr = y[gq_xi(0, rb, cb, r)], yi = y[gq_xi(1, rb, cb, r)];
+                if (i == j && i < HN - 1) {
+                    ((real*)a.result)[(long long)i * a.point_count + q] = yr;
+                    bad |= !gq_finite(yr);
+                } else if (i < j) {
+                    const int idx = HN - 1 + 2 * (i * HN - (i * (i + 1)) / 2 + (j - i - 1));
+                    ((real*)a.result)[(long long)idx * a.point_count + q] = yr;
+                    ((real*)a.result)[(long long)(idx + 1) * a.point_count + q] = yi;
+                    bad |= !(gq_finite(yr) && gq_finite(yi));
+                }
+            }
+#endif
+    gq_flag(a.status, bad, GQ_STATUS_NONFINITE_RESULT);
+#if GQ_TRACE
+    gq_flag(a.status, bad_trace, GQ_STATUS_NONFINITE_TRACE);
+#endif
+}

@@ -128,7 +128,7 @@ class SweepServer:
 
             def variant(c):
                 return (autotune.with_block(src, c[1]), kname,
-                        (c[1] // 64) * hilbert.points_per_wave(hm.n, r.fp64))
+                        (c[1] // 64) * hilbert.points_per_wave(hm.n, r.fp64, split=split))
             consts_vals = cvals
             table_kernel = codegen.kernel_name("gq_rk4_time_table", r.fp64)
             rows = 2 * num_steps + 1

@@ -43,15 +43,18 @@ def test_chain_mfma_matches_per_lane_kernel(gpu):
     assert np.max(np.abs(mfma - lanes64)[ok]) <= 5e-5
 
 
-def test_mfma_continuation_from_final_state(gpu):
-    w = workloads.random_n_level(16, seed=SEED, n_eps=2, n_amp=2, n_t=101, t_end=1.0)
+@pytest.mark.parametrize("n,liouville", [(16, "mfma"), (16, "mfma_f16"), (32, "mfma_f16")])
+def test_mfma_continuation_from_final_state(gpu, n, liouville):
+    # a sweep continued from its final state (rho0 values, t_in) equals the whole sweep; the
+    # split forms read the initial state in their own lane layout
+    w = workloads.random_n_level(n, seed=SEED, n_eps=2, n_amp=2, n_t=101, t_end=1.0)
     args = (w["H"], w["Drive"], w["Col_Ops"], w["mean_operator"])
-    first = mesolve_2D(*args, w["tlist"], var_arrays=w["var_arrays"], liouville="mfma",
+    first = mesolve_2D(*args, w["tlist"], var_arrays=w["var_arrays"], liouville=liouville,
                        output_mode="final_rho")
-    second = mesolve_2D(*args, w["tlist"], var_arrays=w["var_arrays"], liouville="mfma",
+    second = mesolve_2D(*args, w["tlist"], var_arrays=w["var_arrays"], liouville=liouville,
                         output_mode="final_rho", rho0_values=first, t_in=1.0)
     whole = mesolve_2D(*args, np.linspace(0.0, 2.0, 201), var_arrays=w["var_arrays"],
-                       liouville="mfma", output_mode="final_rho")
+                       liouville=liouville, output_mode="final_rho")
     np.testing.assert_allclose(second, whole, atol=2e-5)
 
 

@@ -137,3 +137,80 @@ def test_fp32_and_split_matrix_core_packing(n):
                     assert abs(acc[8 * (e // 4) + 4 * (lane // 32) + e % 4, lane % 32] - want) < 1e-12
                     assert abs(acc_split[8 * (e // 4) + 4 * (lane // 32) + e % 4, lane % 32]
                                - want) < 1e-5 * max(1.0, np.max(np.abs(xb)))
+
+
+def _mfma16(a, b):
+    """v_mfma_f32_16x16x32 on lane-held operands a, b [64][8]: A lane l = A[l % 16][8 (l // 16)
+    + e], B lane l = B[8 (l // 16) + e][l % 16]; returns D [16][16]."""
+    A = np.zeros((16, 32))
+    B = np.zeros((32, 16))
+    for lane in range(64):
+        for e in range(8):
+            A[lane % 16, 8 * (lane // 16) + e] = a[lane, e]
+            B[8 * (lane // 16) + e, lane % 16] = b[lane, e]
+    return A @ B
+
+
+@pytest.mark.parametrize("n", [16, 32])
+@pytest.mark.parametrize("kind", ["f16", "bf16"])
+def test_two_sided_split_packing(n, kind):
+    # kernels/hilbert_split.hip: lane (c, g) = (l % 16, l // 16) holds element (16 rb + 4 g + r,
+    # 16 cb + c) of each real part in register r of tile (P, rb, cb); the state values of K
+    # block kb and column block cb are its B operand, and -- for a Hermitian rho -- the A
+    # operand of the second product's row block cb.  The second product's coefficients (B
+    # operands) are the first's (A operands) for P = 0 and their negation for P = 1, so the
+    # kernel loads the first set only and subtracts the P = 1 second products.  Both products
+    # together must give -i H_eff rho + i rho H_eff^+ in the state's layout.
+    m = _model(n)
+    rng = np.random.default_rng(n + 7)
+    z = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    rho = (z + z.conj().T) / (2 * n)
+    want = -1j * (m.h_eff @ rho) + 1j * (rho @ m.h_eff.conj().T)
+    want_p = (want.real, want.imag)
+    rho_p = (rho.real, rho.imag)
+    nb = n // 16
+    lhi, llo, rhi, rlo = hilbert.pack_two_sided(m, kind)
+    for kb in range(nb):
+        for b in range(nb):
+            np.testing.assert_array_equal(rhi[0, kb, b], lhi[0, b, kb])
+            np.testing.assert_array_equal(rlo[0, kb, b], llo[0, b, kb])
+            np.testing.assert_array_equal(rhi[1, kb, b], -lhi[1, b, kb])
+            np.testing.assert_array_equal(rlo[1, kb, b], -llo[1, b, kb])
+    half = (np.float16 if kind == "f16" else None)
+
+    def rnd(x):
+        return x.astype(half).astype(np.float64) if half else hilbert._round_half(x, kind)
+    # S[kb][cb][lane][e]: part e // 4, row 16 kb + 4 g + e % 4, column 16 cb + c
+    s = np.zeros((nb, nb, 64, 8))
+    for kb in range(nb):
+        for cb in range(nb):
+            for lane in range(64):
+                c, g = lane % 16, lane // 16
+                for e in range(8):
+                    s[kb, cb, lane, e] = rho_p[e // 4][16 * kb + 4 * g + e % 4, 16 * cb + c]
+    sh = rnd(s)
+    sl = rnd(s - sh)
+    scale = np.max(np.abs(want))
+    sign = (1.0, -1.0)
+    for p in range(2):
+        for rb in range(nb):
+            for cb in range(nb):
+                exact = np.zeros((16, 16))
+                split = np.zeros((16, 16))
+                for kb in range(nb):
+                    lf = lhi[p, rb, kb] + llo[p, rb, kb]     # first product, row block rb
+                    rf = lhi[p, cb, kb] + llo[p, cb, kb]     # second product, column block cb
+                    exact += _mfma16(lf, s[kb, cb]) + sign[p] * _mfma16(s[kb, rb], rf)
+                    split += (_mfma16(llo[p, rb, kb], sh[kb, cb]) + _mfma16(lhi[p, rb, kb], sl[kb, cb])
+                              + _mfma16(lhi[p, rb, kb], sh[kb, cb]))
+                    split += sign[p] * (_mfma16(sh[kb, rb], llo[p, cb, kb]) + _mfma16(sl[kb, rb], lhi[p, cb, kb])
+                                        + _mfma16(sh[kb, rb], lhi[p, cb, kb]))
+                for lane in range(64):
+                    c, g = lane % 16, lane // 16
+                    for r in range(4):
+                        w = want_p[p][16 * rb + 4 * g + r, 16 * cb + c]
+                        # D[4 g + r][c] lands in register r of lane (c, g); hi + lo keep 22
+                        # (FP16) or 16 (BF16) significant bits
+                        tol = (2e-6 if kind == "f16" else 5e-5) * scale
+                        assert abs(exact[4 * g + r, c] - w) < tol
+                        assert abs(split[4 * g + r, c] - w) < tol
