@@ -633,9 +633,11 @@ def bench_scaling(args):
                 if form not in ("mfma", "mfma64"):
                     # three half-precision products per FP32-equivalent product; from 16
                     # levels the two-sided kernel runs both sides of the commutator on the
-                    # matrix cores (twice the products of H_eff rho alone)
+                    # matrix cores (twice the products of H_eff rho alone) for the 16 x 16
+                    # blocks on and above the diagonal (3 of 4 at 32 levels)
                     sides = 2 if n >= 16 else 1
-                    issued = sides * 3 * 4 * 8 * n ** 3 * ps / k / 1e12
+                    blocks = {32: 0.75}.get(n, 1.0)
+                    issued = sides * blocks * 3 * 4 * 8 * n ** 3 * ps / k / 1e12
                     entry.update({"half_mfma_issued_tflops": issued,
                                   "half_mfma_issued_frac": issued / 2500.0})
             out["results"][f"N={n} {form}"] = entry

@@ -14,14 +14,17 @@
 // the coefficients (packed on the host, hilbert.pack_two_sided).  The right coefficients in the
 // B layout are the left ones in the A layout, negated for the imaginary part: one set of
 // operands serves both products, and the imaginary part's right products accumulate apart and
-// are subtracted.  Both products of a stage land in the layout of the state: no transposed
-// copy, no data movement between stages.  The three products per side are lo x hi + hi x lo +
-// hi x hi (lo x lo dropped: FP16 parts carry 11 + 11 significant bits, BF16 8 + 8).
+// are subtracted.  Both products of a stage land in the layout of the state: no data movement
+// between stages.  The three products per side are lo x hi + hi x lo + hi x hi (lo x lo
+// dropped: FP16 parts carry 11 + 11 significant bits, BF16 8 + 8).
 //
-// One point per wavefront.  Lane (c, g) = (l % 16, l / 16), NB = N / 16 blocks: register r of
-// tile (P, rb, cb) holds part P (0 Re, 1 Im) of rho[16 rb + 4 g + r][16 cb + c] (the
-// instruction's result layout).  As an operand, K block kb of column block cb is element
-// e = part e / 4 of row 16 kb + 4 g + e % 4: the lane's registers (e / 4, kb, cb, e % 4).
+// One point per wavefront, in NB x NB blocks of 16 x 16 (NB = N / 16).  Lane (c, g) = (l % 16,
+// l / 16): register r of block (rb, cb), part P (0 Re, 1 Im), holds rho[16 rb + 4 g + r][16 cb
+// + c] (the instruction's result layout).  As an operand, K block kb of column block cb is
+// element e = part e / 4 of row 16 kb + 4 g + e % 4: the lane's registers of block (kb, cb).
+// Only the blocks rb <= cb are integrated: the derivative is Hermitian, so block (1, 0) is the
+// conjugate transpose of (0, 1), and at N = 32 it is rebuilt from (0, 1) through LDS once per
+// RHS where the products need it as an operand -- 72 instead of 96 products per RHS.
 
 typedef float gq_f32x4 __attribute__((ext_vector_type(4)));
 #if GQ_MFMA_SPLIT == 1
@@ -55,21 +58,39 @@ __device__ __forceinline__ void gq_split2(float v0, float v1, unsigned& hi, unsi
 #endif
 }
 
+// hi / lo operands of one block from its 8 values (Re r = 0..3, Im r = 0..3)
+__device__ __forceinline__ void gq_split_block(const float (&v)[2][4], gq_half8& hi, gq_half8& lo) {
+    gq_u32x4 hw, lw;
+#pragma unroll
+    for (int e2 = 0; e2 < 4; ++e2) {
+        unsigned hv, lv;
+        gq_split2(v[e2 >> 1][2 * (e2 & 1)], v[e2 >> 1][2 * (e2 & 1) + 1], hv, lv);
+        hw[e2] = hv;
+        lw[e2] = lv;
+    }
+    hi = __builtin_bit_cast(gq_half8, hw);
+    lo = __builtin_bit_cast(gq_half8, lw);
+}
+
 #define HN GQ_HN
 #define NB (HN / 16)              // 16-row (and 16-column) blocks
-#define NX (8 * NB * NB)          // state values per lane
+#define NBLK (NB * (NB + 1) / 2)  // integrated blocks: rb <= cb
+#define NX (8 * NBLK)             // state values per lane
 #define NOPS (2 * NB * NB)        // coefficient operands: (P, block, K block)
 #define HWAVES (GQ_BLOCK / 64)
-#define HLD (HN + 1)              // padded row of the LDS state copy
+#define HLD (HN + 1)              // padded row of the LDS state copies
 // N = 16: the four coefficient operands (hi and lo) stay in registers; N = 32: 16, in LDS
 #define GQ_CONST_LDS (NB > 1)
 #ifndef GQ_HAS_DEP
 #define GQ_HAS_DEP 1
 #endif
+static_assert(NB == 1 || NB == 2, "the two-sided kernel covers N = 16 and 32");
 
-__device__ __forceinline__ constexpr int gq_xi(int p, int rb, int cb, int r) {
-    return ((p * NB + rb) * NB + cb) * 4 + r;
-}
+// integrated block bi <-> (rb, cb): (0, 0), then at N = 32 (0, 1) and (1, 1)
+__device__ __forceinline__ constexpr int gq_brow(int bi) { return bi == 2 ? 1 : 0; }
+__device__ __forceinline__ constexpr int gq_bcol(int bi) { return bi == 0 ? 0 : 1; }
+__device__ __forceinline__ constexpr int gq_blk(int rb, int cb) { return rb + cb; }   // rb <= cb
+__device__ __forceinline__ constexpr int gq_xi(int p, int bi, int r) { return (p * NBLK + bi) * 4 + r; }
 __device__ __forceinline__ constexpr int gq_op(int p, int b, int kb) { return (p * NB + b) * NB + kb; }
 
 // Wavefronts per SIMD: two at N = 32 (at most 256 registers), so that one wavefront's
@@ -88,7 +109,7 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) GQ_SPLIT_OCCUPANCY void GQ_KER
     __shared__ gq_half8 s_left[2][NOPS][64];    // [hi, lo][(P, block, kb)][lane]
 #endif
     __shared__ real s_diag[HWAVES][HN];         // populations, for the jump terms
-    __shared__ real s_t[HWAVES][2][HN * HLD];   // the state, for its Hermitian part
+    __shared__ real s_t[HWAVES][2][HN * HLD];   // block transposes
     __shared__ real s_vco[GQ_NBASIS * HN];      // drive row coefficients per basis term
 #if GQ_NMONO
     // general jump operators (one nonzero per row): a per-wavefront copy of the state and the
@@ -149,52 +170,48 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) GQ_SPLIT_OCCUPANCY void GQ_KER
 #endif
 
     // the lanes with g == c / 4 hold the diagonal element of column c in register c % 4 of
-    // the tiles (0, b, b)
+    // the diagonal blocks
     const bool dlane = (g == (c >> 2));
     const int rd = c & 3;
 
     // ---- initial state: rho0 values (reduced layout) or |0><0| ------------------------------
     real y[NX];
 #pragma unroll
-    for (int rb = 0; rb < NB; ++rb)
+    for (int bi = 0; bi < NBLK; ++bi)
 #pragma unroll
-        for (int cb = 0; cb < NB; ++cb)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = 16 * rb + 4 * g + r, j = 16 * cb + c;
-                real re = R(0.0), im = R(0.0);
+        for (int r = 0; r < 4; ++r) {
+            const int i = 16 * gq_brow(bi) + 4 * g + r, j = 16 * gq_bcol(bi) + c;
+            real re = R(0.0), im = R(0.0);
 #if GQ_RHO0_VALUES
-                const real* __restrict__ r0 = (const real*)a.rho0 + pt.q * (HN * HN - 1);
-                if (i == j) {
-                    if (i < HN - 1) {
-                        re = r0[i];
-                    } else {
-                        real s = R(0.0);
-                        for (int k = 0; k < HN - 1; ++k) s += r0[k];
-                        re = R(1.0) - s;
-                    }
+            const real* __restrict__ r0 = (const real*)a.rho0 + pt.q * (HN * HN - 1);
+            if (i == j) {
+                if (i < HN - 1) {
+                    re = r0[i];
                 } else {
-                    const int lo = i < j ? i : j, hi = i < j ? j : i;
-                    const int u = lo * HN - (lo * (lo + 1)) / 2 + (hi - lo - 1);
-                    re = r0[HN - 1 + 2 * u];
-                    im = (i < j ? R(1.0) : R(-1.0)) * r0[HN - 1 + 2 * u + 1];
+                    real s = R(0.0);
+                    for (int k = 0; k < HN - 1; ++k) s += r0[k];
+                    re = R(1.0) - s;
                 }
-#else
-                re = (i == 0 && j == 0) ? R(1.0) : R(0.0);
-#endif
-                y[gq_xi(0, rb, cb, r)] = re;
-                y[gq_xi(1, rb, cb, r)] = im;
+            } else {
+                const int lo = i < j ? i : j, hi = i < j ? j : i;
+                const int u = lo * HN - (lo * (lo + 1)) / 2 + (hi - lo - 1);
+                re = r0[HN - 1 + 2 * u];
+                im = (i < j ? R(1.0) : R(-1.0)) * r0[HN - 1 + 2 * u + 1];
             }
+#else
+            re = (i == 0 && j == 0) ? R(1.0) : R(0.0);
+#endif
+            y[gq_xi(0, bi, r)] = re;
+            y[gq_xi(1, bi, r)] = im;
+        }
 
 #if GQ_HAS_DEP
-    real depw[NB][NB][4];   // dephasing weights sum_k d_k(i) d_k(j)
+    real depw[NBLK][4];   // dephasing weights sum_k d_k(i) d_k(j)
 #pragma unroll
-    for (int rb = 0; rb < NB; ++rb)
+    for (int bi = 0; bi < NBLK; ++bi)
 #pragma unroll
-        for (int cb = 0; cb < NB; ++cb)
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                depw[rb][cb][r] = cst[GQ_OFF_DEP + (16 * rb + 4 * g + r) * HN + 16 * cb + c];
+        for (int r = 0; r < 4; ++r)
+            depw[bi][r] = cst[GQ_OFF_DEP + (16 * gq_brow(bi) + 4 * g + r) * HN + 16 * gq_bcol(bi) + c];
 #endif
     // population transfer into the lane's diagonal elements (zero weights elsewhere) and the
     // observable's weight of those elements
@@ -227,7 +244,7 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) GQ_SPLIT_OCCUPANCY void GQ_KER
     const real dt2 = dt * R(0.5);
     const real dt6 = dt / R(6.0);
     const int num_steps = a.num_steps;
-    real wdr[NB][NB][4];      // v_i - v_j of the lane's elements at the stage time
+    real wdr[NBLK][4];        // v_i - v_j of the lane's elements at the stage time
 
     auto stage_values = [&](int row) {
         const real* __restrict__ tab = (const real*)a.time_table;
@@ -253,24 +270,49 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) GQ_SPLIT_OCCUPANCY void GQ_KER
             vc[b] = s;
         }
 #pragma unroll
-        for (int rb = 0; rb < NB; ++rb)
+        for (int bi = 0; bi < NBLK; ++bi)
 #pragma unroll
-            for (int cb = 0; cb < NB; ++cb)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) wdr[rb][cb][r] = vr[rb][r] - vc[cb];
+            for (int r = 0; r < 4; ++r) wdr[bi][r] = vr[gq_brow(bi)][r] - vc[gq_bcol(bi)];
     };
 
-    // the RHS at the stage input x, written over x.  The element-wise terms initialise the
-    // accumulators, so x is dead once split and the products accumulate in its place.
+    // the RHS at the stage input x (the integrated blocks), written over x.  The element-wise
+    // terms initialise the accumulators, so x is dead once split and the products accumulate
+    // in its place.
     auto rhs = [&](real* x) {
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             real xd = R(0.0);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) xd = (r == rd) ? x[gq_xi(0, b, b, r)] : xd;
+            for (int r = 0; r < 4; ++r) xd = (r == rd) ? x[gq_xi(0, gq_blk(b, b), r)] : xd;
             if (dlane) dbuf[16 * b + c] = xd;
         }
+        // the operand blocks [kb][cb], from the integrated blocks and, at N = 32, block (1, 0):
+        // element (16 + 4 g + r, c) = conj(rho[c][16 + 4 g + r]), which lane (4 g + r, c / 4)
+        // holds in register c % 4 of block (0, 1)
+        float bv[NB][NB][2][4];
+#pragma unroll
+        for (int kb = 0; kb < NB; ++kb)
+#pragma unroll
+            for (int cb = kb; cb < NB; ++cb)
+#pragma unroll
+                for (int p = 0; p < 2; ++p)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) bv[kb][cb][p][r] = x[gq_xi(p, gq_blk(kb, cb), r)];
+#if NB > 1
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            t_r[(4 * g + r) * HLD + c] = x[gq_xi(0, 1, r)];
+            t_i[(4 * g + r) * HLD + c] = x[gq_xi(1, 1, r)];
+        }
+        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            bv[1][0][0][r] = t_r[c * HLD + 4 * g + r];
+            bv[1][0][1][r] = -t_i[c * HLD + 4 * g + r];
+        }
+#endif
 #if GQ_NMONO
+        // the whole state, for the gather of the general jump terms
 #pragma unroll
         for (int rb = 0; rb < NB; ++rb)
 #pragma unroll
@@ -278,28 +320,16 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) GQ_SPLIT_OCCUPANCY void GQ_KER
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int own = (16 * rb + 4 * g + r) * HLD + 16 * cb + c;
-                    xm_r[own] = x[gq_xi(0, rb, cb, r)];
-                    xm_i[own] = x[gq_xi(1, rb, cb, r)];
+                    xm_r[own] = bv[rb][cb][0][r];
+                    xm_i[own] = bv[rb][cb][1][r];
                 }
 #endif
-        // hi / lo parts of the state, operand blocks [kb][cb]
+        // hi / lo parts of the operand blocks
         gq_half8 sh[NB][NB], sl[NB][NB];
 #pragma unroll
         for (int kb = 0; kb < NB; ++kb)
 #pragma unroll
-            for (int cb = 0; cb < NB; ++cb) {
-                gq_u32x4 hw, lw;
-#pragma unroll
-                for (int e2 = 0; e2 < 4; ++e2) {
-                    const int e = 2 * e2;
-                    unsigned hv, lv;
-                    gq_split2(x[gq_xi(e >> 2, kb, cb, e & 3)], x[gq_xi(e >> 2, kb, cb, (e & 3) + 1)], hv, lv);
-                    hw[e2] = hv;
-                    lw[e2] = lv;
-                }
-                sh[kb][cb] = __builtin_bit_cast(gq_half8, hw);
-                sl[kb][cb] = __builtin_bit_cast(gq_half8, lw);
-            }
+            for (int cb = 0; cb < NB; ++cb) gq_split_block(bv[kb][cb], sh[kb][cb], sl[kb][cb]);
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
         real jsum[NB];
 #pragma unroll
@@ -311,58 +341,54 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) GQ_SPLIT_OCCUPANCY void GQ_KER
         }
         // element-wise terms: drive -i (v_i - v_j) rho_ij, dephasing, jumps; they start the
         // accumulators of the products
-        gq_f32x4 acc[2][NB][NB];
+        gq_f32x4 acc[2][NBLK];
 #pragma unroll
-        for (int rb = 0; rb < NB; ++rb)
+        for (int bi = 0; bi < NBLK; ++bi)
 #pragma unroll
-            for (int cb = 0; cb < NB; ++cb)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const real xr = x[gq_xi(0, rb, cb, r)];
-                    const real xi = x[gq_xi(1, rb, cb, r)];
-                    const real w = wdr[rb][cb][r];
-                    real dr = w * xi;
-                    real di = -w * xr;
+            for (int r = 0; r < 4; ++r) {
+                const real xr = x[gq_xi(0, bi, r)];
+                const real xi = x[gq_xi(1, bi, r)];
+                const real w = wdr[bi][r];
+                real dr = w * xi;
+                real di = -w * xr;
 #if GQ_HAS_DEP
-                    dr = fma(depw[rb][cb][r], xr, dr);
-                    di = fma(depw[rb][cb][r], xi, di);
+                dr = fma(depw[bi][r], xr, dr);
+                di = fma(depw[bi][r], xi, di);
 #endif
 #if GQ_NMONO
-                    const int i = 16 * rb + 4 * g + r;
+                const int i = 16 * gq_brow(bi) + 4 * g + r, cb = gq_bcol(bi);
 #pragma unroll
-                    for (int kk = 0; kk < GQ_NMONO; ++kk) {
-                        const int ci = s_mcol[kk * HN + i];
-                        if (ci >= 0 && mcj[kk][cb] >= 0) {
-                            // w = l_i conj(l_j), times rho[col_i][col_j]
-                            const real lr = s_mre[kk * HN + i], li = s_mim[kk * HN + i];
-                            const real wr = fma(lr, mlr[kk][cb], li * mli[kk][cb]);
-                            const real wi = fma(li, mlr[kk][cb], -lr * mli[kk][cb]);
-                            const real pr = xm_r[ci * HLD + mcj[kk][cb]];
-                            const real pim = xm_i[ci * HLD + mcj[kk][cb]];
-                            dr = fma(wr, pr, fma(-wi, pim, dr));
-                            di = fma(wr, pim, fma(wi, pr, di));
-                        }
+                for (int kk = 0; kk < GQ_NMONO; ++kk) {
+                    const int ci = s_mcol[kk * HN + i];
+                    if (ci >= 0 && mcj[kk][cb] >= 0) {
+                        // w = l_i conj(l_j), times rho[col_i][col_j]
+                        const real lr = s_mre[kk * HN + i], li = s_mim[kk * HN + i];
+                        const real wr = fma(lr, mlr[kk][cb], li * mli[kk][cb]);
+                        const real wi = fma(li, mlr[kk][cb], -lr * mli[kk][cb]);
+                        const real pr = xm_r[ci * HLD + mcj[kk][cb]];
+                        const real pim = xm_i[ci * HLD + mcj[kk][cb]];
+                        dr = fma(wr, pr, fma(-wi, pim, dr));
+                        di = fma(wr, pim, fma(wi, pr, di));
                     }
-#endif
-                    if (rb == cb) dr += (dlane && r == rd) ? jsum[rb] : R(0.0);
-                    acc[0][rb][cb][r] = dr;
-                    acc[1][rb][cb][r] = di;
                 }
+#endif
+                if (gq_brow(bi) == gq_bcol(bi)) dr += (dlane && r == rd) ? jsum[gq_brow(bi)] : R(0.0);
+                acc[0][bi][r] = dr;
+                acc[1][bi][r] = di;
+            }
         // The two products in groups (kb, P): the coefficient operands of one group (one per
-        // block, hi and lo) serve 6 NB^2 products, the left ones (as A operands, one per row
-        // block) and the right ones (as B operands, one per column block).  At N = 32 they come
-        // from LDS one group ahead, so that their latency hides behind the previous group's
-        // products.  The wavefront issues the products at raised priority: the other
-        // wavefront on the SIMD then fills the matrix core's busy cycles with its element-wise
-        // work.
+        // block, hi and lo) serve the left products (as A operands, one per row block) and the
+        // right ones (as B operands, one per column block) of every integrated block.  At
+        // N = 32 they come from LDS one group ahead, so that their latency hides behind the
+        // previous group's products.  The wavefront issues the products at raised priority:
+        // the other wavefront on the SIMD then fills the matrix core's busy cycles with its
+        // element-wise work.
         __builtin_amdgcn_s_setprio(1);
-        gq_f32x4 accn[NB][NB];   // right products of the imaginary part, subtracted
+        gq_f32x4 accn[NBLK];   // right products of the imaginary part, subtracted
 #pragma unroll
-        for (int rb = 0; rb < NB; ++rb)
+        for (int bi = 0; bi < NBLK; ++bi)
 #pragma unroll
-            for (int cb = 0; cb < NB; ++cb)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) accn[rb][cb][r] = 0.0f;
+            for (int r = 0; r < 4; ++r) accn[bi][r] = 0.0f;
         gq_half8 ob[2][NB][2];
         auto group_ops = [&](int gi, gq_half8 (&o)[NB][2]) {
             const int kb = gi >> 1, p = gi & 1;
@@ -377,17 +403,15 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) GQ_SPLIT_OCCUPANCY void GQ_KER
 #pragma unroll
             for (int t = 0; t < 3; ++t)
 #pragma unroll
-                for (int rb = 0; rb < NB; ++rb)
-#pragma unroll
-                    for (int cb = 0; cb < NB; ++cb) {
-                        acc[p][rb][cb] = GQ_MFMA16(o[rb][t == 0 ? 1 : 0], t == 1 ? sl[kb][cb] : sh[kb][cb],
-                                                   acc[p][rb][cb]);
-                        const gq_half8 sa = t == 1 ? sl[kb][rb] : sh[kb][rb];
-                        if (p == 0)
-                            acc[0][rb][cb] = GQ_MFMA16(sa, o[cb][t == 0 ? 1 : 0], acc[0][rb][cb]);
-                        else
-                            accn[rb][cb] = GQ_MFMA16(sa, o[cb][t == 0 ? 1 : 0], accn[rb][cb]);
-                    }
+                for (int bi = 0; bi < NBLK; ++bi) {
+                    const int rb = gq_brow(bi), cb = gq_bcol(bi);
+                    acc[p][bi] = GQ_MFMA16(o[rb][t == 0 ? 1 : 0], t == 1 ? sl[kb][cb] : sh[kb][cb], acc[p][bi]);
+                    const gq_half8 sa = t == 1 ? sl[kb][rb] : sh[kb][rb];
+                    if (p == 0)
+                        acc[0][bi] = GQ_MFMA16(sa, o[cb][t == 0 ? 1 : 0], acc[0][bi]);
+                    else
+                        accn[bi] = GQ_MFMA16(sa, o[cb][t == 0 ? 1 : 0], accn[bi]);
+                }
         };
         group_ops(0, ob[0]);
 #pragma unroll
@@ -403,45 +427,41 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) GQ_SPLIT_OCCUPANCY void GQ_KER
         }
         __builtin_amdgcn_s_setprio(0);
 #pragma unroll
-        for (int p = 0; p < 2; ++p)
+        for (int bi = 0; bi < NBLK; ++bi)
 #pragma unroll
-            for (int rb = 0; rb < NB; ++rb)
-#pragma unroll
-                for (int cb = 0; cb < NB; ++cb)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        x[gq_xi(p, rb, cb, r)] = p == 0 ? acc[0][rb][cb][r] : acc[1][rb][cb][r] - accn[rb][cb][r];
+            for (int r = 0; r < 4; ++r) {
+                x[gq_xi(0, bi, r)] = acc[0][bi][r];
+                x[gq_xi(1, bi, r)] = acc[1][bi][r] - accn[bi][r];
+            }
     };
 
-    // The state after each step is replaced by its Hermitian part (rho + rho^+) / 2, which is
-    // exactly Hermitian (element (j, i) takes the same two values in the other order).  The
-    // right product reads conj(rho_ki) for rho_ik, and the matrix cores round the two sides
-    // of one element in different orders: left alone, the anti-Hermitian part of that rounding
-    // builds up over the steps (the products of rho_ik and conj(rho_ki) do not damp it) and
-    // reaches 1.8e-7 after 2000 steps, with 2.1e-6 error in rho at N = 16.  The reference's
-    // reduced state holds the upper triangle only, Hermitian by construction.
+    // The diagonal blocks after each step are replaced by their Hermitian part (B + B^+) / 2,
+    // which is exactly Hermitian (element (j, i) takes the same two values in the other order);
+    // the off-diagonal block's mirror is not stored.  The right product reads conj(rho_ki) for
+    // rho_ik, and the matrix cores round the two sides of one element in different orders:
+    // left alone, the anti-Hermitian part of that rounding builds up over the steps (the
+    // products of rho_ik and conj(rho_ki) do not damp it) and reaches 1.8e-7 after 2000 steps,
+    // with 2.1e-6 error in rho at N = 16.  The reference's reduced state holds the upper
+    // triangle only, Hermitian by construction.
     auto hermitian_part = [&](real* v) {
 #pragma unroll
-        for (int rb = 0; rb < NB; ++rb)
+        for (int b = 0; b < NB; ++b)
 #pragma unroll
-            for (int cb = 0; cb < NB; ++cb)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int own = (16 * rb + 4 * g + r) * HLD + 16 * cb + c;
-                    t_r[own] = v[gq_xi(0, rb, cb, r)];
-                    t_i[own] = v[gq_xi(1, rb, cb, r)];
-                }
+            for (int r = 0; r < 4; ++r) {
+                const int own = (16 * b + 4 * g + r) * HLD + 16 * b + c;
+                t_r[own] = v[gq_xi(0, gq_blk(b, b), r)];
+                t_i[own] = v[gq_xi(1, gq_blk(b, b), r)];
+            }
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
 #pragma unroll
-        for (int rb = 0; rb < NB; ++rb)
+        for (int b = 0; b < NB; ++b)
 #pragma unroll
-            for (int cb = 0; cb < NB; ++cb)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int tr = (16 * cb + c) * HLD + 16 * rb + 4 * g + r;
-                    v[gq_xi(0, rb, cb, r)] = R(0.5) * (v[gq_xi(0, rb, cb, r)] + t_r[tr]);
-                    v[gq_xi(1, rb, cb, r)] = R(0.5) * (v[gq_xi(1, rb, cb, r)] - t_i[tr]);
-                }
+            for (int r = 0; r < 4; ++r) {
+                const int tr = (16 * b + c) * HLD + 16 * b + 4 * g + r;
+                const int bi = gq_blk(b, b);
+                v[gq_xi(0, bi, r)] = R(0.5) * (v[gq_xi(0, bi, r)] + t_r[tr]);
+                v[gq_xi(1, bi, r)] = R(0.5) * (v[gq_xi(1, bi, r)] - t_i[tr]);
+            }
         // the buffer is written again only after these reads (in program order, per lane)
         asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     };
@@ -453,7 +473,7 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) GQ_SPLIT_OCCUPANCY void GQ_KER
         for (int b = 0; b < NB; ++b) {
             real yd = R(0.0);
 #pragma unroll
-            for (int r = 0; r < 4; ++r) yd = (r == rd) ? v[gq_xi(0, b, b, r)] : yd;
+            for (int r = 0; r < 4; ++r) yd = (r == rd) ? v[gq_xi(0, gq_blk(b, b), r)] : yd;
             s = fma(od[b], yd, s);
         }
         return s;
@@ -540,24 +560,23 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) GQ_SPLIT_OCCUPANCY void GQ_KER
         gq_store_complex(a.result, q, r, R(0.0));
     }
 #elif GQ_OUT_FINAL_RHO
+    // the upper triangle of the integrated blocks (reduced layout)
 #pragma unroll
-    for (int rb = 0; rb < NB; ++rb)
+    for (int bi = 0; bi < NBLK; ++bi)
 #pragma unroll
-        for (int cb = 0; cb < NB; ++cb)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = 16 * rb + 4 * g + r, j = 16 * cb + c;
-                const real yr = y[gq_xi(0, rb, cb, r)], yi = y[gq_xi(1, rb, cb, r)];
-                if (i == j && i < HN - 1) {
-                    ((real*)a.result)[(long long)i * a.point_count + q] = yr;
-                    bad |= !gq_finite(yr);
-                } else if (i < j) {
-                    const int idx = HN - 1 + 2 * (i * HN - (i * (i + 1)) / 2 + (j - i - 1));
-                    ((real*)a.result)[(long long)idx * a.point_count + q] = yr;
-                    ((real*)a.result)[(long long)(idx + 1) * a.point_count + q] = yi;
-                    bad |= !(gq_finite(yr) && gq_finite(yi));
-                }
+        for (int r = 0; r < 4; ++r) {
+            const int i = 16 * gq_brow(bi) + 4 * g + r, j = 16 * gq_bcol(bi) + c;
+            const real yr = y[gq_xi(0, bi, r)], yi = y[gq_xi(1, bi, r)];
+            if (i == j && i < HN - 1) {
+                ((real*)a.result)[(long long)i * a.point_count + q] = yr;
+                bad |= !gq_finite(yr);
+            } else if (i < j) {
+                const int idx = HN - 1 + 2 * (i * HN - (i * (i + 1)) / 2 + (j - i - 1));
+                ((real*)a.result)[(long long)idx * a.point_count + q] = yr;
+                ((real*)a.result)[(long long)(idx + 1) * a.point_count + q] = yi;
+                bad |= !(gq_finite(yr) && gq_finite(yi));
             }
+        }
 #endif
     gq_flag(a.status, bad, GQ_STATUS_NONFINITE_RESULT);
 #if GQ_TRACE
