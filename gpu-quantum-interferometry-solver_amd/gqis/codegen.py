@@ -161,9 +161,9 @@ def _atoms_stage_lines(plan, const_double, const_real):
     adaptive kernels' stage times).  Each sin / cos of an affine function of t is evaluated
     without the library function: FP64 kernels by ``gq_sincos64`` (Cody-Waite reduction to
     |r| <= pi/4 and Taylor polynomials, within 2 ulp of the FP64 function), FP32 kernels by
-    reducing the FP64 angle to [-pi, pi] (two-part 2 pi) and taking the FP32 function of it:
-    FP32 accuracy, as the FP32 RHS, for a fraction of the FP64 evaluation.  Other atoms (t
-    outside such a function) are evaluated in FP64."""
+    ``gq_sincos32`` (the same reduction in FP64, the polynomials in FP32): FP32 accuracy, as the
+    FP32 RHS, for a fraction of the FP64 evaluation.  Other atoms (t outside such a function)
+    are evaluated in FP64."""
     lines = ["__device__ __forceinline__ void gq_atoms_stage(const double t_now, "
              "const real* __restrict__ C, real* __restrict__ TT) {",
              "    (void)t_now; (void)C; (void)TT;"]
@@ -187,7 +187,7 @@ def _atoms_stage_lines(plan, const_double, const_real):
                     lines += ["#if GQ_FP64",
                               f"    const real {name} = gq_{fn}64({a});",
                               "#else",
-                              f"    const real {name} = {fn}((real)gq_reduce_2pi({a}));",
+                              f"    const real {name} = gq_{fn}32({a});",
                               "#endif"]
                 subs[f] = reduced[f]
         rest = e.xreplace(subs)
@@ -519,6 +519,49 @@ def _sincos64_lines():
     return lines
 
 
+def _sincos32_lines():
+    """FP32 sin / cos of an FP64 angle without a library call, for the FP32 adaptive kernels'
+    stage-time atoms: the same Cody and Waite reduction as ``gq_sincos64`` in FP64 to |r| <=
+    pi/4, then r rounded to FP32 and the Taylor polynomials of sin and cos to degree 11 / 12 in
+    FP32 (the first omitted terms are below 2e-10 at |r| = pi/4), and the quadrant.  The FP32
+    library function would carry its large-argument reduction, which the reduced angle never
+    needs, and with it about 90 registers."""
+    from fractions import Fraction
+    from math import factorial
+    pio2 = sp.pi.evalf(40) / 2
+    hi = float(pio2)
+    lo = float(pio2 - sp.Float(hi, 40))
+    inv = float(2 / sp.pi.evalf(40))
+
+    def coef(n):
+        return repr(float(Fraction((-1) ** (n // 2), factorial(n)))) + "f"
+    sin_terms = [coef(n) for n in range(11, 2, -2)]
+    cos_terms = [coef(n) for n in range(12, 1, -2)]
+    lines = ["__device__ __forceinline__ void gq_sincos32(double x, float* s, float* c) {",
+             f"    const double k = rint(x * {inv!r});",
+             f"    double rd = fma(-k, {hi!r}, x);",
+             f"    rd = fma(-k, {lo!r}, rd);",
+             "    const float r = (float)rd;",
+             "    const float r2 = r * r;",
+             f"    float ps = {sin_terms[0]};"]
+    lines += [f"    ps = fmaf(ps, r2, {c});" for c in sin_terms[1:]]
+    lines += [f"    float pc = {cos_terms[0]};"]
+    lines += [f"    pc = fmaf(pc, r2, {c});" for c in cos_terms[1:]]
+    lines += ["    const float sr = fmaf(r * r2, ps, r);",
+              "    const float cr = fmaf(r2, pc, 1.0f);",
+              "    const int q = (int)k & 3;",
+              "    const float sa = (q & 1) ? cr : sr;",
+              "    const float ca = (q & 1) ? sr : cr;",
+              "    *s = (q & 2) ? -sa : sa;",
+              "    *c = ((q + 1) & 2) ? -ca : ca;",
+              "}",
+              "__device__ __forceinline__ float gq_sin32(double x) {",
+              "    float s, c;", "    gq_sincos32(x, &s, &c);", "    return s;", "}",
+              "__device__ __forceinline__ float gq_cos32(double x) {",
+              "    float s, c;", "    gq_sincos32(x, &s, &c);", "    return c;", "}"]
+    return lines
+
+
 def prelude(fp64, acc64, block_size, defines):
     real = "double" if fp64 else "float"
     acc = "double" if (fp64 or acc64) else "float"
@@ -538,7 +581,7 @@ def prelude(fp64, acc64, block_size, defines):
            "__device__ __forceinline__ double gq_reduce_2pi(double x) {",
            "    const double k = rint(x * 0.15915494309189535);",
            "    return fma(-k, 2.4492935982947064e-16, fma(-k, 6.283185307179586, x));",
-           "}"] + _sincos64_lines()
+           "}"] + _sincos64_lines() + _sincos32_lines()
     for k, v in defines.items():
         out.append(f"#define {k} {v}")
     out += ["#ifndef GQ_HILBERT", "#define GQ_HILBERT 0", "#endif"]

@@ -9,8 +9,9 @@
 // trapezoid rule over accepted steps weighted by hs / ((num_steps - warmup) dt_out).
 // Error norms:   Tsit5  sqrt(mean((h * sum e_j k_j / sc)^2)), factor clamp(0.9 err^-1/5, 0.2, 5)
 //                DOP853 Hairer's 5(3) combination,            factor clamp(0.9 err^-1/8, 1/3, 6)
-// with sc = rtol * max(|y|, |y_new|) + atol.  Time, error norm and step controller are FP64; the
-// state and stage sums are FP64 when GQ_STATE_FP64, the RHS is evaluated in `real`.
+// with sc = rtol * max(|y|, |y_new|) + atol.  Time and step controller are FP64; the error sums
+// and norm are in the RHS precision `real` (in FP32 kernels FP32, as the reference's FP32 kernel,
+// gqis/cuda_solvers.py:868-877); the state and stage sums are FP64 when GQ_STATE_FP64.
 //
 // Persistent mode (GQ_PERSISTENT): the grid is the number of resident work-groups; a lane whose
 // point finished takes the next one from a global counter (one atomic per wavefront and refill
@@ -23,6 +24,29 @@ typedef real st_t;
 #endif
 
 #define GQ_NS GQ_NSTAGES
+
+typedef real en_t;
+#if GQ_METHOD_DOP853
+#define GQ_NERR 2
+__device__ __forceinline__ constexpr double gq_err_w(int m, int j) { return m == 0 ? GQ_E5[j] : GQ_E3[j]; }
+#else
+#define GQ_NERR 1
+__device__ __forceinline__ constexpr double gq_err_w(int, int j) { return GQ_E[j]; }
+#endif
+// Stage at which the stored derivatives are folded into partial sums: it balances the arrays
+// held before the fold (y, k_0 .. k_{F-1}, the stage value and its derivative) against those
+// after it (y, k_0, the partial sums of stages F+1 .. GQ_NS-1, the b and error sums, the stage
+// value and its derivative).
+#define GQ_FOLD ((GQ_NS + 2 + GQ_NERR) / 2)
+static_assert(GQ_FOLD >= 1 && GQ_FOLD < GQ_NS, "fold stage inside the step");
+// The derivative at the step start (FSAL, kept for a rejected step's retry) lives in LDS, one
+// column per lane, when the work-group's copy fits in 64 KB: it is read by the first stages and
+// the fold and written once per accepted step, and its registers go to the stages.
+#if GQ_BLOCK * GQ_D * (GQ_FP64 ? 8 : 4) <= 65536
+#define GQ_K0_LDS 1
+#else
+#define GQ_K0_LDS 0
+#endif
 
 // stage values at an FP64 time (the atoms from gq_atoms_stage: FP64, or FP32 functions of
 // FP64-reduced angles in FP32 kernels)
@@ -68,7 +92,13 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_adaptive)(co
     gq_static_t S;
     gq_stage_t G;
     st_t y[GQ_D];
+#if GQ_K0_LDS
+    __shared__ real s_k0[GQ_D][GQ_BLOCK];
+#define GQ_K0(i) s_k0[i][threadIdx.x]
+#else
     real k0[GQ_D];
+#define GQ_K0(i) k0[i]
+#endif
     double tau = 0.0, h = dt_out;
     int step = 0, sub = 0, n_acc = 0, n_rej = 0;
     bool failed = false, bad_res = false, bad_trace = false, overflow = false;
@@ -123,7 +153,12 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_adaptive)(co
                 n_acc = n_rej = 0;
                 failed = false;
                 gq_stage_at(a.t_in, px, py, C, S, G);
-                gq_eval(y, px, py, C, S, G, k0);
+                {
+                    real kstart[GQ_D];
+                    gq_eval(y, px, py, C, S, G, kstart);
+#pragma unroll
+                    for (int i = 0; i < GQ_D; ++i) GQ_K0(i) = kstart[i];
+                }
 #if GQ_OUT_MEAN
                 avg_re = avg_im = 0;
                 gq_observe(y, px, py, C, S, G, op_re, op_im);
@@ -144,69 +179,127 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_adaptive)(co
             const bool last = h * 1.01 >= remaining;
             const double hs = last ? remaining : h;
             const double t0 = (double)step * dt_out + tau;
+            // Folded stage storage: k_0 .. k_{F-1} (F = GQ_FOLD) stay as they are until stage F
+            // is formed; then they are folded into the partial sums P[r] of the later stages and
+            // into the b and error sums, and every later k_s is folded as soon as it exists.  Each
+            // sum still runs over j in ascending order, so stage values and y_new are those of
+            // the unfolded form, while at most about F + 2 derivative-sized arrays are live
+            // instead of all GQ_NS derivatives, y_new and f(y_new) at once.
             real k[GQ_NS][GQ_D];
+            st_t P[GQ_NS][GQ_D];
+            st_t bsum[GQ_D];
+            en_t esum[GQ_NERR][GQ_D];
+            // fold k_s into the later stages' partial sums and the b and error sums
+            auto fold = [&](const int s) {
 #pragma unroll
-            for (int i = 0; i < GQ_D; ++i) k[0][i] = k0[i];
+                for (int i = 0; i < GQ_D; ++i) {
 #pragma unroll
-            for (int s = 1; s < GQ_NS; ++s) {
+                    for (int r = s + 1; r < GQ_NS; ++r)
+                        if (GQ_A[r][s] != 0.0) P[r][i] = fma((st_t)GQ_A[r][s], (st_t)k[s][i], P[r][i]);
+                    if (GQ_B[s] != 0.0) bsum[i] = fma((st_t)GQ_B[s], (st_t)k[s][i], bsum[i]);
+#pragma unroll
+                    for (int m = 0; m < GQ_NERR; ++m)
+                        if (gq_err_w(m, s) != 0.0)
+                            esum[m][i] = fma((en_t)gq_err_w(m, s), (en_t)k[s][i], esum[m][i]);
+                }
+            };
+            // k_0 is read where it is used, so in LDS it holds no registers between uses
+            auto kv = [&](const int j, const int i) -> real { return j == 0 ? GQ_K0(i) : k[j][i]; };
+            // stages 1 .. F-1 from the stored derivatives
+#pragma unroll
+            for (int s = 1; s < GQ_FOLD; ++s) {
                 st_t ys[GQ_D];
 #pragma unroll
                 for (int i = 0; i < GQ_D; ++i) {
                     st_t acc = 0;
 #pragma unroll
                     for (int j = 0; j < s; ++j)
-                        if (GQ_A[s][j] != 0.0) acc = fma((st_t)GQ_A[s][j], (st_t)k[j][i], acc);
+                        if (GQ_A[s][j] != 0.0) acc = fma((st_t)GQ_A[s][j], (st_t)kv(j, i), acc);
                     ys[i] = fma((st_t)hs, acc, y[i]);
                 }
                 gq_stage_at(a.t_in + (t0 + GQ_C[s] * hs), px, py, C, S, G);
                 gq_eval(ys, px, py, C, S, G, k[s]);
             }
-            st_t yn[GQ_D];
-#pragma unroll
-            for (int i = 0; i < GQ_D; ++i) {
-                st_t acc = 0;
-#pragma unroll
-                for (int j = 0; j < GQ_NS; ++j)
-                    if (GQ_B[j] != 0.0) acc = fma((st_t)GQ_B[j], (st_t)k[j][i], acc);
-                yn[i] = fma((st_t)hs, acc, y[i]);
-            }
-            // G now holds the stage values at t0 + hs (the last stage has c = 1)
-            double err_norm;
-#if GQ_METHOD_DOP853
+            // stage F from the stored derivatives, which are then folded
             {
-                double s5 = 0.0, s3 = 0.0;
+                st_t ys[GQ_D];
 #pragma unroll
                 for (int i = 0; i < GQ_D; ++i) {
-                    double e5 = 0.0, e3 = 0.0;
+                    st_t acc = 0;
 #pragma unroll
-                    for (int j = 0; j < GQ_NS; ++j) {
-                        if (GQ_E5[j] != 0.0) e5 = fma(GQ_E5[j], (double)k[j][i], e5);
-                        if (GQ_E3[j] != 0.0) e3 = fma(GQ_E3[j], (double)k[j][i], e3);
+                    for (int j = 0; j < GQ_FOLD; ++j)
+                        if (GQ_A[GQ_FOLD][j] != 0.0) acc = fma((st_t)GQ_A[GQ_FOLD][j], (st_t)kv(j, i), acc);
+                    ys[i] = fma((st_t)hs, acc, y[i]);
+#pragma unroll
+                    for (int r = GQ_FOLD + 1; r < GQ_NS; ++r) {
+                        st_t p = 0;
+#pragma unroll
+                        for (int j = 0; j < GQ_FOLD; ++j)
+                            if (GQ_A[r][j] != 0.0) p = fma((st_t)GQ_A[r][j], (st_t)kv(j, i), p);
+                        P[r][i] = p;
                     }
-                    const double sc = fma(rtol, fmax(fabs((double)y[i]), fabs((double)yn[i])), atol);
-                    s5 = fma(e5 / sc, e5 / sc, s5);
-                    s3 = fma(e3 / sc, e3 / sc, s3);
+                    st_t b = 0;
+#pragma unroll
+                    for (int j = 0; j < GQ_FOLD; ++j)
+                        if (GQ_B[j] != 0.0) b = fma((st_t)GQ_B[j], (st_t)kv(j, i), b);
+                    bsum[i] = b;
+#pragma unroll
+                    for (int m = 0; m < GQ_NERR; ++m) {
+                        en_t e = 0;
+#pragma unroll
+                        for (int j = 0; j < GQ_FOLD; ++j)
+                            if (gq_err_w(m, j) != 0.0) e = fma((en_t)gq_err_w(m, j), (en_t)kv(j, i), e);
+                        esum[m][i] = e;
+                    }
                 }
-                double deno = s5 + 0.01 * s3;
+                gq_stage_at(a.t_in + (t0 + GQ_C[GQ_FOLD] * hs), px, py, C, S, G);
+                gq_eval(ys, px, py, C, S, G, k[GQ_FOLD]);
+                fold(GQ_FOLD);
+            }
+            // stages F+1 .. GQ_NS-1 from the partial sums
+#pragma unroll
+            for (int s = GQ_FOLD + 1; s < GQ_NS; ++s) {
+                st_t ys[GQ_D];
+#pragma unroll
+                for (int i = 0; i < GQ_D; ++i) ys[i] = fma((st_t)hs, P[s][i], y[i]);
+                gq_stage_at(a.t_in + (t0 + GQ_C[s] * hs), px, py, C, S, G);
+                gq_eval(ys, px, py, C, S, G, k[s]);
+                fold(s);
+            }
+            st_t yn[GQ_D];
+#pragma unroll
+            for (int i = 0; i < GQ_D; ++i) yn[i] = fma((st_t)hs, bsum[i], y[i]);
+            // G now holds the stage values at t0 + hs (the last stage has c = 1)
+            double err_norm;
+            const en_t rtol_e = (en_t)rtol, atol_e = (en_t)atol;
+#if GQ_METHOD_DOP853
+            {
+                en_t s5 = 0, s3 = 0;
+#pragma unroll
+                for (int i = 0; i < GQ_D; ++i) {
+                    const en_t sc = fma(rtol_e, fmax(fabs((en_t)y[i]), fabs((en_t)yn[i])), atol_e);
+                    const en_t e5 = esum[0][i] / sc, e3 = esum[1][i] / sc;
+                    s5 = fma(e5, e5, s5);
+                    s3 = fma(e3, e3, s3);
+                }
+                double deno = (double)s5 + 0.01 * (double)s3;
                 if (!(deno > 0.0)) deno = 1.0;
-                err_norm = fabs(hs) * s5 / sqrt(deno * (double)GQ_D);
+                err_norm = fabs(hs) * (double)s5 / sqrt(deno * (double)GQ_D);
             }
             real kf[GQ_D];   // FSAL derivative f(y_new), computed after acceptance
 #else
             real kf[GQ_D];
             gq_eval(yn, px, py, C, S, G, kf);
             {
-                double ssum = 0.0;
+                const en_t hse = (en_t)hs;
+                en_t ssum = 0;
 #pragma unroll
                 for (int i = 0; i < GQ_D; ++i) {
-                    double e = GQ_E[GQ_NS] * (double)kf[i];
-#pragma unroll
-                    for (int j = 0; j < GQ_NS; ++j) e = fma(GQ_E[j], (double)k[j][i], e);
-                    e *= hs;
-                    const double sc = fma(rtol, fmax(fabs((double)y[i]), fabs((double)yn[i])), atol);
+                    const en_t e = hse * fma((en_t)GQ_E[GQ_NS], (en_t)kf[i], esum[0][i]);
+                    const en_t sc = fma(rtol_e, fmax(fabs((en_t)y[i]), fabs((en_t)yn[i])), atol_e);
                     ssum = fma(e / sc, e / sc, ssum);
                 }
-                err_norm = sqrt(ssum / (double)GQ_D);
+                err_norm = sqrt((double)ssum / (double)GQ_D);
             }
 #endif
             const double factor = gq_step_factor(err_norm);   // step_control.hip
@@ -237,7 +330,7 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_adaptive)(co
 #pragma unroll
                 for (int i = 0; i < GQ_D; ++i) {
                     y[i] = yn[i];
-                    k0[i] = kf[i];
+                    GQ_K0(i) = kf[i];
                 }
                 tau = last ? dt_out : tau + hs;
                 h = last ? fmax(h, hs * factor) : hs * factor;

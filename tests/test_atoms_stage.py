@@ -1,7 +1,7 @@
 """The adaptive kernels' stage-time atoms (``gq_atoms_stage``), compiled for the CPU from the
 emitted model code: in FP32 kernels each sin / cos of an affine function of t is reduced to
-[-pi, pi] in FP64 and evaluated in FP32, so the atoms keep FP32 accuracy at any t; FP64 kernels
-evaluate them in FP64.  A wrong part of 2 pi, a missing reduction or a sin / cos swap fails
+|r| <= pi/4 in FP64 and evaluated in FP32 (``gq_sincos32``), so the atoms keep FP32 accuracy at
+any t; FP64 kernels evaluate them in FP64 (``gq_sincos64``).  A wrong part of 2 pi, a missing reduction or a sin / cos swap fails
 these."""
 import ctypes
 import subprocess
@@ -141,3 +141,48 @@ def test_fp64_sincos_is_within_two_ulp_of_the_exact_values(tmp_path):
     assert np.max(np.abs(s - ws)) <= 2.5e-16 and np.max(np.abs(c - wc)) <= 2.5e-16
     # the library functions (glibc) are not closer than that either
     assert np.max(np.abs(np.cos(x) - wc)) <= 2.5e-16
+
+
+SINCOS32_HARNESS = r"""
+#include <cmath>
+using std::rint; using std::fma; using std::fmaf;
+#define __device__
+#define __forceinline__ inline
+%(reduce)s
+extern "C" void sincos32(const double* x, int n, float* s, float* c) {
+    for (int j = 0; j < n; ++j) gq_sincos32(x[j], &s[j], &c[j]);
+}
+"""
+
+
+def test_fp32_sincos_is_within_two_fp32_ulp_of_the_exact_values(tmp_path):
+    """gq_sincos32 (the FP64 Cody-Waite reduction to |r| <= pi/4, the angle rounded to FP32,
+    Taylor polynomials to degree 11 / 12 in FP32) against 40-digit mpmath values: within
+    1.2e-7 absolute (two FP32 ulp of values just below 1) for arguments up to 1e5, including
+    the points next to multiples of pi/4; a dropped low-order term (the degree-9 one of sin is
+    3.1e-7 at pi/4), a wrong quadrant or the FP32 angle before the reduction fails this."""
+    import mpmath as mp
+    src = tmp_path / "sincos32.cpp"
+    src.write_text(SINCOS32_HARNESS % {"reduce": _reduce_source()})
+    lib = tmp_path / "sincos32.so"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-ffp-contract=off", "-shared", "-fPIC", "-w",
+                    str(src), "-o", str(lib)], check=True)
+    fn = ctypes.CDLL(str(lib)).sincos32
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    rng = np.random.default_rng(12)
+    quarter = np.pi / 4 * np.arange(-40, 800)
+    x = np.concatenate([rng.uniform(0.0, 600.0, 6000), rng.uniform(-1e5, 1e5, 3000),
+                        quarter, np.nextafter(quarter, np.inf), np.nextafter(quarter, -np.inf),
+                        [0.0, -0.5, 1.2 * 499.9]])
+    s = np.zeros(len(x), np.float32)
+    c = np.zeros(len(x), np.float32)
+    fn(x.ctypes.data, len(x), s.ctypes.data, c.ctypes.data)
+    with mp.workdps(40):
+        ws = np.array([float(mp.sin(mp.mpf(float(v)))) for v in x])
+        wc = np.array([float(mp.cos(mp.mpf(float(v)))) for v in x])
+    es = np.max(np.abs(s.astype(np.float64) - ws))
+    ec = np.max(np.abs(c.astype(np.float64) - wc))
+    assert es <= 1.2e-7 and ec <= 1.2e-7, (es, ec)
+    # FP32 sin of the FP32-rounded argument, which the reduction in FP64 avoids
+    naive = np.sin(x.astype(np.float32)).astype(np.float64)
+    assert np.max(np.abs(naive - ws)) > 1e-3
