@@ -25,20 +25,6 @@ typedef real st_t;
 
 #define GQ_NS GQ_NSTAGES
 
-typedef real en_t;
-#if GQ_METHOD_DOP853
-#define GQ_NERR 2
-__device__ __forceinline__ constexpr double gq_err_w(int m, int j) { return m == 0 ? GQ_E5[j] : GQ_E3[j]; }
-#else
-#define GQ_NERR 1
-__device__ __forceinline__ constexpr double gq_err_w(int, int j) { return GQ_E[j]; }
-#endif
-// Stage at which the stored derivatives are folded into partial sums: it balances the arrays
-// held before the fold (y, k_0 .. k_{F-1}, the stage value and its derivative) against those
-// after it (y, k_0, the partial sums of stages F+1 .. GQ_NS-1, the b and error sums, the stage
-// value and its derivative).
-#define GQ_FOLD ((GQ_NS + 2 + GQ_NERR) / 2)
-static_assert(GQ_FOLD >= 1 && GQ_FOLD < GQ_NS, "fold stage inside the step");
 // The derivative at the step start (FSAL, kept for a rejected step's retry) lives in LDS, one
 // column per lane, when the work-group's copy fits in 64 KB: it is read by the first stages and
 // the fold and written once per accepted step, and its registers go to the stages.

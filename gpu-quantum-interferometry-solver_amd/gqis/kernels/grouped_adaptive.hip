@@ -2,8 +2,9 @@
 // of each parameter point split over GQ_G lanes (one per wavefront of the work-group, rows
 // [R0, R0 + NR) of gq_grp<g>), as in grouped_rk4.hip.  Semantics are those of the per-lane
 // adaptive kernel (adaptive.hip, after gqis/cuda_solvers.py:731-908): interval snapping,
-// FSAL derivative kept on rejection, FP64 time / error norm / controller, NaN on
-// max_substeps, trapezoid mean.
+// FSAL derivative kept on rejection, FP64 time and controller, error sums in the RHS
+// precision (the waves' partials added in FP64), folded stage storage, NaN on max_substeps,
+// trapezoid mean.
 //
 // Every wave of a group keeps an identical copy of each lane's controller state (time, step
 // size, counters).  Stage inputs go through LDS as in the grouped RK4 kernel; the error norm
@@ -15,8 +16,8 @@
 // identically, so all waves reach the same barriers even if their copies disagreed.  Points
 // are assigned one per lane for the whole integration (no work stealing across groups).
 //
-// The time atoms of a step's stage times (gq_atoms_stage: about 95 instructions per FP64
-// sin / cos; an FP64 reduction and an FP32 function in FP32 kernels) are the same in every
+// The time atoms of a step's stage times (gq_atoms_stage: gq_sincos64 in FP64 kernels, an FP64
+// reduction and FP32 polynomials in FP32 kernels) are the same in every
 // wave: wave (s - 1) % GQ_G evaluates those of stage s at the step start and writes them to
 // LDS before the step's first barrier, after which every wave reads them (s_tt,
 // double-buffered by attempt parity).  The values are those of the one-lane kernel.
@@ -179,11 +180,31 @@ __device__ __forceinline__ void gq_adaptive_body(const gq_sweep_args& a,
             }
         }
 #endif
+        // Folded stage storage, as in the one-lane adaptive kernel: k_0 .. k_{F-1} are kept
+        // until stage F is formed, then folded into the later stages' partial sums and the b
+        // and error sums; each later k_s is folded as soon as it exists, every sum in ascending j.
         real k[GQ_NS][P::NR];
+        st_t Ps[GQ_NS][P::NR];
+        st_t bsum[P::NR];
+        en_t esum[GQ_NERR][P::NR];
+        auto fold = [&](const int s) {
+#pragma unroll
+            for (int r = 0; r < P::NR; ++r) {
+#pragma unroll
+                for (int u = s + 1; u < GQ_NS; ++u)
+                    if (GQ_A[u][s] != 0.0) Ps[u][r] = fma((st_t)GQ_A[u][s], (st_t)k[s][r], Ps[u][r]);
+                if (GQ_B[s] != 0.0) bsum[r] = fma((st_t)GQ_B[s], (st_t)k[s][r], bsum[r]);
+#pragma unroll
+                for (int m = 0; m < GQ_NERR; ++m)
+                    if (gq_err_w(m, s) != 0.0)
+                        esum[m][r] = fma((en_t)gq_err_w(m, s), (en_t)k[s][r], esum[m][r]);
+            }
+        };
 #pragma unroll
         for (int r = 0; r < P::NR; ++r) k[0][r] = k0[r];
+        // stages 1 .. F-1 from the stored derivatives
 #pragma unroll
-        for (int s = 1; s < GQ_NS; ++s) {
+        for (int s = 1; s < GQ_FOLD; ++s) {
             st_t ys[P::NR];
 #pragma unroll
             for (int r = 0; r < P::NR; ++r) {
@@ -199,15 +220,61 @@ __device__ __forceinline__ void gq_adaptive_body(const gq_sweep_args& a,
             P::rhs(s_y[buf], lane, px, py, C, Sg, Gg, k[s]);
             buf = GQ_NEXT_BUF(buf);
         }
+        // stage F from the stored derivatives, which are then folded
+        {
+            st_t ys[P::NR];
+#pragma unroll
+            for (int r = 0; r < P::NR; ++r) {
+                st_t acc = 0;
+#pragma unroll
+                for (int j = 0; j < GQ_FOLD; ++j)
+                    if (GQ_A[GQ_FOLD][j] != 0.0) acc = fma((st_t)GQ_A[GQ_FOLD][j], (st_t)k[j][r], acc);
+                ys[r] = fma((st_t)hs, acc, y[r]);
+#pragma unroll
+                for (int u = GQ_FOLD + 1; u < GQ_NS; ++u) {
+                    st_t p = 0;
+#pragma unroll
+                    for (int j = 0; j < GQ_FOLD; ++j)
+                        if (GQ_A[u][j] != 0.0) p = fma((st_t)GQ_A[u][j], (st_t)k[j][r], p);
+                    Ps[u][r] = p;
+                }
+                st_t b = 0;
+#pragma unroll
+                for (int j = 0; j < GQ_FOLD; ++j)
+                    if (GQ_B[j] != 0.0) b = fma((st_t)GQ_B[j], (st_t)k[j][r], b);
+                bsum[r] = b;
+#pragma unroll
+                for (int m = 0; m < GQ_NERR; ++m) {
+                    en_t e = 0;
+#pragma unroll
+                    for (int j = 0; j < GQ_FOLD; ++j)
+                        if (gq_err_w(m, j) != 0.0) e = fma((en_t)gq_err_w(m, j), (en_t)k[j][r], e);
+                    esum[m][r] = e;
+                }
+            }
+            gq_share<GI>(s_y[buf], ys, lane);
+            __syncthreads();
+            gq_stage_lds<GI>(tt + GQ_FOLD * GQ_TT_STRIDE, lane, px, py, C, Sg, Gg);
+            P::rhs(s_y[buf], lane, px, py, C, Sg, Gg, k[GQ_FOLD]);
+            buf = GQ_NEXT_BUF(buf);
+            fold(GQ_FOLD);
+        }
+        // stages F+1 .. GQ_NS-1 from the partial sums
+#pragma unroll
+        for (int s = GQ_FOLD + 1; s < GQ_NS; ++s) {
+            st_t ys[P::NR];
+#pragma unroll
+            for (int r = 0; r < P::NR; ++r) ys[r] = fma((st_t)hs, Ps[s][r], y[r]);
+            gq_share<GI>(s_y[buf], ys, lane);
+            __syncthreads();
+            gq_stage_lds<GI>(tt + s * GQ_TT_STRIDE, lane, px, py, C, Sg, Gg);
+            P::rhs(s_y[buf], lane, px, py, C, Sg, Gg, k[s]);
+            buf = GQ_NEXT_BUF(buf);
+            fold(s);
+        }
         st_t yn[P::NR];
 #pragma unroll
-        for (int r = 0; r < P::NR; ++r) {
-            st_t acc = 0;
-#pragma unroll
-            for (int j = 0; j < GQ_NS; ++j)
-                if (GQ_B[j] != 0.0) acc = fma((st_t)GQ_B[j], (st_t)k[j][r], acc);
-            yn[r] = fma((st_t)hs, acc, y[r]);
-        }
+        for (int r = 0; r < P::NR; ++r) yn[r] = fma((st_t)hs, bsum[r], y[r]);
         // Gg now holds the stage values at t0 + hs (the last stage has c = 1)
         const real* __restrict__ Yn = s_y[buf];
         gq_share<GI>(s_y[buf], yn, lane);
@@ -215,42 +282,36 @@ __device__ __forceinline__ void gq_adaptive_body(const gq_sweep_args& a,
         buf = GQ_NEXT_BUF(buf);
         real kf[P::NR];
         double err_norm;
+        const en_t rtol_e = (en_t)rtol, atol_e = (en_t)atol;
 #if GQ_METHOD_DOP853
         {
-            double s5 = 0.0, s3 = 0.0;
+            en_t s5 = 0, s3 = 0;
 #pragma unroll
             for (int r = 0; r < P::NR; ++r) {
-                double e5 = 0.0, e3 = 0.0;
-#pragma unroll
-                for (int j = 0; j < GQ_NS; ++j) {
-                    if (GQ_E5[j] != 0.0) e5 = fma(GQ_E5[j], (double)k[j][r], e5);
-                    if (GQ_E3[j] != 0.0) e3 = fma(GQ_E3[j], (double)k[j][r], e3);
-                }
-                const double sc = fma(rtol, fmax(fabs((double)y[r]), fabs((double)yn[r])), atol);
-                s5 = fma(e5 / sc, e5 / sc, s5);
-                s3 = fma(e3 / sc, e3 / sc, s3);
+                const en_t sc = fma(rtol_e, fmax(fabs((en_t)y[r]), fabs((en_t)yn[r])), atol_e);
+                const en_t e5 = esum[0][r] / sc, e3 = esum[1][r] / sc;
+                s5 = fma(e5, e5, s5);
+                s3 = fma(e3, e3, s3);
             }
-            s5 = gq_group_sum(s_red, s5, GI, lane);
-            s3 = gq_group_sum(s_red, s3, GI, lane);
-            double deno = s5 + 0.01 * s3;
+            const double s5g = gq_group_sum(s_red, (double)s5, GI, lane);
+            const double s3g = gq_group_sum(s_red, (double)s3, GI, lane);
+            double deno = s5g + 0.01 * s3g;
             if (!(deno > 0.0)) deno = 1.0;
-            err_norm = fabs(hs) * s5 / sqrt(deno * (double)GQ_D);
+            err_norm = fabs(hs) * s5g / sqrt(deno * (double)GQ_D);
         }
 #else
         P::rhs(Yn, lane, px, py, C, Sg, Gg, kf);
         {
-            double ssum = 0.0;
+            const en_t hse = (en_t)hs;
+            en_t ssum = 0;
 #pragma unroll
             for (int r = 0; r < P::NR; ++r) {
-                double e = GQ_E[GQ_NS] * (double)kf[r];
-#pragma unroll
-                for (int j = 0; j < GQ_NS; ++j) e = fma(GQ_E[j], (double)k[j][r], e);
-                e *= hs;
-                const double sc = fma(rtol, fmax(fabs((double)y[r]), fabs((double)yn[r])), atol);
+                const en_t e = hse * fma((en_t)GQ_E[GQ_NS], (en_t)kf[r], esum[0][r]);
+                const en_t sc = fma(rtol_e, fmax(fabs((en_t)y[r]), fabs((en_t)yn[r])), atol_e);
                 ssum = fma(e / sc, e / sc, ssum);
             }
-            ssum = gq_group_sum(s_red, ssum, GI, lane);
-            err_norm = sqrt(ssum / (double)GQ_D);
+            const double ssg = gq_group_sum(s_red, (double)ssum, GI, lane);
+            err_norm = sqrt(ssg / (double)GQ_D);
         }
 #endif
         const double factor = gq_step_factor(err_norm);   // step_control.hip

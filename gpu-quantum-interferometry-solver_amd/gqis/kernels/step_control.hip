@@ -25,6 +25,24 @@
 #define GQ_ERR_HI 1e5
 #endif
 
+// Error sums are kept in the RHS precision; GQ_NERR embedded combinations (Tsit5: one, DOP853:
+// the 5th- and 3rd-order ones) with weights gq_err_w(m, j).
+typedef real en_t;
+#if GQ_METHOD_DOP853
+#define GQ_NERR 2
+__device__ __forceinline__ constexpr double gq_err_w(int m, int j) { return m == 0 ? GQ_E5[j] : GQ_E3[j]; }
+#else
+#define GQ_NERR 1
+__device__ __forceinline__ constexpr double gq_err_w(int, int j) { return GQ_E[j]; }
+#endif
+// Stage F at which the stage derivatives kept so far are folded into the partial sums of the
+// later stages (folded stage storage): it balances the arrays held before the fold (y,
+// k_0 .. k_{F-1}, the stage value and its derivative) against those after it (y, k_0, the
+// partial sums of stages F+1 .. GQ_NSTAGES-1, the b and error sums, the stage value and its
+// derivative).
+#define GQ_FOLD ((GQ_NSTAGES + 2 + GQ_NERR) / 2)
+static_assert(GQ_FOLD >= 1 && GQ_FOLD < GQ_NSTAGES, "fold stage inside the step");
+
 __device__ __forceinline__ double gq_step_factor(double err) {
     const double x = fmin(fmax(err, GQ_ERR_LO), GQ_ERR_HI);
     const float l2 = __builtin_amdgcn_logf((float)x);                   // log2
