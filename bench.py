@@ -774,6 +774,8 @@ def bench_resonator(args):
             rej = info["adaptive_steps"]["rejected"]
             pub_s, pub_steps = RESONATOR_1024_TSIT5A[nph]
             row.update({"tsit5a_kernel_s": info["gpu_kernel_s"],
+                        "tsit5a_kernel": info.get("kernel"),
+                        "tsit5a_lanes_per_point": info.get("lanes_per_point"),
                         "tsit5a_mean_accepted": float(acc.mean()),
                         "tsit5a_rejected_frac": float(rej.sum() / max(acc.sum(), 1)),
                         "tsit5a_finite": bool(np.all(np.isfinite(out))),
