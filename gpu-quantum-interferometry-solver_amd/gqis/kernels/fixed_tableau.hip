@@ -31,35 +31,94 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_fixed)(const
 
 #if GQ_SCHEME == 0
     const double p0 = a.solver_param[0];
+    // a_sj of the tableau (anas5: plus its fitted part p0 * GQ_A1)
+    auto nz = [](const int s, const int j) { return GQ_A[s][j] != 0.0 || GQ_A1[s][j] != 0.0; };
+    auto coef = [&](const int s, const int j) { return (real)(GQ_A[s][j] + GQ_A1[s][j] * p0); };
+    // Folded stage storage: k_0 .. k_{F-1} are kept until stage F is formed and then folded into
+    // the partial sums Ps of the later stages; each later k_s is folded as soon as it exists.
+    // Every sum runs over j in ascending order, as in the unfolded form, with about half of the
+    // stage vectors live at once.
+    static_assert(GQ_TFOLD >= 1 && GQ_TFOLD < GQ_NSTAGES, "fold stage inside the step");
     real k[GQ_NSTAGES][GQ_D];
+    real Ps[GQ_NSTAGES][GQ_D];
     gq_rhs(y, px, py, C, S, G, k[0]);
     for (int step = 0; step < num_steps; ++step) {
         real yn[GQ_D];
-#pragma unroll
-        for (int i = 0; i < GQ_D; ++i) yn[i] = y[i];
-#pragma unroll
-        for (int s = 0; s < GQ_NSTAGES; ++s) {
-            if (s > 0) {
-                real yt[GQ_D];
-#pragma unroll
-                for (int i = 0; i < GQ_D; ++i) {
-                    real acc = R(0.0);
-#pragma unroll
-                    for (int j = 0; j < s; ++j)
-                        if (GQ_A[s][j] != 0.0 || GQ_A1[s][j] != 0.0)
-                            acc = fma((real)(GQ_A[s][j] + GQ_A1[s][j] * p0), k[j][i], acc);
-                    yt[i] = fma(dt, acc, y[i]);
-                }
-                if (GQ_ROW_OF[s] != GQ_ROW_OF[s - 1]) {
-                    gq_fixed_atoms(a, C, step, GQ_ROW_OF[s], TT);
-                    gq_stage(px, py, C, S, TT, G);
-                }
-                gq_rhs(yt, px, py, C, S, G, k[s]);
-            }
+        // y_new takes each k_s as soon as it exists
+        auto take_b = [&](const int s) {
             if (GQ_B[s] != 0.0) {
 #pragma unroll
                 for (int i = 0; i < GQ_D; ++i) yn[i] = fma(dt * (real)GQ_B[s], k[s][i], yn[i]);
             }
+        };
+        auto fold = [&](const int s) {
+#pragma unroll
+            for (int u = s + 1; u < GQ_NSTAGES; ++u)
+                if (nz(u, s)) {
+#pragma unroll
+                    for (int i = 0; i < GQ_D; ++i) Ps[u][i] = fma(coef(u, s), k[s][i], Ps[u][i]);
+                }
+        };
+        auto stage_values = [&](const int s) {
+            if (GQ_ROW_OF[s] != GQ_ROW_OF[s - 1]) {
+                gq_fixed_atoms(a, C, step, GQ_ROW_OF[s], TT);
+                gq_stage(px, py, C, S, TT, G);
+            }
+        };
+#pragma unroll
+        for (int i = 0; i < GQ_D; ++i) yn[i] = y[i];
+        take_b(0);
+        // stages 1 .. F-1 from the stored derivatives
+#pragma unroll
+        for (int s = 1; s < GQ_TFOLD; ++s) {
+            real yt[GQ_D];
+#pragma unroll
+            for (int i = 0; i < GQ_D; ++i) {
+                real acc = R(0.0);
+#pragma unroll
+                for (int j = 0; j < s; ++j)
+                    if (nz(s, j)) acc = fma(coef(s, j), k[j][i], acc);
+                yt[i] = fma(dt, acc, y[i]);
+            }
+            stage_values(s);
+            gq_rhs(yt, px, py, C, S, G, k[s]);
+            take_b(s);
+        }
+        // stage F from the stored derivatives, which are then folded
+        {
+            constexpr int s = GQ_TFOLD;
+            real yt[GQ_D];
+#pragma unroll
+            for (int i = 0; i < GQ_D; ++i) {
+                real acc = R(0.0);
+#pragma unroll
+                for (int j = 0; j < s; ++j)
+                    if (nz(s, j)) acc = fma(coef(s, j), k[j][i], acc);
+                yt[i] = fma(dt, acc, y[i]);
+#pragma unroll
+                for (int u = s + 1; u < GQ_NSTAGES; ++u) {
+                    real p = R(0.0);
+#pragma unroll
+                    for (int j = 0; j < s; ++j)
+                        if (nz(u, j)) p = fma(coef(u, j), k[j][i], p);
+                    Ps[u][i] = p;
+                }
+            }
+            stage_values(s);
+            gq_rhs(yt, px, py, C, S, G, k[s]);
+            fold(s);
+            take_b(s);
+        }
+        // stages F+1 .. GQ_NSTAGES-1 from the partial sums
+#pragma unroll
+        for (int s = GQ_TFOLD + 1; s < GQ_NSTAGES; ++s) {
+            real yt[GQ_D];
+#pragma unroll
+            for (int i = 0; i < GQ_D; ++i) yt[i] = fma(dt, Ps[s][i], y[i]);
+            stage_values(s);
+            gq_rhs(yt, px, py, C, S, G, k[s]);
+            fold(s);
+            take_b(s);
         }
 #pragma unroll
         for (int i = 0; i < GQ_D; ++i) y[i] = yn[i];

@@ -41,35 +41,90 @@ __device__ __forceinline__ void gq_tab_body(const gq_sweep_args& a, real (*s_y)[
 
 #if GQ_SCHEME == 0
     const double p0 = a.solver_param[0];
+    auto nz = [](const int s, const int j) { return GQ_A[s][j] != 0.0 || GQ_A1[s][j] != 0.0; };
+    auto coef = [&](const int s, const int j) { return (real)(GQ_A[s][j] + GQ_A1[s][j] * p0); };
+    // folded stage storage, as in the one-lane kernel (fixed_tableau.hip)
+    static_assert(GQ_TFOLD >= 1 && GQ_TFOLD < GQ_NSTAGES, "fold stage inside the step");
     real k[GQ_NSTAGES][P::NR];
+    real Ps[GQ_NSTAGES][P::NR];
     const real* Yst = eval(y, k[0]);
     for (int step = 0; step < num_steps; ++step) {
         real yn[P::NR];
-#pragma unroll
-        for (int r = 0; r < P::NR; ++r) yn[r] = y[r];
-#pragma unroll
-        for (int s = 0; s < GQ_NSTAGES; ++s) {
-            if (s > 0) {
-                real yt[P::NR];
-#pragma unroll
-                for (int r = 0; r < P::NR; ++r) {
-                    real acc = R(0.0);
-#pragma unroll
-                    for (int j = 0; j < s; ++j)
-                        if (GQ_A[s][j] != 0.0 || GQ_A1[s][j] != 0.0)
-                            acc = fma((real)(GQ_A[s][j] + GQ_A1[s][j] * p0), k[j][r], acc);
-                    yt[r] = fma(dt, acc, y[r]);
-                }
-                if (GQ_ROW_OF[s] != GQ_ROW_OF[s - 1]) {
-                    gq_fixed_atoms(a, C, step, GQ_ROW_OF[s], TT);
-                    P::stage(px, py, C, Sg, TT, Gg);
-                }
-                eval(yt, k[s]);
-            }
+        // y_new takes each k_s as soon as it exists
+        auto take_b = [&](const int s) {
             if (GQ_B[s] != 0.0) {
 #pragma unroll
                 for (int r = 0; r < P::NR; ++r) yn[r] = fma(dt * (real)GQ_B[s], k[s][r], yn[r]);
             }
+        };
+        auto fold = [&](const int s) {
+#pragma unroll
+            for (int u = s + 1; u < GQ_NSTAGES; ++u)
+                if (nz(u, s)) {
+#pragma unroll
+                    for (int r = 0; r < P::NR; ++r) Ps[u][r] = fma(coef(u, s), k[s][r], Ps[u][r]);
+                }
+        };
+        auto stage_values = [&](const int s) {
+            if (GQ_ROW_OF[s] != GQ_ROW_OF[s - 1]) {
+                gq_fixed_atoms(a, C, step, GQ_ROW_OF[s], TT);
+                P::stage(px, py, C, Sg, TT, Gg);
+            }
+        };
+#pragma unroll
+        for (int r = 0; r < P::NR; ++r) yn[r] = y[r];
+        take_b(0);
+        // stages 1 .. F-1 from the stored derivatives
+#pragma unroll
+        for (int s = 1; s < GQ_TFOLD; ++s) {
+            real yt[P::NR];
+#pragma unroll
+            for (int r = 0; r < P::NR; ++r) {
+                real acc = R(0.0);
+#pragma unroll
+                for (int j = 0; j < s; ++j)
+                    if (nz(s, j)) acc = fma(coef(s, j), k[j][r], acc);
+                yt[r] = fma(dt, acc, y[r]);
+            }
+            stage_values(s);
+            eval(yt, k[s]);
+            take_b(s);
+        }
+        // stage F from the stored derivatives, which are then folded
+        {
+            constexpr int s = GQ_TFOLD;
+            real yt[P::NR];
+#pragma unroll
+            for (int r = 0; r < P::NR; ++r) {
+                real acc = R(0.0);
+#pragma unroll
+                for (int j = 0; j < s; ++j)
+                    if (nz(s, j)) acc = fma(coef(s, j), k[j][r], acc);
+                yt[r] = fma(dt, acc, y[r]);
+#pragma unroll
+                for (int u = s + 1; u < GQ_NSTAGES; ++u) {
+                    real p = R(0.0);
+#pragma unroll
+                    for (int j = 0; j < s; ++j)
+                        if (nz(u, j)) p = fma(coef(u, j), k[j][r], p);
+                    Ps[u][r] = p;
+                }
+            }
+            stage_values(s);
+            eval(yt, k[s]);
+            fold(s);
+            take_b(s);
+        }
+        // stages F+1 .. GQ_NSTAGES-1 from the partial sums
+#pragma unroll
+        for (int s = GQ_TFOLD + 1; s < GQ_NSTAGES; ++s) {
+            real yt[P::NR];
+#pragma unroll
+            for (int r = 0; r < P::NR; ++r) yt[r] = fma(dt, Ps[s][r], y[r]);
+            stage_values(s);
+            eval(yt, k[s]);
+            fold(s);
+            take_b(s);
         }
 #pragma unroll
         for (int r = 0; r < P::NR; ++r) y[r] = yn[r];

@@ -2,6 +2,12 @@
 // (row 0 of the table is t = 0, then step s row r at t = s dt + c_r dt), computed in FP64 once
 // per launch when GQ_TIME_TABLE is 1, otherwise per lane from the same stage times.
 
+// Stage at which the Butcher-tableau kernels fold their stored stage derivatives into the
+// partial sums of the later stages: it balances the vectors live before the fold (y, y_new,
+// k_0 .. k_{F-1}, the stage value and its derivative) against those after it (y, y_new, the
+// partial sums of stages F+1 .. GQ_NSTAGES-1, the stage value and its derivative).
+#define GQ_TFOLD (GQ_NSTAGES / 2)
+
 __device__ __forceinline__ void gq_fixed_atoms(const gq_sweep_args& a, const real* __restrict__ C,
                                                int step, int row, real* __restrict__ TT) {
     // row < 0: t = 0 (the initial state)
