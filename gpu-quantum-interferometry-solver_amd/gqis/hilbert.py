@@ -426,11 +426,13 @@ def lds_bytes(n, fp64, block, jmax, split=None, valu=False, mono=0, sparse_k=0, 
     basis terms add their row coefficients (``hilbert_rk4.hip``).  ``sparse_k`` > 0:
     ``kernels/hilbert_sparse.hip`` with that ELL width (a state copy per wavefront)."""
     if sparse_k:
-        # state copies, the static diagonal coefficients, drive rows (up to 8 basis terms),
-        # jump lists and, for the table form, the ELL and jump-operator tables
+        # state copies (rows padded to a power of two for the swizzled layout), the static
+        # diagonal coefficients, drive rows (up to 8 basis terms), jump lists and, for the
+        # table form, the ELL and jump-operator tables
         r = 8 if fp64 else 4
-        return ((block // 64 + 1) * n * n * 2 * r + 8 * n * r + n * jmax * (r + 4)
-                + n * sparse_k * (4 + 2 * r) + mono * n * (4 + 2 * r))
+        npad = max(8, 1 << (n - 1).bit_length())
+        return ((block // 64) * n * npad * 2 * r + n * n * 2 * r + 8 * n * r
+                + n * jmax * (r + 4) + n * sparse_k * (4 + 2 * r) + mono * n * (4 + 2 * r))
     if fp64 and not valu:
         waves, pts = block // 64, (2 if n == 8 else 1)
         total = (n // 8) ** 2 * 4 * 64 * 8 + waves * (2 * pts * n * (n + 1) + pts * n + 64) * 8 \

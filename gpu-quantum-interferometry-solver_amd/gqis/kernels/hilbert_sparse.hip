@@ -25,17 +25,24 @@
 //
 // Layout: rho is Hermitian, so only the N (N + 1) / 2 elements i <= j are integrated: element u
 // of the upper triangle in row-major order is slot m = u / 64 of lane u % 64.  The wavefront
-// keeps the stage input in an LDS copy of the full rho (row-major, complex): the caller writes
-// each element and, below the diagonal, its conjugate; the RHS reads each element's neighbours
-// rho[k][j] and rho[i][k] from there (consecutive lanes: consecutive columns of one row).  The
-// static part of c_ij is a per-element LDS table, the drive part v_i - v_j comes from the drive
-// row table.
+// keeps the stage input in an LDS copy of the full rho (complex): the caller writes each element
+// and, below the diagonal, its conjugate; the RHS reads each element's neighbours rho[k][j] and
+// rho[i][k] from there.  Element (i, j) of the copy sits at i HNP + (j ^ i), HNP the power of
+// two >= N: the column is XOR-swizzled by the row, so the conjugate writes (lanes of one row
+// writing down a column) fall into distinct banks, where a plain row-major copy with a row of
+// 64 or 128 words put all 16 lanes of a write group into one bank; row reads stay a
+// permutation of consecutive words.  XOR neighbours remain one XOR away: (i ^ m, j) is at
+// addr ^ (m HNP + m), (i, j ^ m) at addr ^ m.  The static part of c_ij is a per-element LDS
+// table (row-major), the drive part v_i - v_j comes from the drive row table.
 
 #define HN GQ_HN
 #define HK GQ_HK                  // off-diagonal nonzeros per row (ELL width)
 #define HT (HN * (HN + 1) / 2)    // elements i <= j
 #define HE ((HT + 63) / 64)       // complex elements (slots) per lane
 #define HWAVES (GQ_BLOCK / 64)
+// row stride of the state copy: the power of two >= HN
+#define HNP (HN <= 8 ? 8 : HN <= 16 ? 16 : HN <= 32 ? 32 : HN <= 64 ? 64 : 128)
+static_assert(HN <= 128, "state copy rows up to 128 columns");
 #ifndef GQ_ADAPTIVE
 #define GQ_ADAPTIVE 0
 #endif
@@ -44,7 +51,7 @@ static_assert((HN & (HN - 1)) == 0, "the XOR form addresses a register of qubits
 #endif
 
 extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_sparse)(const gq_sweep_args a) {
-    __shared__ real2 s_rho[HWAVES][HN * HN];
+    __shared__ real2 s_rho[HWAVES][HN * HNP];
     __shared__ real2 s_cf[HN * HN];                 // static c_ij
     __shared__ real s_vrow[GQ_NBASIS * HN];         // drive row coefficients v_i per basis term
     __shared__ real s_jw[HN * GQ_JMAX];
@@ -162,8 +169,8 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_spar
     // diagonal)
     auto store = [&](int m, real vr, real vi) {
         if (used[m]) {
-            rho[ri[m] * HN + rj[m]] = real2{vr, vi};
-            if (ri[m] != rj[m]) rho[rj[m] * HN + ri[m]] = real2{vr, -vi};
+            rho[ri[m] * HNP + (rj[m] ^ ri[m])] = real2{vr, vi};
+            if (ri[m] != rj[m]) rho[rj[m] * HNP + (ri[m] ^ rj[m])] = real2{vr, -vi};
         }
     };
     // the stage input x + c k
@@ -183,7 +190,9 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_spar
             int b = ri[m] * HN + rj[m];
             asm volatile("" : "+v"(b));
             const int i = b / HN, j = b - (b / HN) * HN;
-            const real2 x = rho[b];
+            // the element's slot in the swizzled copy (with HNP = HN: b ^ i)
+            const int bs = (HN == HNP) ? (b ^ i) : i * HNP + (j ^ i);
+            const real2 x = rho[bs];
             const real2 cf = s_cf[b];
             real dvi = R(0.0), dvj = R(0.0);
 #pragma unroll
@@ -199,12 +208,12 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_spar
             // -i sum_k v_k rho[i ^ mask_k][j] and +i sum_k conj(v_k) rho[i][j ^ mask_k]
 #pragma unroll
             for (int k = 0; k < HK; ++k) {
-                // HN is a power of two: (i ^ mask) HN + j = b ^ (mask HN), i HN + (j ^ mask) =
-                // b ^ mask
-                const real2 p = rho[b ^ (gq_xmask[k] * HN)];
+                // HNP = HN: (i ^ mask) HN + (j ^ i ^ mask) = bs ^ (mask HN + mask) and
+                // i HN + (j ^ mask ^ i) = bs ^ mask
+                const real2 p = rho[bs ^ (gq_xmask[k] * HNP + gq_xmask[k])];
                 lr = fma(gq_xre[k], p.x, fma(-gq_xim[k], p.y, lr));
                 li = fma(gq_xre[k], p.y, fma(gq_xim[k], p.x, li));
-                const real2 u = rho[b ^ gq_xmask[k]];
+                const real2 u = rho[bs ^ gq_xmask[k]];
                 ur = fma(gq_xre[k], u.x, fma(gq_xim[k], u.y, ur));
                 ui = fma(gq_xre[k], u.y, fma(-gq_xim[k], u.x, ui));
             }
@@ -213,12 +222,12 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_spar
             for (int k = 0; k < HK; ++k) {
                 const int c = s_ecol[i * HK + k];
                 const real2 hv = s_eval[i * HK + k];
-                const real2 p = rho[c * HN + j];
+                const real2 p = rho[c * HNP + (j ^ c)];
                 lr = fma(hv.x, p.x, fma(-hv.y, p.y, lr));
                 li = fma(hv.x, p.y, fma(hv.y, p.x, li));
                 const int cj = s_ecol[j * HK + k];
                 const real2 hj = s_eval[j * HK + k];
-                const real2 u = rho[i * HN + cj];
+                const real2 u = rho[i * HNP + (cj ^ i)];
                 ur = fma(hj.x, u.x, fma(hj.y, u.y, ur));
                 ui = fma(hj.x, u.y, fma(-hj.y, u.x, ui));
             }
@@ -231,7 +240,8 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_spar
 #pragma unroll
             for (int kk = 0; kk < GQ_NMONO; ++kk) {
                 if ((i & gq_mmask[kk]) == gq_mcond[kk] && (j & gq_mmask[kk]) == gq_mcond[kk]) {
-                    const real2 p = rho[b ^ (gq_mmask[kk] * HN + gq_mmask[kk])];
+                    // (i ^ b, j ^ b) sits at (i ^ b) HN + (j ^ i) = bs ^ (b HN)
+                    const real2 p = rho[bs ^ (gq_mmask[kk] * HNP)];
                     dr = fma(gq_mw[kk], p.x, dr);
                     di = fma(gq_mw[kk], p.y, di);
                 }
@@ -245,7 +255,7 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_spar
                     const real2 l = s_ml[kk * HN + i], lj = s_ml[kk * HN + j];
                     const real wr = fma(l.x, lj.x, l.y * lj.y);
                     const real wi = fma(l.y, lj.x, -l.x * lj.y);
-                    const real2 p = rho[ci2 * HN + cj2];
+                    const real2 p = rho[ci2 * HNP + (cj2 ^ ci2)];
                     dr = fma(wr, p.x, fma(-wi, p.y, dr));
                     di = fma(wr, p.y, fma(wi, p.x, di));
                 }
@@ -256,7 +266,7 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_spar
                 // single-entry jumps into the population i: sum_b |l|^2 rho_bb
 #pragma unroll
                 for (int e = 0; e < GQ_JMAX; ++e)
-                    dr = fma(s_jw[i * GQ_JMAX + e], rho[s_jb[i * GQ_JMAX + e] * (HN + 1)].x, dr);
+                    dr = fma(s_jw[i * GQ_JMAX + e], rho[s_jb[i * GQ_JMAX + e] * HNP].x, dr);
             }
             kout[m] = dr;
             kout[m + HE] = di;
