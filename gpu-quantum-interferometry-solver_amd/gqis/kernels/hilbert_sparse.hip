@@ -50,7 +50,17 @@ static_assert(HN <= 128, "state copy rows up to 128 columns");
 static_assert((HN & (HN - 1)) == 0, "the XOR form addresses a register of qubits");
 #endif
 
-extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_hilbert_sparse)(const gq_sweep_args a) {
+// At least two wavefronts per SIMD: a wavefront is one point, and with one per SIMD the LDS
+// reads of the right-hand side go unhidden.  The bound caps the registers at 256; it binds for
+// the FP32 kernels with the largest stage storage at 32 levels (DOP853, AB5, Tsit5A, DOP853A:
+// 274-357 registers otherwise; Tsit5A on the 5-qubit chain 1.84 -> 1.19 s, DOP853 4.40 ->
+// 2.96 s on 128 x 256 points, with at most 136 B of spills).  FP64 copies of rho keep such
+// kernels at one work-group per compute unit anyway (LDS).
+#ifndef GQ_SPARSE_MIN_WAVES
+#define GQ_SPARSE_MIN_WAVES 2
+#endif
+extern "C" __global__ __launch_bounds__(GQ_BLOCK) __attribute__((amdgpu_waves_per_eu(GQ_SPARSE_MIN_WAVES)))
+void GQ_KERNEL(gq_hilbert_sparse)(const gq_sweep_args a) {
     __shared__ real2 s_rho[HWAVES][HN * HNP];
     __shared__ real2 s_cf[HN * HN];                 // static c_ij
     __shared__ real s_vrow[GQ_NBASIS * HN];         // drive row coefficients v_i per basis term
