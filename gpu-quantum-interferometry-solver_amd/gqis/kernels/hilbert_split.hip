@@ -109,7 +109,8 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) GQ_SPLIT_OCCUPANCY void GQ_KER
     __shared__ gq_half8 s_left[2][NOPS][64];    // [hi, lo][(P, block, kb)][lane]
 #endif
     __shared__ real s_diag[HWAVES][HN];         // populations, for the jump terms
-    __shared__ real s_t[HWAVES][2][HN * HLD];   // block transposes
+    __shared__ __align__(16) real s_t[HWAVES][2][HN * HLD];   // block transposes
+    static_assert((HN * HLD * 4) % 16 == 0, "16-byte aligned transpose buffers");
     __shared__ real s_vco[GQ_NBASIS * HN];      // drive row coefficients per basis term
 #if GQ_NMONO
     // general jump operators (one nonzero per row): a per-wavefront copy of the state and the
@@ -299,16 +300,24 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) GQ_SPLIT_OCCUPANCY void GQ_KER
 #pragma unroll
                     for (int r = 0; r < 4; ++r) bv[kb][cb][p][r] = x[gq_xi(p, gq_blk(kb, cb), r)];
 #if NB > 1
+        // block (0, 1) goes through LDS column-major with a column stride of GQ_TS = 20 words:
+        // each lane stores its four rows of one column as one 16-byte write (c * 20 mod 64
+        // are 16 distinct multiples of 4: no two lanes of a 16-lane group share a bank), and
+        // the transposed reads at (4 g + r) * 20 + c fall into 64 distinct banks (16 g + 20 r +
+        // c mod 64).  A row-major copy with rows of 33 words put up to four lanes on one bank
+        // on both sides.
+        {
+            constexpr int GQ_TS = 20;
+            const gq_f32x4 wr = {x[gq_xi(0, 1, 0)], x[gq_xi(0, 1, 1)], x[gq_xi(0, 1, 2)], x[gq_xi(0, 1, 3)]};
+            const gq_f32x4 wi = {x[gq_xi(1, 1, 0)], x[gq_xi(1, 1, 1)], x[gq_xi(1, 1, 2)], x[gq_xi(1, 1, 3)]};
+            *(gq_f32x4*)&t_r[c * GQ_TS + 4 * g] = wr;
+            *(gq_f32x4*)&t_i[c * GQ_TS + 4 * g] = wi;
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            t_r[(4 * g + r) * HLD + c] = x[gq_xi(0, 1, r)];
-            t_i[(4 * g + r) * HLD + c] = x[gq_xi(1, 1, r)];
-        }
-        asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            bv[1][0][0][r] = t_r[c * HLD + 4 * g + r];
-            bv[1][0][1][r] = -t_i[c * HLD + 4 * g + r];
+            for (int r = 0; r < 4; ++r) {
+                bv[1][0][0][r] = t_r[(4 * g + r) * GQ_TS + c];
+                bv[1][0][1][r] = -t_i[(4 * g + r) * GQ_TS + c];
+            }
         }
 #endif
 #if GQ_NMONO
