@@ -756,8 +756,11 @@ def bench_resonator(args):
         call(w, tl[:2], output_mode="final")                # symbolic model only
         prep = time.perf_counter() - t0
         out, info, _ = call(w, tl)                          # compile
+        state_before = gpu_state()
         out, info, _ = call(w, tl)
+        state_after = gpu_state()
         row = {"levels": 2 * nph, "states": (2 * nph) ** 2 - 1, "symbolic_s": prep,
+               "gpu_state_before_rk4": state_before, "gpu_state_after_rk4": state_after,
                "rk4_kernel_s": info["gpu_kernel_s"], "rk4_kernel": info.get("kernel"),
                "rk4_lanes_per_point": info.get("lanes_per_point"),
                "rk4_published_rtx3080_s": RESONATOR_1024_RK4_S[nph],
