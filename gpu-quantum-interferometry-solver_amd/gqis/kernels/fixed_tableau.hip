@@ -33,7 +33,11 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_fixed)(const
     const double p0 = a.solver_param[0];
     // a_sj of the tableau (anas5: plus its fitted part p0 * GQ_A1)
     auto nz = [](const int s, const int j) { return GQ_A[s][j] != 0.0 || GQ_A1[s][j] != 0.0; };
-    auto coef = [&](const int s, const int j) { return (real)(GQ_A[s][j] + GQ_A1[s][j] * p0); };
+    // a_sj + p0 * a1_sj only where the tableau has a fitted part (anas5): elsewhere the
+    // compile-time a_sj, which a run-time p0 * 0 would turn into a value held in a register
+    auto coef = [&](const int s, const int j) {
+        return GQ_A1[s][j] != 0.0 ? (real)(GQ_A[s][j] + GQ_A1[s][j] * p0) : (real)GQ_A[s][j];
+    };
     // Folded stage storage: k_0 .. k_{F-1} are kept until stage F is formed and then folded into
     // the partial sums Ps of the later stages; each later k_s is folded as soon as it exists.
     // Every sum runs over j in ascending order, as in the unfolded form, with about half of the

@@ -42,7 +42,11 @@ __device__ __forceinline__ void gq_tab_body(const gq_sweep_args& a, real (*s_y)[
 #if GQ_SCHEME == 0
     const double p0 = a.solver_param[0];
     auto nz = [](const int s, const int j) { return GQ_A[s][j] != 0.0 || GQ_A1[s][j] != 0.0; };
-    auto coef = [&](const int s, const int j) { return (real)(GQ_A[s][j] + GQ_A1[s][j] * p0); };
+    // a_sj + p0 * a1_sj only where the tableau has a fitted part (anas5): elsewhere the
+    // compile-time a_sj, which a run-time p0 * 0 would turn into a value held in a register
+    auto coef = [&](const int s, const int j) {
+        return GQ_A1[s][j] != 0.0 ? (real)(GQ_A[s][j] + GQ_A1[s][j] * p0) : (real)GQ_A[s][j];
+    };
     // folded stage storage, as in the one-lane kernel (fixed_tableau.hip)
     static_assert(GQ_TFOLD >= 1 && GQ_TFOLD < GQ_NSTAGES, "fold stage inside the step");
     real k[GQ_NSTAGES][P::NR];

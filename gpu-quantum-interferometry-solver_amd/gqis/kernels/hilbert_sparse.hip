@@ -310,11 +310,15 @@ void GQ_KERNEL(gq_hilbert_sparse)(const gq_sweep_args a) {
     const double rtol = a.solver_param[0], atol = a.solver_param[1];
     const int max_sub = (int)a.solver_param[2];
     const double inv_avg_time = 1.0 / ((double)(num_steps - a.warmup_steps) * dt_out);
-    bool cre[HE], cim[HE];
+    // components in the norm, as bits of one word per lane (bit m: the real part of slot m, bit
+    // HE + m: its imaginary part); an array of bools would take a 64-bit lane mask of scalar
+    // registers per entry
+    static_assert(2 * HE <= 32, "norm mask bits");
+    unsigned cmask = 0u;
 #pragma unroll
     for (int m = 0; m < HE; ++m) {
-        cre[m] = used[m] && (ri[m] < rj[m] || ri[m] < HN - 1);
-        cim[m] = used[m] && ri[m] < rj[m];
+        if (used[m] && (ri[m] < rj[m] || ri[m] < HN - 1)) cmask |= 1u << m;
+        if (used[m] && ri[m] < rj[m]) cmask |= 1u << (HE + m);
     }
     auto uniform = [](double v) {
         const long long b = __double_as_longlong(v);
@@ -386,7 +390,7 @@ void GQ_KERNEL(gq_hilbert_sparse)(const gq_sweep_args a) {
             double s5 = 0.0, s3 = 0.0;
 #pragma unroll
             for (int e = 0; e < 2 * HE; ++e) {
-                if (!(e < HE ? cre[e] : cim[e - HE])) continue;
+                if (!((cmask >> e) & 1u)) continue;
                 double e5 = 0.0, e3 = 0.0;
 #pragma unroll
                 for (int j = 0; j < GQ_NSTAGES; ++j) {
@@ -410,7 +414,7 @@ void GQ_KERNEL(gq_hilbert_sparse)(const gq_sweep_args a) {
             double ssum = 0.0;
 #pragma unroll
             for (int e = 0; e < 2 * HE; ++e) {
-                if (!(e < HE ? cre[e] : cim[e - HE])) continue;
+                if (!((cmask >> e) & 1u)) continue;
                 double d = GQ_E[GQ_NSTAGES] * (double)kf[e];
 #pragma unroll
                 for (int j = 0; j < GQ_NSTAGES; ++j) d = fma(GQ_E[j], (double)k[j][e], d);
@@ -446,6 +450,9 @@ void GQ_KERNEL(gq_hilbert_sparse)(const gq_sweep_args a) {
             ++n_rej;
             h = hs * factor;
         }
+        // the controller is the same in every lane: scalar registers hold it
+        h = uniform(h);
+        tau = uniform(tau);
         if (++sub > max_sub) {
             failed = true;
 #pragma unroll
@@ -520,39 +527,101 @@ void GQ_KERNEL(gq_hilbert_sparse)(const gq_sweep_args a) {
     (void)dt6;
     auto row_at = [&](int step, int row) { return row < 0 ? 0 : 1 + step * GQ_NROWS + row; };
 #if GQ_SCHEME == 0
-    // explicit Runge-Kutta from a Butcher tableau, k1 of a step from the end of the previous one
+    // explicit Runge-Kutta from a Butcher tableau, k1 of a step from the end of the previous
+    // one.  Folded stage storage (as in fixed_tableau.hip): k_0 .. k_{F-1} are kept until stage
+    // F is formed, then folded into partial sums Ps of the later stages; y_new takes each k_s
+    // as soon as it exists.  Every sum runs in the unfolded form's order, with about 8 instead
+    // of 14 state-sized arrays live for DOP853.
     const double p0 = a.solver_param[0];
+    auto nz = [](const int s, const int j) { return GQ_A[s][j] != 0.0 || GQ_A1[s][j] != 0.0; };
+    // a_sj + p0 * a1_sj only where the tableau has a fitted part (anas5): elsewhere the
+    // compile-time a_sj, which a run-time p0 * 0 would turn into a value held in a register
+    auto coef = [&](const int s, const int j) {
+        return GQ_A1[s][j] != 0.0 ? (real)(GQ_A[s][j] + GQ_A1[s][j] * p0) : (real)GQ_A[s][j];
+    };
+    static_assert(GQ_TFOLD >= 1 && GQ_TFOLD < GQ_NSTAGES, "fold stage inside the step");
     real k[GQ_NSTAGES][2 * HE];
+    real Ps[GQ_NSTAGES][2 * HE];
+    real yn[2 * HE];
+    auto take_b = [&](const int s) {
+        if (GQ_B[s] != 0.0) {
+#pragma unroll
+            for (int e = 0; e < 2 * HE; ++e) yn[e] = fma(dt * (real)GQ_B[s], k[s][e], yn[e]);
+        }
+    };
+    auto fold = [&](const int s) {
+#pragma unroll
+        for (int u = s + 1; u < GQ_NSTAGES; ++u)
+            if (nz(u, s)) {
+#pragma unroll
+                for (int e = 0; e < 2 * HE; ++e) Ps[u][e] = fma(coef(u, s), k[s][e], Ps[u][e]);
+            }
+    };
+    // the stage input from the stored derivatives k_0 .. k_{s-1}
+    auto store_raw = [&](const int s) {
+#pragma unroll
+        for (int m = 0; m < HE; ++m) {
+            real ar = R(0.0), ai = R(0.0);
+#pragma unroll
+            for (int j = 0; j < s; ++j) {
+                if (nz(s, j)) {
+                    ar = fma(coef(s, j), k[j][m], ar);
+                    ai = fma(coef(s, j), k[j][m + HE], ai);
+                }
+            }
+            store(m, fma(dt, ar, y[m]), fma(dt, ai, y[m + HE]));
+        }
+    };
     put(y, R(0.0), y);
     rhs(k[0]);
     for (int step = 0; step < num_steps; ++step) {
 #pragma unroll
-        for (int s = 1; s < GQ_NSTAGES; ++s) {
+        for (int e = 0; e < 2 * HE; ++e) yn[e] = y[e];
+        take_b(0);
+        // stages 1 .. F-1 from the stored derivatives
 #pragma unroll
-            for (int m = 0; m < HE; ++m) {
-                real ar = R(0.0), ai = R(0.0);
+        for (int s = 1; s < GQ_TFOLD; ++s) {
+            store_raw(s);
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            if (GQ_ROW_OF[s] != GQ_ROW_OF[s - 1]) stage_values(row_at(step, GQ_ROW_OF[s]));
+            rhs(k[s]);
+            take_b(s);
+        }
+        // stage F from the stored derivatives, which are then folded
+        {
+            constexpr int s = GQ_TFOLD;
+            store_raw(s);
 #pragma unroll
-                for (int j = 0; j < s; ++j) {
-                    if (GQ_A[s][j] != 0.0 || GQ_A1[s][j] != 0.0) {
-                        const real c = (real)(GQ_A[s][j] + GQ_A1[s][j] * p0);
-                        ar = fma(c, k[j][m], ar);
-                        ai = fma(c, k[j][m + HE], ai);
-                    }
+            for (int u = s + 1; u < GQ_NSTAGES; ++u) {
+#pragma unroll
+                for (int e = 0; e < 2 * HE; ++e) {
+                    real p = R(0.0);
+#pragma unroll
+                    for (int j = 0; j < s; ++j)
+                        if (nz(u, j)) p = fma(coef(u, j), k[j][e], p);
+                    Ps[u][e] = p;
                 }
-                store(m, fma(dt, ar, y[m]), fma(dt, ai, y[m + HE]));
             }
             asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
             if (GQ_ROW_OF[s] != GQ_ROW_OF[s - 1]) stage_values(row_at(step, GQ_ROW_OF[s]));
             rhs(k[s]);
+            fold(s);
+            take_b(s);
+        }
+        // stages F+1 .. GQ_NSTAGES-1 from the partial sums
+#pragma unroll
+        for (int s = GQ_TFOLD + 1; s < GQ_NSTAGES; ++s) {
+#pragma unroll
+            for (int m = 0; m < HE; ++m)
+                store(m, fma(dt, Ps[s][m], y[m]), fma(dt, Ps[s][m + HE], y[m + HE]));
+            asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+            if (GQ_ROW_OF[s] != GQ_ROW_OF[s - 1]) stage_values(row_at(step, GQ_ROW_OF[s]));
+            rhs(k[s]);
+            fold(s);
+            take_b(s);
         }
 #pragma unroll
-        for (int e = 0; e < 2 * HE; ++e) {
-            real yn = y[e];
-#pragma unroll
-            for (int s = 0; s < GQ_NSTAGES; ++s)
-                if (GQ_B[s] != 0.0) yn = fma(dt * (real)GQ_B[s], k[s][e], yn);
-            y[e] = yn;
-        }
+        for (int e = 0; e < 2 * HE; ++e) y[e] = yn[e];
         if (GQ_ROW_OF[GQ_NSTAGES - 1] != GQ_END_ROW) stage_values(row_at(step, GQ_END_ROW));
         if (step + 1 < num_steps) {
             put(y, R(0.0), y);
