@@ -264,7 +264,7 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_adaptive)(co
 #pragma unroll
                 for (int i = 0; i < GQ_D; ++i) {
                     const en_t sc = fma(rtol_e, fmax(fabs((en_t)y[i]), fabs((en_t)yn[i])), atol_e);
-                    const en_t e5 = esum[0][i] / sc, e3 = esum[1][i] / sc;
+                    const en_t e5 = gq_err_ratio(esum[0][i], sc), e3 = gq_err_ratio(esum[1][i], sc);
                     s5 = fma(e5, e5, s5);
                     s3 = fma(e3, e3, s3);
                 }
@@ -283,7 +283,8 @@ extern "C" __global__ __launch_bounds__(GQ_BLOCK) void GQ_KERNEL(gq_adaptive)(co
                 for (int i = 0; i < GQ_D; ++i) {
                     const en_t e = hse * fma((en_t)GQ_E[GQ_NS], (en_t)kf[i], esum[0][i]);
                     const en_t sc = fma(rtol_e, fmax(fabs((en_t)y[i]), fabs((en_t)yn[i])), atol_e);
-                    ssum = fma(e / sc, e / sc, ssum);
+                    const en_t er = gq_err_ratio(e, sc);
+                    ssum = fma(er, er, ssum);
                 }
                 err_norm = sqrt((double)ssum / (double)GQ_D);
             }

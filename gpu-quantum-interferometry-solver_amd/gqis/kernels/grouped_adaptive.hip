@@ -289,7 +289,7 @@ __device__ __forceinline__ void gq_adaptive_body(const gq_sweep_args& a,
 #pragma unroll
             for (int r = 0; r < P::NR; ++r) {
                 const en_t sc = fma(rtol_e, fmax(fabs((en_t)y[r]), fabs((en_t)yn[r])), atol_e);
-                const en_t e5 = esum[0][r] / sc, e3 = esum[1][r] / sc;
+                const en_t e5 = gq_err_ratio(esum[0][r], sc), e3 = gq_err_ratio(esum[1][r], sc);
                 s5 = fma(e5, e5, s5);
                 s3 = fma(e3, e3, s3);
             }
@@ -308,7 +308,8 @@ __device__ __forceinline__ void gq_adaptive_body(const gq_sweep_args& a,
             for (int r = 0; r < P::NR; ++r) {
                 const en_t e = hse * fma((en_t)GQ_E[GQ_NS], (en_t)kf[r], esum[0][r]);
                 const en_t sc = fma(rtol_e, fmax(fabs((en_t)y[r]), fabs((en_t)yn[r])), atol_e);
-                ssum = fma(e / sc, e / sc, ssum);
+                const en_t er = gq_err_ratio(e, sc);
+                ssum = fma(er, er, ssum);
             }
             const double ssg = gq_group_sum(s_red, (double)ssum, GI, lane);
             err_norm = sqrt(ssg / (double)GQ_D);

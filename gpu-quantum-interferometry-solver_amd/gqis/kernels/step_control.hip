@@ -43,6 +43,12 @@ __device__ __forceinline__ constexpr double gq_err_w(int, int j) { return GQ_E[j
 #define GQ_FOLD ((GQ_NSTAGES + 2 + GQ_NERR) / 2)
 static_assert(GQ_FOLD >= 1 && GQ_FOLD < GQ_NSTAGES, "fold stage inside the step");
 
+// e / sc in the error norms: an IEEE division in FP64; in FP32 (the error sums of FP32
+// kernels) e times the hardware reciprocal of sc (within 1 ulp), about a fifth of the
+// instructions of an IEEE FP32 division -- the norm only steers the step size
+__device__ __forceinline__ double gq_err_ratio(double e, double sc) { return e / sc; }
+__device__ __forceinline__ float gq_err_ratio(float e, float sc) { return e * __builtin_amdgcn_rcpf(sc); }
+
 __device__ __forceinline__ double gq_step_factor(double err) {
     const double x = fmin(fmax(err, GQ_ERR_LO), GQ_ERR_HI);
     const float l2 = __builtin_amdgcn_logf((float)x);                   // log2
