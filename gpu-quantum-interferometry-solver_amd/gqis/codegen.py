@@ -95,7 +95,7 @@ class HipPrinter(C99CodePrinter):
             if 0 < n <= 8:
                 return "(" + "*".join([b] * n) + ")"
             if -8 <= n < 0:
-                return f"({self._lit}(1.0)/(" + "*".join([b] * (-n)) + "))"
+                return "gq_rcp(" + "*".join([b] * (-n)) + ")"
         if exp == sp.Rational(1, 2):
             return f"sqrt({self._print(base)})"
         if exp == -sp.Rational(1, 2):
@@ -581,7 +581,18 @@ def prelude(fp64, acc64, block_size, defines):
            "__device__ __forceinline__ double gq_reduce_2pi(double x) {",
            "    const double k = rint(x * 0.15915494309189535);",
            "    return fma(-k, 2.4492935982947064e-16, fma(-k, 6.283185307179586, x));",
-           "}"] + _sincos64_lines() + _sincos32_lines()
+           "}"] + _sincos64_lines() + _sincos32_lines() + [
+           "// 1 / x for the model code's negative powers: an IEEE division in FP64; in FP32 the",
+           "// hardware reciprocal (within 1 ulp of 1 / x, about a fifth of the instructions of an",
+           "// IEEE FP32 division; the FP32 right-hand side carries 6e-8 relative rounding anyway)",
+           "__device__ __forceinline__ double gq_rcp(double x) { return 1.0 / x; }",
+           "__device__ __forceinline__ float gq_rcp(float x) {",
+           "#if defined(__HIP_DEVICE_COMPILE__)",
+           "    return __builtin_amdgcn_rcpf(x);",
+           "#else",
+           "    return 1.0f / x;   // host builds of the generated code (CPU checks)",
+           "#endif",
+           "}"]
     for k, v in defines.items():
         out.append(f"#define {k} {v}")
     out += ["#ifndef GQ_HILBERT", "#define GQ_HILBERT 0", "#endif"]

@@ -31,6 +31,7 @@ inline real gq_sign(real x) { return (x > R(0)) - (x < R(0)); }
 inline double gq_reduce_2pi(double x) { return x; }
 inline double gq_cos64(double x) { return std::cos(x); }
 inline double gq_sin64(double x) { return std::sin(x); }
+inline double gq_rcp(double x) { return 1.0 / x; }
 namespace lane {
 %(lane)s
 }
