@@ -186,3 +186,32 @@ def test_fp32_sincos_is_within_two_fp32_ulp_of_the_exact_values(tmp_path):
     # FP32 sin of the FP32-rounded argument, which the reduction in FP64 avoids
     naive = np.sin(x.astype(np.float32)).astype(np.float64)
     assert np.max(np.abs(naive - ws)) > 1e-3
+
+
+def test_negative_powers_print_as_gq_rcp_and_match_the_division_on_the_host(tmp_path):
+    """A standalone negative integer power in model code (the form common subexpressions such as
+    1 / sqrt(d^2 + D^2) take) goes through gq_rcp, whose FP32 device overload is the hardware
+    reciprocal and whose FP64 one an IEEE division; a quotient inside a product stays a
+    division.  Compiled for the CPU from the prelude, the FP64 overload equals 1.0 / x bit for
+    bit, so FP64 kernels keep their results."""
+    x, y = sp.symbols("x y", real=True)
+    pr = codegen.HipPrinter({"x": "x", "y": "y"})
+    assert pr.doprint(1 / x) == "gq_rcp(x)"
+    assert pr.doprint(x**-3) == "gq_rcp(x*x*x)"
+    assert pr.doprint(y / x**3) == "y/(x*x*x)"
+    src = tmp_path / "rcp.cpp"
+    pre = codegen.prelude(True, True, 256, {})
+    body = pre[pre.index("__device__ __forceinline__ double gq_rcp"):]
+    body = body[:body.index("#ifndef GQ_HILBERT")] if "#ifndef GQ_HILBERT" in body else body
+    src.write_text("#define __device__\n#define __forceinline__ inline\n" + body.split("#define ")[0]
+                   + '\nextern "C" void rcp(const double* x, int n, double* out) {'
+                   " for (int j = 0; j < n; ++j) out[j] = gq_rcp(x[j]); }\n")
+    lib = tmp_path / "rcp.so"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-ffp-contract=off", "-shared", "-fPIC", "-w",
+                    str(src), "-o", str(lib)], check=True)
+    fn = ctypes.CDLL(str(lib)).rcp
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    v = np.concatenate([np.random.default_rng(13).uniform(-1e3, 1e3, 4000), [1e-300, -3.0, 7.0]])
+    out = np.zeros_like(v)
+    fn(v.ctypes.data, len(v), out.ctypes.data)
+    assert np.array_equal(out, 1.0 / v)
