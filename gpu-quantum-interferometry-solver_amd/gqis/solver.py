@@ -760,8 +760,10 @@ def auto_lanes_per_point(spec, d, fp64, nnz, points, natoms=0):
       give a compute unit at least 8 wavefronts (work-groups per CU are limited by their LDS
       copies of the state): 4 lanes at 143 FP32 states, 8 from 195 to 399 (at 195, 4 lanes
       fit one work-group per CU and take 3.74 s against 2.81 s);
-    * sparse rows, adaptive solvers: one lane up to 64 words on such a sweep (0.55 s against
-      0.85 s with 8 lanes for Tsit5A at 63 FP32 states), else as the fixed-step tableaux;
+    * sparse rows, adaptive solvers: one lane up to 64 words on such a sweep (Tsit5A on
+      1024 x 1024 points, 63 FP32 states: 0.347 s against 0.490-0.664 s with 2, 4 or 8 lanes
+      on the 8-level resonator, 0.344 s against 0.389-0.568 s on the 3-qubit chain), else as
+      the fixed-step tableaux;
     * sparse rows, other solvers (more stage vectors): 8 lanes from 48 states, 16 where 8 would
       leave a wavefront more than 36 words of a vector or, fixed-step, more than 160 words of
       stage vectors (the 4-qubit chain, 255 FP32 states: DOP853 6.7 against 35.4 ns per
