@@ -1,6 +1,7 @@
 """The reference's qubit-resonator benchmark model (drive inside a square root and a reciprocal:
 per-point stage values at every new stage time) on the GPU against the FP64 oracle: RK4 in FP64
-and FP32, adaptive Tsit5 with its accepted / rejected counts, two and three Fock states."""
+and FP32, adaptive Tsit5 with its accepted / rejected counts in FP64 and, in FP32, with the
+integration error's size, two and three Fock states."""
 import numpy as np
 import pytest
 
@@ -78,3 +79,22 @@ def test_single_lds_copy_matches_oracle(gpu, solver):
               lambda *a, **k: integrate.fixed_step("dp5", *a, **k))
         ref, _ = fn(m.f, m.ground_state(), dt, steps, obs=m.obs)
     assert _rel(got, ref) <= 1e-10
+
+
+def test_tsit5a_fp32_matches_oracle(gpu):
+    # the FP32 kernel (FP32 right-hand side with the reciprocal of E, stage-time cos by FP32
+    # polynomials, FP32 error sums) against the FP64 oracle at the same tolerance: its own
+    # step sequence, so the check is the integration error's size (1e-4 of the map's scale for
+    # rtol = atol = 1e-6) and the mean accepted steps (within 5% of the oracle's)
+    w, m, args = _case(2, periods=2)
+    tl = np.linspace(0.0, float(w["tlist"][-1]), 9)
+    opts = {"step_rtol": 1e-6, "step_atol": 1e-6, "max_substeps": 1000, "count_steps": True}
+    va = {k: v.astype(np.float32) for k, v in w["var_arrays"].items()}
+    got, info = mesolve_2D(*args[:4], tl, var_arrays=va, solver="tsit5a", adaptive_options=opts,
+                           return_timing_info=True)
+    ref, _, acc, rej = integrate.tsit5a(m.f, m.ground_state(), float(tl[1]), 8, rtol=1e-6,
+                                        atol=1e-6, max_substeps=1000, obs=m.obs, mode="mean")
+    assert info["kernel"].endswith("f32")
+    assert _rel(got, ref) <= 1e-4
+    mean_acc = float(np.mean(info["adaptive_steps"]["accepted"]))
+    assert abs(mean_acc - float(np.mean(acc))) <= 0.05 * float(np.mean(acc))
